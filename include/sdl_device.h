@@ -1,0 +1,91 @@
+/* Device-resident extension of the sparsediff_linalg drop-in (no reference
+ * counterpart).  The reference's fills read and write host arrays; callers
+ * that keep operands in HBM (a Newton solver on the GPU, the batched refill
+ * benchmark) use these entry points instead.  All pointers named d* are device
+ * pointers; stream arguments are hipStream_t (NULL = the library stream).
+ * Functions return 0 on success and nonzero on error (message in
+ * sdl_last_error()).  Refills allocate nothing. */
+#ifndef SDL_DEVICE_H
+#define SDL_DEVICE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- runtime ---- */
+void sdl_init(void);
+const char *sdl_last_error(void);
+void *sdl_library_stream(void);
+void sdl_synchronize(void);
+long long sdl_device_alloc_count(void);
+long long sdl_device_free_count(void);
+/* Device memory and timing helpers for callers without a HIP toolchain
+   (events are recorded on the given stream, NULL = library stream). */
+#include <stddef.h>
+void *sdl_dev_alloc(size_t bytes);
+void sdl_dev_free(void *p);
+int sdl_memcpy_htod(void *dst, const void *src, size_t bytes);
+int sdl_memcpy_dtoh(void *dst, const void *src, size_t bytes);
+int sdl_memset_d(void *dst, int value, size_t bytes);
+void *sdl_event_create(void);
+void sdl_event_record(void *ev, void *stream);
+float sdl_event_elapsed_ms(void *start, void *stop);
+void sdl_event_destroy(void *ev);
+int sdl_device_name(char *buf, int len);
+
+/* ---- C = B^T diag(d) A on device-resident CSR operands (the hot path of
+   BTA_matrices_alloc / BTDA_matrices_fill_values for two sparse operands,
+   reference src/matmul_dispatchers.c:335-375) ---- */
+typedef struct sdl_btda_plan sdl_btda_plan;
+
+enum
+{
+    SDL_SYM_EXACT = 0,     /* count pass + fill pass */
+    SDL_SYM_PREDICTED = 1  /* one pass sized by the row-nnz predictor, exact fallback */
+};
+
+/* A: m x nA CSR (dAp[m+1], dAi[nnzA]); B: m x nB CSR.  The plan copies the
+   structure it needs; the caller's arrays may be freed afterwards. */
+int sdl_btda_plan_create(int m, int nA, int nB, const int *dAp, const int *dAi,
+                         const int *dBp, const int *dBi, int mode, sdl_btda_plan **out);
+void sdl_btda_plan_destroy(sdl_btda_plan *plan);
+long long sdl_btda_plan_nnz(const sdl_btda_plan *plan);
+long long sdl_btda_plan_terms(const sdl_btda_plan *plan);
+int sdl_btda_plan_is_pull(const sdl_btda_plan *plan);
+/* Device row pointer (int64, nB+1 entries) and column indices of C. */
+const long long *sdl_btda_plan_rowptr(const sdl_btda_plan *plan);
+const int *sdl_btda_plan_colind(const sdl_btda_plan *plan);
+/* Copy C's structure to host int32 arrays (fails if nnz > INT_MAX). */
+int sdl_btda_plan_structure(const sdl_btda_plan *plan, int *hp, int *hi);
+/* Symbolic-pass statistics: rows, terms, nnz, hash/bitmap/global rows,
+   overflow and predicted rows (8 values). */
+int sdl_btda_plan_stats(const sdl_btda_plan *plan, long long *out8);
+
+/* One refill: dCx[nnz] = B^T diag(dd) A with dAx/dBx the CSR values of A and
+   B (dd may be NULL = identity). */
+int sdl_btda_refill(const sdl_btda_plan *plan, const double *dAx, const double *dBx,
+                    const double *dd, double *dCx, void *stream);
+/* K refills sharing A and B: refill r reads dD + r*ldd (length m) and writes
+   dC + r*ldc (length nnz).  One launch. */
+int sdl_btda_refill_batched(const sdl_btda_plan *plan, const double *dAx,
+                            const double *dBx, const double *dD, long long ldd, int K,
+                            double *dC, long long ldc, void *stream);
+
+/* ---- synthetic workload generators (host; deterministic in seed) ---- */
+/* Row lengths uniform in [lo, hi]; writes Ap[m+1]; returns nnz. */
+long long sdl_gen_row_lengths_uniform(uint64_t seed, int m, int lo, int hi, int *Ap);
+/* Columns: distinct uniform in [0, n), sorted per row; values N(0,1).  Ax may
+   be NULL. */
+void sdl_gen_fill_rows(uint64_t seed, int m, int n, const int *Ap, int *Ai, double *Ax);
+/* Uniform doubles in [lo, hi). */
+void sdl_gen_uniform(uint64_t seed, long long count, double lo, double hi, double *out);
+/* Standard normal doubles (Box-Muller). */
+void sdl_gen_normal(uint64_t seed, long long count, double *out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,146 @@
+// Device-resident C ABI (include/sdl_device.h): B^T diag(d) A plans on HBM
+// operands, single and batched refills.
+#include "../../include/sdl_device.h"
+#include "sparse_ops.h"
+#include "spgemm.h"
+
+#include <climits>
+#include <new>
+
+struct sdl_btda_plan
+{
+    int m = 0, nA = 0, nB = 0;
+    sdl::DBuf<int> Ap, Ai;            // A's structure (copied)
+    sdl::DBuf<int> Lp, Li, Lperm;     // B's columns: CSC structure + CSR positions
+    sdl::ProductPlan P;
+};
+
+namespace {
+hipStream_t as_stream(void *s) { return s ? (hipStream_t) s : sdl::stream(); }
+
+int nnz_of(const int *dp, int m)
+{
+    int v = 0;
+    sdl::d2h(&v, dp + m, sizeof(int));
+    return v;
+}
+} // namespace
+
+extern "C" {
+#pragma GCC visibility push(default)
+
+int sdl_btda_plan_create(int m, int nA, int nB, const int *dAp, const int *dAi,
+                         const int *dBp, const int *dBi, int mode, sdl_btda_plan **out)
+{
+    if (!out || m < 0 || nA < 0 || nB < 0 || !dAp || !dBp)
+    {
+        sdl::set_error("sdl_btda_plan_create: invalid arguments");
+        return 1;
+    }
+    sdl_btda_plan *pl = new (std::nothrow) sdl_btda_plan();
+    if (!pl)
+    {
+        sdl::set_error("sdl_btda_plan_create: out of host memory");
+        return 1;
+    }
+    pl->m = m;
+    pl->nA = nA;
+    pl->nB = nB;
+    int nnzA = nnz_of(dAp, m), nnzB = nnz_of(dBp, m);
+    pl->Ap.alloc((size_t) m + 1);
+    pl->Ai.alloc(nnzA);
+    sdl::d2d(pl->Ap.p, dAp, ((size_t) m + 1) * sizeof(int));
+    sdl::d2d(pl->Ai.p, dAi, (size_t) nnzA * sizeof(int));
+    pl->Lp.alloc((size_t) nB + 1);
+    pl->Li.alloc(nnzB);
+    pl->Lperm.alloc(nnzB);
+    sdl::transpose_structure(m, nB, nnzB, dBp, dBi, pl->Lp.p, pl->Li.p, pl->Lperm.p);
+    sdl::LView L;
+    L.Lm = nB;
+    L.Lp = pl->Lp.p;
+    L.Li = pl->Li.p;
+    L.Lperm = pl->Lperm.p;
+    L.blocks = 1;
+    sdl::AView A;
+    A.Am = m;
+    A.An = nA;
+    A.Ap = pl->Ap.p;
+    A.Ai = pl->Ai.p;
+    sdl::product_plan(pl->P, L, A, mode == SDL_SYM_PREDICTED ? sdl::SymMode::Predicted
+                                                             : sdl::SymMode::Exact,
+                      nullptr);
+    *out = pl;
+    return 0;
+}
+
+void sdl_btda_plan_destroy(sdl_btda_plan *plan)
+{
+    if (!plan) return;
+    sdl::sync();
+    delete plan;
+}
+
+long long sdl_btda_plan_nnz(const sdl_btda_plan *plan) { return plan ? plan->P.nnz : -1; }
+long long sdl_btda_plan_terms(const sdl_btda_plan *plan) { return plan ? plan->P.terms : -1; }
+int sdl_btda_plan_is_pull(const sdl_btda_plan *plan) { return plan ? (int) plan->P.pull : 0; }
+
+const long long *sdl_btda_plan_rowptr(const sdl_btda_plan *plan)
+{
+    return plan ? (const long long *) plan->P.Cp.p : nullptr;
+}
+
+const int *sdl_btda_plan_colind(const sdl_btda_plan *plan) { return plan ? plan->P.Ci.p : nullptr; }
+
+int sdl_btda_plan_structure(const sdl_btda_plan *plan, int *hp, int *hi)
+{
+    if (!plan) return 1;
+    if (plan->P.nnz > INT_MAX)
+    {
+        sdl::set_error("sdl_btda_plan_structure: nnz %lld exceeds INT_MAX", (long long) plan->P.nnz);
+        return 1;
+    }
+    sdl::DBuf<int> p32(plan->P.m_out + 1);
+    sdl::narrow_i64(plan->P.m_out + 1, plan->P.Cp.p, p32.p, sdl::stream());
+    sdl::d2h(hp, p32.p, (size_t) (plan->P.m_out + 1) * sizeof(int));
+    sdl::d2h(hi, plan->P.Ci.p, (size_t) plan->P.nnz * sizeof(int));
+    return 0;
+}
+
+int sdl_btda_plan_stats(const sdl_btda_plan *plan, long long *o)
+{
+    if (!plan || !o) return 1;
+    const sdl::SymStats &s = plan->P.stats;
+    o[0] = s.rows;
+    o[1] = s.terms;
+    o[2] = s.nnz;
+    o[3] = s.hash_rows;
+    o[4] = s.bitmap_rows;
+    o[5] = s.global_rows;
+    o[6] = s.overflow_rows;
+    o[7] = s.predicted_rows;
+    return 0;
+}
+
+int sdl_btda_refill(const sdl_btda_plan *plan, const double *dAx, const double *dBx,
+                    const double *dd, double *dCx, void *stream)
+{
+    if (!plan) return 1;
+    sdl::product_numeric(plan->P, dBx, dd, dAx, dCx, as_stream(stream));
+    return 0;
+}
+
+int sdl_btda_refill_batched(const sdl_btda_plan *plan, const double *dAx, const double *dBx,
+                            const double *dD, long long ldd, int K, double *dC, long long ldc,
+                            void *stream)
+{
+    if (!plan || K < 0 || ldd < plan->m || ldc < plan->P.nnz)
+    {
+        sdl::set_error("sdl_btda_refill_batched: invalid arguments");
+        return 1;
+    }
+    sdl::product_numeric_batched(plan->P, dBx, dD, ldd, K, dAx, dC, ldc, as_stream(stream));
+    return 0;
+}
+
+#pragma GCC visibility pop
+} // extern "C"

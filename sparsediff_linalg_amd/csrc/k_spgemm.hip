@@ -1,0 +1,867 @@
+// Gustavson SpGEMM on CDNA4: C = kron(I_blocks, L) . diag(d) . A.
+//
+// Symbolic (alloc): per-row term bound -> rows binned by size class on the
+// device -> one workgroup per row with an LDS hash table (64..8192 slots), an
+// LDS bitmap (dense rows; columns come out sorted for free) or a global bitmap
+// (huge rows) -> count pass -> exclusive scan -> fill pass with the same
+// accumulators sized from the exact count -> bitonic sort of each hash table.
+//
+// Refill schedule: the reference's plan policy (terms <= 3 nnz(C)) picks a
+// pull schedule, where every output entry owns the list of its terms in
+// ascending k; otherwise a row accumulator walks each row's k in order.
+// Either way every entry is the left fold
+//     C[i,j] = (((0 + (L[i,k1] d[k1]) A[k1,j]) + (L[i,k2] d[k2]) A[k2,j]) + ...)
+// in ascending k with individually rounded products and sums (the library is
+// built with -ffp-contract=off), which is gustavson.c:127-135 / :220-228 bit
+// for bit.
+#include "spgemm.h"
+
+#include <algorithm>
+#include <vector>
+
+namespace sdl {
+
+namespace {
+
+constexpr int NCLASS = 11;      // 0 empty, 1..8 hash 64..8192, 9 LDS bitmap, 10 global bitmap
+constexpr int CLS_BITMAP = 9;
+constexpr int CLS_GLOBAL = 10;
+constexpr int HASH_MAX = 8192;
+
+int g_lds_bitmap_words = -1;    // largest bitmap held in LDS by one workgroup
+
+int lds_bitmap_words()
+{
+    if (g_lds_bitmap_words < 0)
+    {
+        int dev = 0;
+        SDL_HIP(hipGetDevice(&dev));
+        hipDeviceProp_t prop;
+        SDL_HIP(hipGetDeviceProperties(&prop, dev));
+        size_t lds = prop.sharedMemPerBlock;
+        if (lds > 160 * 1024) lds = 160 * 1024;
+        // keep 8 KB for the kernel's static arrays
+        g_lds_bitmap_words = (int) ((lds - 8192) / 4);
+    }
+    return g_lds_bitmap_words;
+}
+
+// ------------------------------------------------------------- helpers ----
+template <int G> __device__ __forceinline__ int block_excl_scan_int(int x, int *total, int *sh_w)
+{
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    int inc = x;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1)
+    {
+        int y = __shfl_up(inc, off, 64);
+        if (lane >= off) inc += y;
+    }
+    if (G == 64)
+    {
+        *total = __shfl(inc, 63, 64);
+        return inc - x;
+    }
+    if (lane == 63) sh_w[w] = inc;
+    __syncthreads();
+    int pre = 0, tot = 0;
+#pragma unroll
+    for (int i = 0; i < G / 64; i++)
+    {
+        int s = sh_w[i];
+        if (i < w) pre += s;
+        tot += s;
+    }
+    __syncthreads();
+    *total = tot;
+    return pre + inc - x;
+}
+
+template <int G> __device__ __forceinline__ long long block_sum_ll(long long x, long long *sh)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+    if (G == 64) return x;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) sh[w] = x;
+    __syncthreads();
+    long long t = 0;
+#pragma unroll
+    for (int i = 0; i < G / 64; i++) t += sh[i];
+    __syncthreads();
+    return t;
+}
+
+// Shared state of the load-balanced term expansion of one output row: a chunk
+// of up to G entries of L's row with their A-row start and exclusive offsets.
+template <int G> struct Expand
+{
+    int start[G];
+    int k[G];
+    int q[G];
+    int off[G + 1];
+    int wsum[G / 64];
+};
+
+// Calls f(t, q, k) for every term of output row (b, r) -- t indexes A's
+// entries, q indexes L's entries, k is the A row.  All G threads must call it.
+template <int G, typename F>
+__device__ __forceinline__ void for_each_term(Expand<G> &ex, int q0, int q1, int koff,
+                                              const int *__restrict__ Li,
+                                              const int *__restrict__ Ap, F &&f)
+{
+    for (int qb = q0; qb < q1; qb += G)
+    {
+        int q = qb + (int) threadIdx.x;
+        int len = 0, st = 0, k = 0;
+        if (q < q1)
+        {
+            k = Li[q] + koff;
+            st = Ap[k];
+            len = Ap[k + 1] - st;
+        }
+        int tot;
+        int ex_off = block_excl_scan_int<G>(len, &tot, ex.wsum);
+        ex.start[threadIdx.x] = st;
+        ex.k[threadIdx.x] = k;
+        ex.q[threadIdx.x] = q;
+        ex.off[threadIdx.x] = ex_off;
+        if (threadIdx.x == 0) ex.off[G] = tot;
+        __syncthreads();
+        for (int fi = (int) threadIdx.x; fi < tot; fi += G)
+        {
+            // last slot s with off[s] <= fi (slots with len 0 share offsets)
+            int lo = 0, hi = G - 1;
+            while (lo < hi)
+            {
+                int mid = (lo + hi + 1) >> 1;
+                if (ex.off[mid] <= fi) lo = mid;
+                else hi = mid - 1;
+            }
+            f(ex.start[lo] + (fi - ex.off[lo]), ex.q[lo], ex.k[lo]);
+        }
+        __syncthreads();
+    }
+}
+
+template <int S> __device__ __forceinline__ unsigned hash_slot(int col)
+{
+    constexpr int LOG2 = __builtin_ctz(S);
+    return ((unsigned) col * 2654435761u) >> (32 - LOG2);
+}
+
+// Insert col; returns false if the probe sequence ran over the whole table.
+template <int S> __device__ __forceinline__ bool hash_insert(int *tab, int col)
+{
+    unsigned h = hash_slot<S>(col);
+    for (int probe = 0; probe < S; probe++)
+    {
+        int prev = atomicCAS(&tab[h], -1, col);
+        if (prev == -1 || prev == col) return true;
+        h = (h + 1) & (S - 1);
+    }
+    return false;
+}
+
+// Ascending bitonic sort of S unsigned keys in LDS by G threads.
+template <int G, int S> __device__ __forceinline__ void bitonic_sort(unsigned *a)
+{
+    for (int k = 2; k <= S; k <<= 1)
+    {
+        for (int j = k >> 1; j > 0; j >>= 1)
+        {
+            for (int p = (int) threadIdx.x; p < S / 2; p += G)
+            {
+                int i = (p / j) * 2 * j + (p % j);
+                int ixj = i + j;
+                unsigned x = a[i], y = a[ixj];
+                bool up = (i & k) == 0;
+                if ((x > y) == up)
+                {
+                    a[i] = y;
+                    a[ixj] = x;
+                }
+            }
+            __syncthreads();
+        }
+    }
+}
+
+__device__ __forceinline__ int lower_bound_int(const int *a, int n, int x)
+{
+    int lo = 0, hi = n;
+    while (lo < hi)
+    {
+        int mid = (lo + hi) >> 1;
+        if (a[mid] < x) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+struct RowCtx
+{
+    int q0, q1, koff;
+};
+
+__device__ __forceinline__ RowCtx row_ctx(int row, int Lm, const int *Lp, int nL)
+{
+    int b = row / Lm, r = row - b * Lm;
+    RowCtx c;
+    c.q0 = Lp[r];
+    c.q1 = Lp[r + 1];
+    c.koff = b * nL;
+    return c;
+}
+
+// ------------------------------------------------------------ kernels ----
+__global__ void k_row_terms(int64_t m_out, int Lm, const int *__restrict__ Lp,
+                            const int *__restrict__ Li, int nL, const int *__restrict__ Ap,
+                            int64_t *__restrict__ terms)
+{
+    int64_t row = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (row >= m_out) return;
+    RowCtx c = row_ctx((int) row, Lm, Lp, nL);
+    int64_t t = 0;
+    for (int q = c.q0; q < c.q1; q++)
+    {
+        int k = Li[q] + c.koff;
+        t += Ap[k + 1] - Ap[k];
+    }
+    terms[row] = t;
+}
+
+__device__ __forceinline__ int classify(int64_t cap, int nwords, int lds_words)
+{
+    if (cap <= 0) return 0;
+    bool bm_ok = nwords <= lds_words;
+    if (bm_ok && (cap * 8 >= nwords || 2 * cap > HASH_MAX)) return CLS_BITMAP;
+    if (2 * cap <= HASH_MAX)
+    {
+        int64_t s = 64;
+        int c = 1;
+        while (s < 2 * cap)
+        {
+            s <<= 1;
+            c++;
+        }
+        return c;
+    }
+    return CLS_GLOBAL;
+}
+
+// mode 0: symbolic accumulator classes; mode 1: 0 empty, 1 <= t1, 2 <= t2, 3 larger
+template <typename T>
+__global__ void k_classify(int64_t m, const T *__restrict__ cap, int mode, int nwords,
+                           int lds_words, int t1, int t2, unsigned char *__restrict__ cls,
+                           int *__restrict__ counts)
+{
+    __shared__ int hist[NCLASS];
+    if (threadIdx.x < NCLASS) hist[threadIdx.x] = 0;
+    __syncthreads();
+    int64_t row = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (row < m)
+    {
+        int64_t v = (int64_t) cap[row];
+        int c = mode == 0 ? classify(v, nwords, lds_words)
+                          : (v <= 0 ? 0 : (v <= t1 ? 1 : (v <= t2 ? 2 : 3)));
+        cls[row] = (unsigned char) c;
+        atomicAdd(&hist[c], 1);
+    }
+    __syncthreads();
+    if (threadIdx.x < NCLASS && hist[threadIdx.x]) atomicAdd(&counts[threadIdx.x], hist[threadIdx.x]);
+}
+
+__global__ void k_scatter_rows(int64_t m, const unsigned char *__restrict__ cls,
+                               int *__restrict__ cursor, int *__restrict__ rows)
+{
+    int64_t row = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (row >= m) return;
+    int c = cls[row];
+    if (c == 0) return;
+    int pos = atomicAdd(&cursor[c], 1);
+    rows[pos] = (int) row;
+}
+
+// PASS 0: count distinct columns -> rowcnt.  PASS 1: write sorted columns.
+template <int G, int S, int PASS>
+__global__ void __launch_bounds__(G)
+    k_sym_hash(const int *__restrict__ rows, int Lm, const int *__restrict__ Lp,
+               const int *__restrict__ Li, int nL, const int *__restrict__ Ap,
+               const int *__restrict__ Ai, int *__restrict__ rowcnt,
+               const int64_t *__restrict__ Cp, int *__restrict__ Ci)
+{
+    __shared__ int tab[S];
+    __shared__ Expand<G> ex;
+    __shared__ long long red[G / 64];
+    const int row = rows[blockIdx.x];
+    for (int i = threadIdx.x; i < S; i += G) tab[i] = -1;
+    __syncthreads();
+    RowCtx c = row_ctx(row, Lm, Lp, nL);
+    for_each_term<G>(ex, c.q0, c.q1, c.koff, Li, Ap,
+                     [&](int t, int, int) { hash_insert<S>(tab, Ai[t]); });
+    if (PASS == 0)
+    {
+        long long cnt = 0;
+        for (int i = threadIdx.x; i < S; i += G) cnt += tab[i] != -1;
+        cnt = block_sum_ll<G>(cnt, red);
+        if (threadIdx.x == 0) rowcnt[row] = (int) cnt;
+    }
+    else
+    {
+        bitonic_sort<G, S>((unsigned *) tab);
+        int64_t base = Cp[row];
+        int cnt = (int) (Cp[row + 1] - base);
+        for (int i = threadIdx.x; i < cnt; i += G) Ci[base + i] = tab[i];
+    }
+}
+
+// Bitmap accumulator over all n columns, in LDS (GLOBAL = false) or in a
+// per-workgroup slice of global scratch (GLOBAL = true, grid-strided rows).
+template <int G, int PASS, bool GLOBAL>
+__global__ void __launch_bounds__(G)
+    k_sym_bitmap(const int *__restrict__ rows, int nrows, int nwords, int Lm,
+                 const int *__restrict__ Lp, const int *__restrict__ Li, int nL,
+                 const int *__restrict__ Ap, const int *__restrict__ Ai,
+                 int *__restrict__ rowcnt, const int64_t *__restrict__ Cp,
+                 int *__restrict__ Ci, unsigned *__restrict__ gscratch)
+{
+    extern __shared__ unsigned lds_bm[];
+    __shared__ Expand<G> ex;
+    __shared__ long long red[G / 64];
+    __shared__ int wsum[G / 64];
+    unsigned *bm = GLOBAL ? gscratch + (size_t) blockIdx.x * nwords : lds_bm;
+    for (int ri = blockIdx.x; ri < nrows; ri += (GLOBAL ? gridDim.x : nrows))
+    {
+        const int row = rows[ri];
+        for (int i = threadIdx.x; i < nwords; i += G) bm[i] = 0u;
+        if (GLOBAL) __threadfence_block();
+        __syncthreads();
+        RowCtx c = row_ctx(row, Lm, Lp, nL);
+        for_each_term<G>(ex, c.q0, c.q1, c.koff, Li, Ap, [&](int t, int, int) {
+            int col = Ai[t];
+            atomicOr(&bm[col >> 5], 1u << (col & 31));
+        });
+        if (GLOBAL) __threadfence_block();
+        __syncthreads();
+        // contiguous word range per thread keeps the output ascending
+        int per = (nwords + G - 1) / G;
+        int w0 = min(nwords, (int) threadIdx.x * per), w1 = min(nwords, w0 + per);
+        int cnt = 0;
+        for (int w = w0; w < w1; w++) cnt += __popc(bm[w]);
+        if (PASS == 0)
+        {
+            long long tot = block_sum_ll<G>((long long) cnt, red);
+            if (threadIdx.x == 0) rowcnt[row] = (int) tot;
+        }
+        else
+        {
+            int tot;
+            int pos = block_excl_scan_int<G>(cnt, &tot, wsum);
+            int64_t base = Cp[row] + pos;
+            for (int w = w0; w < w1; w++)
+            {
+                unsigned bits = bm[w];
+                while (bits)
+                {
+                    int b = __ffs(bits) - 1;
+                    bits &= bits - 1;
+                    Ci[base++] = w * 32 + b;
+                }
+            }
+        }
+        __syncthreads();
+        if (!GLOBAL) break;
+    }
+}
+
+// Term lists of one output row: entries' term counts, offsets, scatter, then
+// each entry's list sorted by k (k is unique within an entry).  cols/off live
+// in LDS (CAP > 0) or in global memory (CAP == 0: cols = Ci, off = gscratch).
+template <int G, int CAP>
+__global__ void __launch_bounds__(G)
+    k_terms(const int *__restrict__ rows, int Lm, const int *__restrict__ Lp,
+            const int *__restrict__ Li, const int *__restrict__ Lperm, int nL,
+            const int *__restrict__ Ap, const int *__restrict__ Ai,
+            const int64_t *__restrict__ Cp, const int *__restrict__ Ci,
+            const int64_t *__restrict__ toff, int64_t *__restrict__ ep, int *__restrict__ tA,
+            int *__restrict__ tB, int *__restrict__ tK, int *__restrict__ gscratch)
+{
+    __shared__ int s_cols[CAP > 0 ? CAP : 1];
+    __shared__ int s_off[CAP > 0 ? CAP : 1];
+    __shared__ Expand<G> ex;
+    __shared__ int wsum[G / 64];
+    const int row = rows[blockIdx.x];
+    const int64_t cbase = Cp[row];
+    const int cnt = (int) (Cp[row + 1] - cbase);
+    const int *cols;
+    int *off;
+    if (CAP > 0)
+    {
+        for (int i = threadIdx.x; i < cnt; i += G)
+        {
+            s_cols[i] = Ci[cbase + i];
+            s_off[i] = 0;
+        }
+        cols = s_cols;
+        off = s_off;
+    }
+    else
+    {
+        cols = Ci + cbase;
+        off = gscratch + cbase;
+        for (int i = threadIdx.x; i < cnt; i += G) off[i] = 0;
+        __threadfence_block();
+    }
+    __syncthreads();
+    RowCtx c = row_ctx(row, Lm, Lp, nL);
+    for_each_term<G>(ex, c.q0, c.q1, c.koff, Li, Ap, [&](int t, int, int) {
+        atomicAdd(&off[lower_bound_int(cols, cnt, Ai[t])], 1);
+    });
+    // blocked exclusive scan of the per-entry counts
+    {
+        int per = (cnt + G - 1) / G;
+        int e0 = min(cnt, (int) threadIdx.x * per), e1 = min(cnt, e0 + per);
+        int s = 0;
+        for (int e = e0; e < e1; e++) s += off[e];
+        int tot;
+        int pre = block_excl_scan_int<G>(s, &tot, wsum);
+        const int64_t rbase = toff[row];
+        for (int e = e0; e < e1; e++)
+        {
+            int v = off[e];
+            off[e] = pre;
+            ep[cbase + e] = rbase + pre;
+            pre += v;
+        }
+        if (CAP == 0) __threadfence_block();
+        __syncthreads();
+    }
+    const int64_t rbase = toff[row];
+    for_each_term<G>(ex, c.q0, c.q1, c.koff, Li, Ap, [&](int t, int q, int k) {
+        int pos = lower_bound_int(cols, cnt, Ai[t]);
+        int64_t slot = rbase + atomicAdd(&off[pos], 1);
+        tA[slot] = t;
+        tB[slot] = Lperm ? Lperm[q] : q;
+        tK[slot] = k;
+    });
+    __threadfence_block();
+    __syncthreads();
+    // off[e] is now the end of entry e's list; sort each list by k
+    for (int e = threadIdx.x; e < cnt; e += G)
+    {
+        int64_t s1 = rbase + off[e];
+        int64_t s0 = e == 0 ? rbase : rbase + off[e - 1];
+        for (int64_t i = s0 + 1; i < s1; i++)
+        {
+            int kk = tK[i], aa = tA[i], bb = tB[i];
+            int64_t j = i - 1;
+            while (j >= s0 && tK[j] > kk)
+            {
+                tK[j + 1] = tK[j];
+                tA[j + 1] = tA[j];
+                tB[j + 1] = tB[j];
+                j--;
+            }
+            tK[j + 1] = kk;
+            tA[j + 1] = aa;
+            tB[j + 1] = bb;
+        }
+    }
+}
+
+__global__ void k_set_last(int64_t *ep, int64_t n, int64_t v)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) ep[n] = v;
+}
+
+// ----------------------------------------------------------- numeric ----
+__global__ void __launch_bounds__(256)
+    k_num_pull(int64_t nnz, const int64_t *__restrict__ ep, const int *__restrict__ tA,
+               const int *__restrict__ tB, const int *__restrict__ tK,
+               const double *__restrict__ Lval, const double *__restrict__ d,
+               const double *__restrict__ Ax, double *__restrict__ Cx)
+{
+    int64_t e = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nnz) return;
+    int64_t s0 = ep[e], s1 = ep[e + 1];
+    double acc = 0.0;
+    for (int64_t s = s0; s < s1; s++)
+    {
+        double v = Lval[tB[s]];
+        if (d) v = v * d[tK[s]];
+        acc = acc + v * Ax[tA[s]];
+    }
+    Cx[e] = acc;
+}
+
+constexpr int NT_REG = 4;
+
+__global__ void __launch_bounds__(256)
+    k_num_pull_batched(int64_t nnz, const int64_t *__restrict__ ep, const int *__restrict__ tA,
+                       const int *__restrict__ tB, const int *__restrict__ tK,
+                       const double *__restrict__ Lval, const double *__restrict__ D,
+                       int64_t ldd, int K, const double *__restrict__ Ax,
+                       double *__restrict__ C, int64_t ldc)
+{
+    int64_t e = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nnz) return;
+    const int64_t s0 = ep[e];
+    const int nt = (int) (ep[e + 1] - s0);
+    double bv[NT_REG], av[NT_REG];
+    int kk[NT_REG];
+#pragma unroll
+    for (int j = 0; j < NT_REG; j++)
+    {
+        if (j < nt)
+        {
+            bv[j] = Lval[tB[s0 + j]];
+            av[j] = Ax[tA[s0 + j]];
+            kk[j] = tK[s0 + j];
+        }
+        else
+        {
+            bv[j] = 0.0;
+            av[j] = 0.0;
+            kk[j] = 0;
+        }
+    }
+    if (nt <= NT_REG)
+    {
+        for (int r = 0; r < K; r++)
+        {
+            const double *dr = D + (int64_t) r * ldd;
+            double acc = 0.0;
+#pragma unroll
+            for (int j = 0; j < NT_REG; j++)
+                if (j < nt) acc = acc + (bv[j] * dr[kk[j]]) * av[j];
+            __builtin_nontemporal_store(acc, C + (int64_t) r * ldc + e);
+        }
+    }
+    else
+    {
+        for (int r = 0; r < K; r++)
+        {
+            const double *dr = D + (int64_t) r * ldd;
+            double acc = 0.0;
+            for (int j = 0; j < nt; j++)
+            {
+                int64_t s = s0 + j;
+                acc = acc + (Lval[tB[s]] * dr[tK[s]]) * Ax[tA[s]];
+            }
+            __builtin_nontemporal_store(acc, C + (int64_t) r * ldc + e);
+        }
+    }
+}
+
+// Ordered row accumulator: k walks L's row in storage order (one barrier per
+// k); the columns of one A row are distinct, so its terms update distinct
+// slots in parallel.  CAP == 0: values accumulate in Cx itself.
+template <int G, int CAP>
+__global__ void __launch_bounds__(G)
+    k_num_rowacc(const int *__restrict__ rows, int Lm, const int *__restrict__ Lp,
+                 const int *__restrict__ Li, const int *__restrict__ Lperm, int nL,
+                 const int *__restrict__ Ap, const int *__restrict__ Ai,
+                 const int64_t *__restrict__ Cp, const int *__restrict__ Ci,
+                 const double *__restrict__ Lval, const double *__restrict__ d,
+                 const double *__restrict__ Ax, double *__restrict__ Cx)
+{
+    __shared__ int s_cols[CAP > 0 ? CAP : 1];
+    __shared__ double s_val[CAP > 0 ? CAP : 1];
+    const int row = rows[blockIdx.x];
+    const int64_t cbase = Cp[row];
+    const int cnt = (int) (Cp[row + 1] - cbase);
+    const int *cols = CAP > 0 ? s_cols : Ci + cbase;
+    double *val = CAP > 0 ? s_val : Cx + cbase;
+    for (int i = threadIdx.x; i < cnt; i += G)
+    {
+        if (CAP > 0) s_cols[i] = Ci[cbase + i];
+        val[i] = 0.0;
+    }
+    if (CAP == 0) __threadfence_block();
+    __syncthreads();
+    RowCtx c = row_ctx(row, Lm, Lp, nL);
+    for (int q = c.q0; q < c.q1; q++)
+    {
+        int k = Li[q] + c.koff;
+        double v = Lval[Lperm ? Lperm[q] : q];
+        if (d) v = v * d[k];
+        for (int t = Ap[k] + (int) threadIdx.x; t < Ap[k + 1]; t += G)
+        {
+            int pos = lower_bound_int(cols, cnt, Ai[t]);
+            val[pos] = val[pos] + v * Ax[t];
+        }
+        if (CAP == 0) __threadfence_block();
+        __syncthreads();
+    }
+    if (CAP > 0)
+        for (int i = threadIdx.x; i < cnt; i += G) Cx[cbase + i] = s_val[i];
+}
+
+// ------------------------------------------------------------- host ----
+struct Bins
+{
+    std::vector<int> count = std::vector<int>(NCLASS, 0);
+    std::vector<int> off = std::vector<int>(NCLASS + 1, 0);
+    DBuf<int> rows;
+};
+
+template <typename T>
+void bin_rows(Bins &B, const T *cap, int64_t m, int nwords, int mode = 0, int t1 = 0, int t2 = 0)
+{
+    DBuf<unsigned char> cls(m);
+    DBuf<int> counts(NCLASS);
+    dzero(counts.p, counts.bytes());
+    unsigned grid = (unsigned) ((m + 255) / 256);
+    if (m > 0)
+        hipLaunchKernelGGL(k_classify<T>, dim3(grid), dim3(256), 0, stream(), m, cap, mode,
+                           nwords, lds_bitmap_words(), t1, t2, cls.p, counts.p);
+    SDL_HIP(hipGetLastError());
+    d2h(B.count.data(), counts.p, NCLASS * sizeof(int));
+    B.off[0] = 0;
+    for (int c = 0; c < NCLASS; c++) B.off[c + 1] = B.off[c] + (c == 0 ? 0 : B.count[c]);
+    B.rows.alloc(std::max(1, B.off[NCLASS]));
+    std::vector<int> cur(NCLASS);
+    for (int c = 0; c < NCLASS; c++) cur[c] = B.off[c];
+    h2d(counts.p, cur.data(), NCLASS * sizeof(int));
+    if (m > 0)
+        hipLaunchKernelGGL(k_scatter_rows, dim3(grid), dim3(256), 0, stream(), m, cls.p,
+                           counts.p, B.rows.p);
+    SDL_HIP(hipGetLastError());
+    sync();
+}
+
+template <int PASS>
+void run_symbolic_class(int c, const Bins &B, const ProductPlan &P, int nwords, int *rowcnt,
+                        int64_t *Cp, int *Ci)
+{
+    int nrows = B.count[c];
+    if (nrows == 0) return;
+    const int *rows = B.rows.p + B.off[c];
+    const int nL = P.A.Am / P.L.blocks;
+    const LView &L = P.L;
+    const AView &A = P.A;
+#define SDL_HASH_CASE(CLS, G, S)                                                          \
+    case CLS:                                                                             \
+        hipLaunchKernelGGL((k_sym_hash<G, S, PASS>), dim3(nrows), dim3(G), 0, stream(), rows, \
+                           L.Lm, L.Lp, L.Li, nL, A.Ap, A.Ai, rowcnt, Cp, Ci);            \
+        break;
+    switch (c)
+    {
+        SDL_HASH_CASE(1, 64, 64)
+        SDL_HASH_CASE(2, 64, 128)
+        SDL_HASH_CASE(3, 64, 256)
+        SDL_HASH_CASE(4, 128, 512)
+        SDL_HASH_CASE(5, 128, 1024)
+        SDL_HASH_CASE(6, 256, 2048)
+        SDL_HASH_CASE(7, 256, 4096)
+        SDL_HASH_CASE(8, 256, 8192)
+    case CLS_BITMAP:
+    {
+        size_t lds = (size_t) nwords * 4;
+        auto fn = k_sym_bitmap<256, PASS, false>;
+        SDL_HIP(hipFuncSetAttribute((const void *) fn,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds));
+        hipLaunchKernelGGL(fn, dim3(nrows), dim3(256), lds, stream(), rows, nrows, nwords,
+                           L.Lm, L.Lp, L.Li, nL, A.Ap, A.Ai, rowcnt, Cp, Ci,
+                           (unsigned *) nullptr);
+        break;
+    }
+    case CLS_GLOBAL:
+    {
+        int grid = std::min(nrows, 512);
+        DBuf<unsigned> scratch((size_t) grid * nwords);
+        hipLaunchKernelGGL((k_sym_bitmap<256, PASS, true>), dim3(grid), dim3(256), 0,
+                           stream(), rows, nrows, nwords, L.Lm, L.Lp, L.Li, nL, A.Ap, A.Ai,
+                           rowcnt, Cp, Ci, scratch.p);
+        SDL_HIP(hipGetLastError());
+        sync();
+        break;
+    }
+    default: break;
+    }
+#undef SDL_HASH_CASE
+    SDL_HIP(hipGetLastError());
+}
+
+} // namespace
+
+int64_t product_count_terms(const LView &L, const AView &A, int64_t *row_terms_out)
+{
+    int64_t m_out = (int64_t) L.Lm * L.blocks;
+    if (m_out == 0) return 0;
+    DBuf<int64_t> tmp;
+    int64_t *terms = row_terms_out;
+    if (!terms)
+    {
+        tmp.alloc(m_out);
+        terms = tmp.p;
+    }
+    hipLaunchKernelGGL(k_row_terms, dim3((unsigned) ((m_out + 255) / 256)), dim3(256), 0,
+                       stream(), m_out, L.Lm, L.Lp, L.Li, A.Am / L.blocks, A.Ap, terms);
+    SDL_HIP(hipGetLastError());
+    DBuf<int64_t> off(m_out + 1);
+    return scan_i64(terms, off.p, m_out, true);
+}
+
+void product_plan(ProductPlan &P, const LView &L, const AView &A, SymMode mode,
+                  const float *pred, bool allow_pull)
+{
+    (void) mode;
+    (void) pred;
+    SDL_REQUIRE(L.blocks >= 1 && A.Am % L.blocks == 0, "product: A rows not a multiple of blocks");
+    P.L = L;
+    P.A = A;
+    P.m_out = (int64_t) L.Lm * L.blocks;
+    P.n = A.An;
+    const int64_t m_out = P.m_out;
+    const int nL = A.Am / L.blocks;
+    const int nwords = (A.An + 31) / 32;
+    P.stats = SymStats();
+    P.stats.rows = m_out;
+    P.Cp.alloc(m_out + 1);
+    if (m_out == 0)
+    {
+        dzero(P.Cp.p, sizeof(int64_t));
+        P.nnz = 0;
+        P.terms = 0;
+        P.Ci.alloc(0);
+        return;
+    }
+
+    // per-row term bound and its scan
+    DBuf<int64_t> rterms(m_out);
+    hipLaunchKernelGGL(k_row_terms, dim3((unsigned) ((m_out + 255) / 256)), dim3(256), 0,
+                       stream(), m_out, L.Lm, L.Lp, L.Li, nL, A.Ap, rterms.p);
+    SDL_HIP(hipGetLastError());
+    DBuf<int64_t> toff(m_out + 1);
+    P.terms = scan_i64(rterms.p, toff.p, m_out, true);
+    P.stats.terms = P.terms;
+
+    // count pass, accumulators sized from the term bound
+    DBuf<int> rowcnt(m_out);
+    dzero(rowcnt.p, rowcnt.bytes());
+    {
+        Bins B;
+        bin_rows<int64_t>(B, rterms.p, m_out, nwords);
+        for (int c = 1; c < NCLASS; c++)
+            run_symbolic_class<0>(c, B, P, nwords, rowcnt.p, nullptr, nullptr);
+    }
+    P.nnz = scan_i32_to_i64(rowcnt.p, P.Cp.p, m_out, true);
+    P.stats.nnz = P.nnz;
+    P.Ci.alloc(P.nnz);
+
+    // fill pass, accumulators sized from the exact count
+    Bins B2;
+    bin_rows<int>(B2, rowcnt.p, m_out, nwords);
+    for (int c = 1; c < NCLASS; c++)
+    {
+        run_symbolic_class<1>(c, B2, P, nwords, nullptr, P.Cp.p, P.Ci.p);
+        if (c >= 1 && c <= 8) P.stats.hash_rows += B2.count[c];
+    }
+    P.stats.bitmap_rows = B2.count[CLS_BITMAP];
+    P.stats.global_rows = B2.count[CLS_GLOBAL];
+
+    // refill schedule (reference policy: a plan iff terms <= 3 nnz, gustavson.h:45-52)
+    P.pull = allow_pull && P.terms > 0 && P.terms <= 3 * P.nnz &&
+             P.terms < (int64_t) 0x7fffffff && P.nnz < (int64_t) 0x7fffffff;
+    if (P.pull)
+    {
+        P.ep.alloc(P.nnz + 1);
+        P.tA.alloc(P.terms);
+        P.tB.alloc(P.terms);
+        P.tK.alloc(P.terms);
+        // rows binned by nnz: LDS lists up to 2048 entries, global beyond
+        Bins BT;
+        bin_rows<int>(BT, rowcnt.p, m_out, nwords, 1, 128, 2048);
+        DBuf<int> gscratch;
+        for (int c = 1; c <= 3; c++)
+        {
+            int nrows = BT.count[c];
+            if (!nrows) continue;
+            const int *rows = BT.rows.p + BT.off[c];
+            if (c == 1)
+                hipLaunchKernelGGL((k_terms<64, 128>), dim3(nrows), dim3(64), 0, stream(), rows,
+                                   L.Lm, L.Lp, L.Li, L.Lperm, nL, A.Ap, A.Ai, P.Cp.p, P.Ci.p,
+                                   toff.p, P.ep.p, P.tA.p, P.tB.p, P.tK.p, (int *) nullptr);
+            else if (c == 2)
+                hipLaunchKernelGGL((k_terms<256, 2048>), dim3(nrows), dim3(256), 0, stream(),
+                                   rows, L.Lm, L.Lp, L.Li, L.Lperm, nL, A.Ap, A.Ai, P.Cp.p,
+                                   P.Ci.p, toff.p, P.ep.p, P.tA.p, P.tB.p, P.tK.p,
+                                   (int *) nullptr);
+            else
+            {
+                gscratch.alloc(P.nnz);
+                hipLaunchKernelGGL((k_terms<256, 0>), dim3(nrows), dim3(256), 0, stream(), rows,
+                                   L.Lm, L.Lp, L.Li, L.Lperm, nL, A.Ap, A.Ai, P.Cp.p, P.Ci.p,
+                                   toff.p, P.ep.p, P.tA.p, P.tB.p, P.tK.p, gscratch.p);
+            }
+            SDL_HIP(hipGetLastError());
+        }
+        hipLaunchKernelGGL(k_set_last, dim3(1), dim3(64), 0, stream(), P.ep.p, P.nnz, P.terms);
+        SDL_HIP(hipGetLastError());
+        sync();
+    }
+    else
+    {
+        // row-accumulator bins: rows with <= 1024 entries accumulate in LDS, larger
+        // rows in place in C.x
+        Bins BR;
+        bin_rows<int>(BR, rowcnt.p, m_out, nwords, 1, 1024, 1024);
+        int small = BR.count[1], large = BR.count[3];
+        P.rbins.alloc(std::max(1, small + large));
+        P.rbin_off[0] = 0;
+        P.rbin_off[1] = small;
+        P.rbin_off[2] = small + large;
+        d2d(P.rbins.p, BR.rows.p + BR.off[1], (size_t) small * sizeof(int));
+        d2d(P.rbins.p + small, BR.rows.p + BR.off[3], (size_t) large * sizeof(int));
+        sync();
+    }
+}
+
+void product_numeric(const ProductPlan &P, const double *Lval, const double *d,
+                     const double *Ax, double *Cx, hipStream_t s)
+{
+    if (P.nnz == 0) return;
+    if (P.pull)
+    {
+        hipLaunchKernelGGL(k_num_pull, dim3((unsigned) ((P.nnz + 255) / 256)), dim3(256), 0, s,
+                           P.nnz, P.ep.p, P.tA.p, P.tB.p, P.tK.p, Lval, d, Ax, Cx);
+    }
+    else
+    {
+        const int nL = P.A.Am / P.L.blocks;
+        int n0 = P.rbin_off[1] - P.rbin_off[0];
+        int n1 = P.rbin_off[2] - P.rbin_off[1];
+        if (n0)
+            hipLaunchKernelGGL((k_num_rowacc<256, 1024>), dim3(n0), dim3(256), 0, s, P.rbins.p,
+                               P.L.Lm, P.L.Lp, P.L.Li, P.L.Lperm, nL, P.A.Ap, P.A.Ai, P.Cp.p,
+                               P.Ci.p, Lval, d, Ax, Cx);
+        if (n1)
+            hipLaunchKernelGGL((k_num_rowacc<256, 0>), dim3(n1), dim3(256), 0, s,
+                               P.rbins.p + P.rbin_off[1], P.L.Lm, P.L.Lp, P.L.Li, P.L.Lperm, nL,
+                               P.A.Ap, P.A.Ai, P.Cp.p, P.Ci.p, Lval, d, Ax, Cx);
+    }
+    SDL_HIP(hipGetLastError());
+}
+
+void product_numeric_batched(const ProductPlan &P, const double *Lval, const double *D,
+                             int64_t ldd, int K, const double *Ax, double *Cx, int64_t ldc,
+                             hipStream_t s)
+{
+    if (P.nnz == 0 || K <= 0) return;
+    if (P.pull)
+    {
+        hipLaunchKernelGGL(k_num_pull_batched, dim3((unsigned) ((P.nnz + 255) / 256)),
+                           dim3(256), 0, s, P.nnz, P.ep.p, P.tA.p, P.tB.p, P.tK.p, Lval, D,
+                           ldd, K, Ax, Cx, ldc);
+        SDL_HIP(hipGetLastError());
+    }
+    else
+    {
+        for (int r = 0; r < K; r++)
+            product_numeric(P, Lval, D + (int64_t) r * ldd, Ax, Cx + (int64_t) r * ldc, s);
+    }
+}
+
+} // namespace sdl

@@ -1,0 +1,189 @@
+// Library runtime: the library stream, device allocation with counters (the
+// fill phase must not allocate; tests read the counters), error reporting.
+#include "sdl_internal.h"
+#include "../../include/sdl_device.h"
+
+#include <atomic>
+#include <cstdarg>
+#include <cstring>
+#include <mutex>
+
+namespace sdl {
+
+namespace {
+std::once_flag g_once;
+hipStream_t g_stream = nullptr;
+std::atomic<long long> g_dev_allocs{0};
+std::atomic<long long> g_dev_frees{0};
+thread_local char g_err[512] = {0};
+
+void init_once()
+{
+    int ndev = 0;
+    hipError_t e = hipGetDeviceCount(&ndev);
+    if (e != hipSuccess || ndev == 0)
+    {
+        std::fprintf(stderr,
+                     "sparsediff_linalg_amd: no HIP device available (%s); this library "
+                     "has no CPU path.\n",
+                     e != hipSuccess ? hipGetErrorString(e) : "0 devices");
+        std::abort();
+    }
+    // Default priority only: every stream of the library runs at priority 0.
+    SDL_HIP(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
+}
+} // namespace
+
+[[noreturn]] void fatal(const char *what, const char *file, int line)
+{
+    std::fprintf(stderr, "sparsediff_linalg_amd: fatal: %s (%s:%d)\n", what, file, line);
+    std::fflush(stderr);
+    std::abort();
+}
+
+void set_error(const char *fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    std::vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+}
+
+hipStream_t stream()
+{
+    std::call_once(g_once, init_once);
+    return g_stream;
+}
+
+void sync() { SDL_HIP(hipStreamSynchronize(stream())); }
+
+void *dmalloc(size_t bytes)
+{
+    stream();
+    void *p = nullptr;
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e != hipSuccess)
+    {
+        char msg[160];
+        std::snprintf(msg, sizeof(msg), "hipMalloc of %zu bytes failed: %s", bytes,
+                      hipGetErrorString(e));
+        fatal(msg, __FILE__, __LINE__);
+    }
+    g_dev_allocs++;
+    return p;
+}
+
+void dfree(void *p)
+{
+    if (!p) return;
+    // hipFree synchronises the device; fine: frees happen only in free paths.
+    SDL_HIP(hipFree(p));
+    g_dev_frees++;
+}
+
+void h2d(void *dst, const void *src, size_t bytes)
+{
+    if (bytes == 0) return;
+    SDL_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, stream()));
+}
+
+void d2h(void *dst, const void *src, size_t bytes)
+{
+    if (bytes == 0) return;
+    SDL_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, stream()));
+    SDL_HIP(hipStreamSynchronize(stream()));
+}
+
+void d2d(void *dst, const void *src, size_t bytes)
+{
+    if (bytes == 0) return;
+    SDL_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, stream()));
+}
+
+void dzero(void *dst, size_t bytes)
+{
+    if (bytes == 0) return;
+    SDL_HIP(hipMemsetAsync(dst, 0, bytes, stream()));
+}
+
+} // namespace sdl
+
+extern "C" {
+#pragma GCC visibility push(default)
+
+const char *sdl_last_error(void) { return sdl::g_err; }
+
+void sdl_init(void) { (void) sdl::stream(); }
+
+void *sdl_library_stream(void) { return (void *) sdl::stream(); }
+
+void sdl_synchronize(void) { sdl::sync(); }
+
+long long sdl_device_alloc_count(void) { return sdl::g_dev_allocs.load(); }
+
+long long sdl_device_free_count(void) { return sdl::g_dev_frees.load(); }
+
+void *sdl_dev_alloc(size_t bytes) { return sdl::dmalloc(bytes); }
+
+void sdl_dev_free(void *p) { sdl::dfree(p); }
+
+int sdl_memcpy_htod(void *dst, const void *src, size_t bytes)
+{
+    sdl::h2d(dst, src, bytes);
+    sdl::sync();
+    return 0;
+}
+
+int sdl_memcpy_dtoh(void *dst, const void *src, size_t bytes)
+{
+    sdl::d2h(dst, src, bytes);
+    return 0;
+}
+
+int sdl_memset_d(void *dst, int value, size_t bytes)
+{
+    SDL_HIP(hipMemsetAsync(dst, value, bytes, sdl::stream()));
+    sdl::sync();
+    return 0;
+}
+
+void *sdl_event_create(void)
+{
+    sdl::stream();
+    hipEvent_t e;
+    SDL_HIP(hipEventCreate(&e));
+    return (void *) e;
+}
+
+void sdl_event_record(void *ev, void *stream)
+{
+    SDL_HIP(hipEventRecord((hipEvent_t) ev, stream ? (hipStream_t) stream : sdl::stream()));
+}
+
+float sdl_event_elapsed_ms(void *start, void *stop)
+{
+    SDL_HIP(hipEventSynchronize((hipEvent_t) stop));
+    float ms = 0.f;
+    SDL_HIP(hipEventElapsedTime(&ms, (hipEvent_t) start, (hipEvent_t) stop));
+    return ms;
+}
+
+void sdl_event_destroy(void *ev)
+{
+    if (ev) SDL_HIP(hipEventDestroy((hipEvent_t) ev));
+}
+
+int sdl_device_name(char *buf, int len)
+{
+    sdl::stream();
+    int dev = 0;
+    SDL_HIP(hipGetDevice(&dev));
+    hipDeviceProp_t prop;
+    SDL_HIP(hipGetDeviceProperties(&prop, dev));
+    std::snprintf(buf, (size_t) len, "%s|%s|cu=%d|lds=%zu|l2=%d", prop.name, prop.gcnArchName,
+                  prop.multiProcessorCount, (size_t) prop.sharedMemPerBlock, prop.l2CacheSize);
+    return 0;
+}
+
+#pragma GCC visibility pop
+} // extern "C"

@@ -1,0 +1,81 @@
+// Device-resident sparse x sparse product C = kron(I_blocks, L) . diag(d) . A
+// (the reference's Gustavson core, src/sparse/gustavson.c), split into an
+// alloc-phase plan (exact structure + fill schedule) and refills.
+#pragma once
+
+#include "sdl_internal.h"
+
+namespace sdl {
+
+// Rows of a compressed matrix on the device: row r has entries
+// (Li[q], value) for Lp[r] <= q < Lp[r+1].  The value of entry q lives at
+// Lval[Lperm ? Lperm[q] : q] of the value array the refill is given, so a
+// CSC mirror needs no value copy: Lperm maps its entries to the owner's CSR.
+struct LView
+{
+    int Lm = 0;
+    const int *Lp = nullptr;
+    const int *Li = nullptr;
+    const int *Lperm = nullptr;
+    int blocks = 1;
+};
+
+struct AView
+{
+    int Am = 0, An = 0;
+    const int *Ap = nullptr;
+    const int *Ai = nullptr;
+};
+
+enum class SymMode
+{
+    Exact = 0,      // count pass + fill pass, tables sized from the term bound / exact count
+    Predicted = 1,  // one pass with tables sized from a per-row prediction; overflowing
+                    // rows fall back to the exact path
+};
+
+struct SymStats
+{
+    long long rows = 0, terms = 0, nnz = 0;
+    long long hash_rows = 0, bitmap_rows = 0, global_rows = 0;
+    long long overflow_rows = 0;       // predicted path: rows that fell back
+    long long predicted_rows = 0;      // predicted path: rows served by the prediction
+};
+
+struct ProductPlan
+{
+    LView L;
+    AView A;
+    int64_t m_out = 0, n = 0, nnz = 0, terms = 0;
+    DBuf<int64_t> Cp;   // m_out + 1
+    DBuf<int> Ci;       // nnz
+    // Pull schedule (terms <= 3 * nnz, the reference's plan policy): the terms
+    // of entry e are [ep[e], ep[e+1]) in ascending k.
+    bool pull = false;
+    DBuf<int64_t> ep;
+    DBuf<int> tA, tB, tK;
+    // Row-accumulator schedule (otherwise): rows binned by nnz.
+    DBuf<int> rbins;                 // row ids grouped by class
+    int rbin_off[8] = {0};           // class c is rbins[rbin_off[c] .. rbin_off[c+1])
+    SymStats stats;
+};
+
+// Build the exact structure and the refill schedule.  pred (optional, device,
+// m_out floats) holds predicted per-row nnz for SymMode::Predicted.
+void product_plan(ProductPlan &P, const LView &L, const AView &A, SymMode mode,
+                  const float *pred, bool allow_pull = true);
+
+// One refill: Cx = values of kron(I, L) D A.  Lval / Ax / d are device arrays
+// (d may be null = identity).  Launches on `s`, allocates nothing.
+void product_numeric(const ProductPlan &P, const double *Lval, const double *d,
+                     const double *Ax, double *Cx, hipStream_t s);
+
+// K refills sharing operands: refill r reads d + r*ldd and writes Cx + r*ldc.
+void product_numeric_batched(const ProductPlan &P, const double *Lval, const double *D,
+                             int64_t ldd, int K, const double *Ax, double *Cx, int64_t ldc,
+                             hipStream_t s);
+
+// Number of terms sum_r sum_{q in L_r} len(A_k); device pass, synchronises.
+int64_t product_count_terms(const LView &L, const AView &A, int64_t *row_terms_out);
+
+} // namespace sdl
