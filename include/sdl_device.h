@@ -34,6 +34,9 @@ void sdl_event_record(void *ev, void *stream);
 float sdl_event_elapsed_ms(void *start, void *stop);
 void sdl_event_destroy(void *ev);
 int sdl_device_name(char *buf, int len);
+/* Streaming device copy kernel (16-byte aligned), the HBM-counter calibration
+   reference: exactly `bytes` read and `bytes` written. */
+int sdl_copy_kernel(void *dst, const void *src, size_t bytes, void *stream);
 
 /* ---- C = B^T diag(d) A on device-resident CSR operands (the hot path of
    BTA_matrices_alloc / BTDA_matrices_fill_values for two sparse operands,
@@ -72,6 +75,51 @@ int sdl_btda_refill(const sdl_btda_plan *plan, const double *dAx, const double *
 int sdl_btda_refill_batched(const sdl_btda_plan *plan, const double *dAx,
                             const double *dBx, const double *dD, long long ldd, int K,
                             double *dC, long long ldc, void *stream);
+
+/* Alloc-phase reservation of the batched-refill workspace for up to Kmax
+   refills per call (d transposed to k-major + term operand pairs: 8*m*Kmax +
+   16*terms bytes).  Batched refills with K <= Kmax then use the interleaved
+   kernel; larger K fall back to direct gathers.  Allocates; refills do not. */
+int sdl_btda_plan_reserve(sdl_btda_plan *plan, int Kmax);
+
+/* Gather the term operand pairs of the current A / B values into the
+   reserved workspace (call again whenever A or B values change). */
+int sdl_btda_bind_values(const sdl_btda_plan *plan, const double *dAx, const double *dBx,
+                         void *stream);
+enum
+{
+    SDL_BATCH_VALUES_BOUND = 1, /* operand pairs already bound: skip the gather */
+    SDL_BATCH_D_KMAJOR = 2      /* dD is k-major: refill r of row k at dD[k*ldd + r] */
+};
+/* Batched refill with options; ev_start / ev_stop (hipEvent_t, may be NULL)
+   are recorded on the stream right before / after the main refill kernel. */
+int sdl_btda_refill_batched_ex(const sdl_btda_plan *plan, const double *dAx, const double *dBx,
+                               const double *dD, long long ldd, int K, double *dC,
+                               long long ldc, unsigned flags, void *ev_start, void *ev_stop,
+                               void *stream);
+
+/* Launch configuration of the batched refill: variant 0 = one thread per
+   entry over all K; 1 / 2 = grid over (entry tile, chunk of rchunk refills)
+   with 1 / 2 entries per thread; 3 = interleaved (needs the reservation). */
+typedef struct sdl_refill_cfg
+{
+    int variant;
+    int rchunk;
+} sdl_refill_cfg;
+int sdl_btda_refill_batched_cfg(const sdl_btda_plan *plan, const double *dAx,
+                                const double *dBx, const double *dD, long long ldd, int K,
+                                double *dC, long long ldc, const sdl_refill_cfg *cfg,
+                                void *stream);
+/* Candidate configurations (returns the count, writes up to max). */
+int sdl_refill_candidates(sdl_refill_cfg *out, int max);
+/* Autotuner: times every candidate on this plan and these buffers (reps timed
+   launches each, after 2 warm-ups), keeps the fastest in the plan for later
+   sdl_btda_refill_batched calls, and reports it. */
+int sdl_btda_autotune(sdl_btda_plan *plan, const double *dAx, const double *dBx,
+                      const double *dD, long long ldd, int K, double *dC, long long ldc,
+                      int reps, sdl_refill_cfg *best, float *best_us);
+/* Pin a configuration (NULL: back to the default heuristic). */
+int sdl_btda_set_refill_cfg(sdl_btda_plan *plan, const sdl_refill_cfg *cfg);
 
 /* ---- synthetic workload generators (host; deterministic in seed) ---- */
 /* Row lengths uniform in [lo, hi]; writes Ap[m+1]; returns nnz. */

@@ -319,3 +319,42 @@ def ref_fill_timer(m, nA, nB, Ap, Ai, Ax, Bp, Bi, Bx, ds):
     r.sparse_matrix_free(Cm)
     r.free_CSC_matrix(Bc)
     return alloc_s, fills, Cx
+
+
+def port_fill_timer(m, nA, nB, Ap, Ai, Ax, Bp, Bi, Bx, ds):
+    """Times the C restatement: alloc (transpose + symbolic + plan) once, then
+    one fill per d in ds through the plan or accumulator (the reference's
+    policy).  Returns (alloc_s, [fill_s...])."""
+    import time
+    o = lib()
+    t0 = time.perf_counter()
+    Lp, Li, Lx = transpose(m, nB, Bp, Bi, Bx)
+    Lp, Li = np.ascontiguousarray(Lp, np.int32), np.ascontiguousarray(Li, np.int32)
+    Ap, Ai = np.ascontiguousarray(Ap, np.int32), np.ascontiguousarray(Ai, np.int32)
+    Cp = np.zeros(nB + 1, np.int64)
+    nnz = o.orc_spgemm_symbolic(nB, I(Lp), I(Li), 1, m, nA, I(Ap), I(Ai), L(Cp), None)
+    Ci = np.zeros(max(nnz, 1), np.int32)
+    o.orc_spgemm_symbolic(nB, I(Lp), I(Li), 1, m, nA, I(Ap), I(Ai), L(Cp), I(Ci))
+    terms = o.orc_spgemm_plan(nB, I(Lp), I(Li), 1, m, nA, I(Ap), I(Ai), L(Cp), I(Ci), 3 * nnz,
+                              None)
+    plan = None
+    if terms > 0:
+        plan = np.zeros(terms, np.int64)
+        o.orc_spgemm_plan(nB, I(Lp), I(Li), 1, m, nA, I(Ap), I(Ai), L(Cp), I(Ci), 3 * nnz, L(plan))
+    alloc_s = time.perf_counter() - t0
+    Cx = np.zeros(max(nnz, 1))
+    acc = np.zeros(max(nA, 1))
+    Lx = np.ascontiguousarray(Lx)
+    Ax = np.ascontiguousarray(Ax, np.float64)
+    fills = []
+    for d in ds:
+        dd = np.ascontiguousarray(d, np.float64)
+        t1 = time.perf_counter()
+        if plan is not None:
+            o.orc_spgemm_numeric_planned(nB, I(Lp), I(Li), D(Lx), 1, D(dd), m, I(Ap), D(Ax),
+                                         L(plan), nnz, D(Cx))
+        else:
+            o.orc_spgemm_numeric(nB, I(Lp), I(Li), D(Lx), 1, D(dd), m, nA, I(Ap), I(Ai), D(Ax),
+                                 L(Cp), I(Ci), D(acc), D(Cx))
+        fills.append(time.perf_counter() - t1)
+    return alloc_s, fills

@@ -47,6 +47,10 @@ class matrix(C.Structure):
     pass
 
 
+class sdl_refill_cfg(C.Structure):
+    _fields_ = [("variant", C.c_int), ("rchunk", C.c_int)]
+
+
 _to_csr_fn = C.CFUNCTYPE(C.POINTER(CSR_matrix), C.POINTER(matrix))
 _gather_alloc_fn = C.CFUNCTYPE(C.POINTER(matrix), C.POINTER(matrix), c_int_p, C.c_int)
 _gather_fill_fn = C.CFUNCTYPE(None, C.POINTER(matrix), C.POINTER(matrix))
@@ -84,6 +88,7 @@ SIGNATURES = {
     "sdl_event_elapsed_ms": (C.c_float, [vp, vp]),
     "sdl_event_destroy": (None, [vp]),
     "sdl_device_name": (C.c_int, [C.c_char_p, C.c_int]),
+    "sdl_copy_kernel": (C.c_int, [vp, vp, C.c_size_t, vp]),
     # device-resident B^T D A plans
     "sdl_btda_plan_create": (C.c_int, [C.c_int, C.c_int, C.c_int, vp, vp, vp, vp, C.c_int,
                                        C.POINTER(vp)]),
@@ -98,6 +103,16 @@ SIGNATURES = {
     "sdl_btda_refill": (C.c_int, [vp, vp, vp, vp, vp, vp]),
     "sdl_btda_refill_batched": (C.c_int, [vp, vp, vp, vp, C.c_longlong, C.c_int, vp,
                                           C.c_longlong, vp]),
+    "sdl_btda_refill_batched_cfg": (C.c_int, [vp, vp, vp, vp, C.c_longlong, C.c_int, vp,
+                                              C.c_longlong, C.POINTER(sdl_refill_cfg), vp]),
+    "sdl_refill_candidates": (C.c_int, [C.POINTER(sdl_refill_cfg), C.c_int]),
+    "sdl_btda_autotune": (C.c_int, [vp, vp, vp, vp, C.c_longlong, C.c_int, vp, C.c_longlong,
+                                    C.c_int, C.POINTER(sdl_refill_cfg), C.POINTER(C.c_float)]),
+    "sdl_btda_set_refill_cfg": (C.c_int, [vp, C.POINTER(sdl_refill_cfg)]),
+    "sdl_btda_plan_reserve": (C.c_int, [vp, C.c_int]),
+    "sdl_btda_bind_values": (C.c_int, [vp, vp, vp, vp]),
+    "sdl_btda_refill_batched_ex": (C.c_int, [vp, vp, vp, vp, C.c_longlong, C.c_int, vp,
+                                             C.c_longlong, C.c_uint, vp, vp, vp]),
     # generators
     "sdl_gen_row_lengths_uniform": (C.c_longlong, [C.c_uint64, C.c_int, C.c_int, C.c_int,
                                                    c_int_p]),

@@ -6,6 +6,7 @@
 
 #include <climits>
 #include <new>
+#include <vector>
 
 struct sdl_btda_plan
 {
@@ -139,6 +140,147 @@ int sdl_btda_refill_batched(const sdl_btda_plan *plan, const double *dAx, const 
         return 1;
     }
     sdl::product_numeric_batched(plan->P, dBx, dD, ldd, K, dAx, dC, ldc, as_stream(stream));
+    return 0;
+}
+
+int sdl_btda_refill_batched_cfg(const sdl_btda_plan *plan, const double *dAx,
+                                const double *dBx, const double *dD, long long ldd, int K,
+                                double *dC, long long ldc, const sdl_refill_cfg *cfg,
+                                void *stream)
+{
+    if (!plan || !cfg || K < 0 || ldd < plan->m || ldc < plan->P.nnz)
+    {
+        sdl::set_error("sdl_btda_refill_batched_cfg: invalid arguments");
+        return 1;
+    }
+    sdl::RefillCfg c;
+    c.variant = cfg->variant;
+    c.rchunk = cfg->rchunk;
+    sdl::product_numeric_batched(plan->P, dBx, dD, ldd, K, dAx, dC, ldc, as_stream(stream), &c);
+    return 0;
+}
+
+int sdl_refill_candidates(sdl_refill_cfg *out, int max)
+{
+    std::vector<sdl::RefillCfg> v = sdl::refill_candidates();
+    int n = 0;
+    for (auto &c : v)
+    {
+        if (n >= max) break;
+        out[n].variant = c.variant;
+        out[n].rchunk = c.rchunk;
+        n++;
+    }
+    return (int) v.size();
+}
+
+int sdl_btda_autotune(sdl_btda_plan *plan, const double *dAx, const double *dBx,
+                      const double *dD, long long ldd, int K, double *dC, long long ldc,
+                      int reps, sdl_refill_cfg *best, float *best_us)
+{
+    if (!plan || K <= 0 || ldd < plan->m || ldc < plan->P.nnz)
+    {
+        sdl::set_error("sdl_btda_autotune: invalid arguments");
+        return 1;
+    }
+    if (reps < 1) reps = 1;
+    hipEvent_t e0, e1;
+    SDL_HIP(hipEventCreate(&e0));
+    SDL_HIP(hipEventCreate(&e1));
+    hipStream_t s = sdl::stream();
+    float best_t = 1e30f;
+    sdl::RefillCfg best_c = sdl::default_refill_cfg(plan->P, K);
+    for (const sdl::RefillCfg &c : sdl::refill_candidates())
+    {
+        for (int w = 0; w < 2; w++)
+            sdl::product_numeric_batched(plan->P, dBx, dD, ldd, K, dAx, dC, ldc, s, &c);
+        SDL_HIP(hipEventRecord(e0, s));
+        for (int r = 0; r < reps; r++)
+            sdl::product_numeric_batched(plan->P, dBx, dD, ldd, K, dAx, dC, ldc, s, &c);
+        SDL_HIP(hipEventRecord(e1, s));
+        SDL_HIP(hipEventSynchronize(e1));
+        float ms = 0.f;
+        SDL_HIP(hipEventElapsedTime(&ms, e0, e1));
+        float us = ms * 1000.f / reps;
+        if (us < best_t)
+        {
+            best_t = us;
+            best_c = c;
+        }
+    }
+    SDL_HIP(hipEventDestroy(e0));
+    SDL_HIP(hipEventDestroy(e1));
+    plan->P.tuned = best_c;
+    if (best)
+    {
+        best->variant = best_c.variant;
+        best->rchunk = best_c.rchunk;
+    }
+    if (best_us) *best_us = best_t;
+    return 0;
+}
+
+int sdl_btda_bind_values(const sdl_btda_plan *plan, const double *dAx, const double *dBx,
+                         void *stream)
+{
+    if (!plan) return 1;
+    if (plan->P.batch_cap == 0)
+    {
+        sdl::set_error("sdl_btda_bind_values: no batch workspace reserved");
+        return 1;
+    }
+    sdl::product_bind_values(plan->P, dBx, dAx, as_stream(stream));
+    return 0;
+}
+
+int sdl_btda_refill_batched_ex(const sdl_btda_plan *plan, const double *dAx, const double *dBx,
+                               const double *dD, long long ldd, int K, double *dC,
+                               long long ldc, unsigned flags, void *ev_start, void *ev_stop,
+                               void *stream)
+{
+    if (!plan || K < 0 || ldc < plan->P.nnz)
+    {
+        sdl::set_error("sdl_btda_refill_batched_ex: invalid arguments");
+        return 1;
+    }
+    if ((flags & SDL_BATCH_D_KMAJOR) ? ldd < K : ldd < plan->m)
+    {
+        sdl::set_error("sdl_btda_refill_batched_ex: ldd too small for the layout");
+        return 1;
+    }
+    if ((flags & (SDL_BATCH_D_KMAJOR | SDL_BATCH_VALUES_BOUND)) &&
+        (K > plan->P.batch_cap || !plan->P.pull))
+    {
+        sdl::set_error("sdl_btda_refill_batched_ex: K-major / bound refills need a pull plan "
+                       "with a reservation >= K");
+        return 1;
+    }
+    unsigned o = 0;
+    if (flags & SDL_BATCH_VALUES_BOUND) o |= sdl::BATCH_VALUES_BOUND;
+    if (flags & SDL_BATCH_D_KMAJOR) o |= sdl::BATCH_D_KMAJOR;
+    sdl::product_numeric_batched(plan->P, dBx, dD, ldd, K, dAx, dC, ldc, as_stream(stream),
+                                 nullptr, o, ev_start, ev_stop);
+    return 0;
+}
+
+int sdl_btda_plan_reserve(sdl_btda_plan *plan, int Kmax)
+{
+    if (!plan || Kmax < 0) return 1;
+    sdl::product_reserve_batch(plan->P, Kmax);
+    sdl::sync();
+    return 0;
+}
+
+int sdl_btda_set_refill_cfg(sdl_btda_plan *plan, const sdl_refill_cfg *cfg)
+{
+    if (!plan) return 1;
+    if (!cfg)
+    {
+        plan->P.tuned = sdl::RefillCfg();
+        return 0;
+    }
+    plan->P.tuned.variant = cfg->variant;
+    plan->P.tuned.rchunk = cfg->rchunk;
     return 0;
 }
 

@@ -496,6 +496,7 @@ __global__ void __launch_bounds__(256)
 }
 
 constexpr int NT_REG = 4;
+typedef double dvec2 __attribute__((ext_vector_type(2)));
 
 __global__ void __launch_bounds__(256)
     k_num_pull_batched(int64_t nnz, const int64_t *__restrict__ ep, const int *__restrict__ tA,
@@ -551,6 +552,298 @@ __global__ void __launch_bounds__(256)
             }
             __builtin_nontemporal_store(acc, C + (int64_t) r * ldc + e);
         }
+    }
+}
+
+// Batched pull refill, r-chunked: grid covers (entry tile, chunk of RC
+// refills).  Tiles map XCD-aware: the chunks of one tile run back to back on
+// one XCD, so its term data is read from HBM once and then hit in that XCD's
+// L2.  EPT consecutive entries per thread (EPT = 2 stores 16 bytes per lane).
+template <int EPT, int RC>
+__global__ void __launch_bounds__(256)
+    k_num_pull_rchunk(int64_t nnz, int ntiles, int nchunks, const int64_t *__restrict__ ep,
+                      const int *__restrict__ tA, const int *__restrict__ tB,
+                      const int *__restrict__ tK, const double *__restrict__ Lval,
+                      const double *__restrict__ D, int64_t ldd, int K,
+                      const double *__restrict__ Ax, double *__restrict__ C, int64_t ldc)
+{
+    const int b = blockIdx.x;
+    const int xcd = b & 7, s = b >> 3;
+    const int tile = (s / nchunks) * 8 + xcd;
+    const int chunk = s % nchunks;
+    if (tile >= ntiles) return;
+    const int64_t e0 = ((int64_t) tile * blockDim.x + threadIdx.x) * EPT;
+    if (e0 >= nnz) return;
+    const int r0 = chunk * RC;
+    const int rn = min(RC, K - r0);
+    double bv[EPT][2], av[EPT][2];
+    int kk[EPT][2], nt[EPT];
+    bool simple = true;
+#pragma unroll
+    for (int u = 0; u < EPT; u++)
+    {
+        int64_t e = e0 + u;
+        nt[u] = 0;
+        if (e < nnz)
+        {
+            int64_t sa = ep[e];
+            nt[u] = (int) (ep[e + 1] - sa);
+            if (nt[u] > 2) simple = false;
+#pragma unroll
+            for (int j = 0; j < 2; j++)
+            {
+                if (j < nt[u])
+                {
+                    bv[u][j] = Lval[tB[sa + j]];
+                    av[u][j] = Ax[tA[sa + j]];
+                    kk[u][j] = tK[sa + j];
+                }
+                else
+                {
+                    bv[u][j] = 0.0;
+                    av[u][j] = 0.0;
+                    kk[u][j] = 0;
+                }
+            }
+        }
+    }
+    const bool full = (e0 + EPT <= nnz);
+    if (simple && rn == RC)
+    {
+        double dv[RC][EPT][2];
+#pragma unroll
+        for (int r = 0; r < RC; r++)
+        {
+            const double *dr = D + (int64_t) (r0 + r) * ldd;
+#pragma unroll
+            for (int u = 0; u < EPT; u++)
+#pragma unroll
+                for (int j = 0; j < 2; j++) dv[r][u][j] = (j < nt[u]) ? dr[kk[u][j]] : 0.0;
+        }
+#pragma unroll
+        for (int r = 0; r < RC; r++)
+        {
+            double out[EPT];
+#pragma unroll
+            for (int u = 0; u < EPT; u++)
+            {
+                double acc = 0.0;
+                if (nt[u] > 0) acc = acc + (bv[u][0] * dv[r][u][0]) * av[u][0];
+                if (nt[u] > 1) acc = acc + (bv[u][1] * dv[r][u][1]) * av[u][1];
+                out[u] = acc;
+            }
+            double *cr = C + (int64_t) (r0 + r) * ldc + e0;
+            if (EPT == 2 && full && ((ldc & 1) == 0))
+            {
+                dvec2 v2;
+                v2.x = out[0];
+                v2.y = out[EPT - 1];
+                __builtin_nontemporal_store(v2, (dvec2 *) cr);
+            }
+            else
+            {
+#pragma unroll
+                for (int u = 0; u < EPT; u++)
+                    if (e0 + u < nnz) __builtin_nontemporal_store(out[u], cr + u);
+            }
+        }
+        return;
+    }
+    // general lists / partial chunk
+    for (int r = r0; r < r0 + rn; r++)
+    {
+        const double *dr = D + (int64_t) r * ldd;
+#pragma unroll
+        for (int u = 0; u < EPT; u++)
+        {
+            int64_t e = e0 + u;
+            if (e >= nnz) break;
+            int64_t sa = ep[e];
+            double acc = 0.0;
+            for (int j = 0; j < nt[u]; j++)
+            {
+                int64_t q = sa + j;
+                acc = acc + (Lval[tB[q]] * dr[tK[q]]) * Ax[tA[q]];
+            }
+            __builtin_nontemporal_store(acc, C + (int64_t) r * ldc + e);
+        }
+    }
+}
+
+// Batch preparation for the interleaved refill.
+// k_gather_terms: each term's operand pair tv[s] = (L value, A value) in term
+// order (depends only on the operand values).
+// k_transpose_d: D ([K][ldd], refill-major) -> DT ([m][ldt], k-major: the K
+// values of one k contiguous), through a 16 x 64 LDS tile.
+constexpr int TR_K = 64;   // k per transpose tile
+constexpr int TR_R = 16;   // refills per transpose tile
+
+__global__ void __launch_bounds__(256)
+    k_gather_terms(int64_t terms, const int *__restrict__ tA, const int *__restrict__ tB,
+                   const double *__restrict__ Lval, const double *__restrict__ Ax,
+                   double *__restrict__ tv)
+{
+    int64_t s = (int64_t) blockIdx.x * 256 + threadIdx.x;
+    if (s >= terms) return;
+    dvec2 v;
+    v.x = Lval[tB[s]];
+    v.y = Ax[tA[s]];
+    *(dvec2 *) (tv + 2 * s) = v;
+}
+
+__global__ void __launch_bounds__(256)
+    k_transpose_d(int m, int K, const double *__restrict__ D, int64_t ldd, double *__restrict__ DT,
+                  int ldt)
+{
+    __shared__ double tile[TR_R][TR_K + 1];
+    const int nkt = (m + TR_K - 1) / TR_K;
+    const int kt = (int) blockIdx.x % nkt, rt = (int) blockIdx.x / nkt;
+    const int k0 = kt * TR_K, r0 = rt * TR_R;
+    for (int i = threadIdx.x; i < TR_R * TR_K; i += 256)
+    {
+        int rr = i / TR_K, kk = i % TR_K;
+        int r = r0 + rr, k = k0 + kk;
+        tile[rr][kk] = (r < K && k < m) ? D[(int64_t) r * ldd + k] : 0.0;
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < TR_R * TR_K; i += 256)
+    {
+        int kk = i / TR_R, rr = i % TR_R;
+        int r = r0 + rr, k = k0 + kk;
+        if (k < m && r < ldt) DT[(int64_t) k * ldt + r] = tile[rr][kk];
+    }
+}
+
+// Interleaved batched refill: grid over (entry tile, chunk of RC refills),
+// XCD-aware tile mapping; each term reads its RC values of d as one
+// contiguous, aligned run of DT.
+template <int RC>
+__global__ void __launch_bounds__(256)
+    k_num_pull_interleaved(int64_t nnz, int ntiles, int nchunks, const int64_t *__restrict__ ep,
+                           const int *__restrict__ tK, const double *__restrict__ tv,
+                           const double *__restrict__ DT, int ldt, int K,
+                           double *__restrict__ C, int64_t ldc)
+{
+    const int b = blockIdx.x;
+    const int xcd = b & 7, sl = b >> 3;
+    const int tile = (sl / nchunks) * 8 + xcd;
+    const int chunk = sl % nchunks;
+    if (tile >= ntiles) return;
+    const int64_t e = (int64_t) tile * 256 + threadIdx.x;
+    if (e >= nnz) return;
+    const int r0 = chunk * RC;
+    const int64_t s0 = ep[e], s1 = ep[e + 1];
+    double out[RC];
+#pragma unroll
+    for (int r = 0; r < RC; r++) out[r] = 0.0;
+    for (int64_t s = s0; s < s1; s++)
+    {
+        const int k = tK[s];
+        const dvec2 ba = *(const dvec2 *) (tv + 2 * s);
+        const dvec2 *dk = (const dvec2 *) (DT + (int64_t) k * ldt + r0);
+        dvec2 dv[RC / 2];
+#pragma unroll
+        for (int h = 0; h < RC / 2; h++) dv[h] = dk[h];
+#pragma unroll
+        for (int h = 0; h < RC / 2; h++)
+        {
+            out[2 * h] = out[2 * h] + (ba.x * dv[h].x) * ba.y;
+            out[2 * h + 1] = out[2 * h + 1] + (ba.x * dv[h].y) * ba.y;
+        }
+    }
+    const int rn = min(RC, K - r0);
+    double *cp = C + (int64_t) r0 * ldc + e;
+    if (rn == RC)
+    {
+#pragma unroll
+        for (int r = 0; r < RC; r++) __builtin_nontemporal_store(out[r], cp + (int64_t) r * ldc);
+    }
+    else
+    {
+        for (int r = 0; r < rn; r++) __builtin_nontemporal_store(out[r], cp + (int64_t) r * ldc);
+    }
+}
+
+typedef double dvec4 __attribute__((ext_vector_type(4)));
+
+// Interleaved batched refill with EPT adjacent entries per thread: one
+// EPT*8-byte store per refill (VEC: C rows are EPT-aligned).
+template <int RC, int EPT, bool VEC>
+__global__ void __launch_bounds__(256)
+    k_num_pull_interleaved_v(int64_t nnz, int ntiles, int nchunks,
+                             const int64_t *__restrict__ ep, const int *__restrict__ tK,
+                             const double *__restrict__ tv, const double *__restrict__ DT,
+                             int ldt, int K, double *__restrict__ C, int64_t ldc)
+{
+    const int b = blockIdx.x;
+    const int xcd = b & 7, sl = b >> 3;
+    const int tile = (sl / nchunks) * 8 + xcd;
+    const int chunk = sl % nchunks;
+    if (tile >= ntiles) return;
+    const int64_t e0 = ((int64_t) tile * 256 + threadIdx.x) * EPT;
+    if (e0 >= nnz) return;
+    const int r0 = chunk * RC;
+    int64_t sb[EPT + 1];
+#pragma unroll
+    for (int u = 0; u <= EPT; u++) sb[u] = ep[min(e0 + u, nnz)];
+    double out[EPT][RC];
+#pragma unroll
+    for (int u = 0; u < EPT; u++)
+#pragma unroll
+        for (int r = 0; r < RC; r++) out[u][r] = 0.0;
+#pragma unroll
+    for (int u = 0; u < EPT; u++)
+    {
+        for (int64_t s = sb[u]; s < sb[u + 1]; s++)
+        {
+            const int k = tK[s];
+            const dvec2 ba = *(const dvec2 *) (tv + 2 * s);
+            const dvec2 *dk = (const dvec2 *) (DT + (int64_t) k * ldt + r0);
+            dvec2 dv[RC / 2];
+#pragma unroll
+            for (int h = 0; h < RC / 2; h++) dv[h] = dk[h];
+#pragma unroll
+            for (int h = 0; h < RC / 2; h++)
+            {
+                out[u][2 * h] = out[u][2 * h] + (ba.x * dv[h].x) * ba.y;
+                out[u][2 * h + 1] = out[u][2 * h + 1] + (ba.x * dv[h].y) * ba.y;
+            }
+        }
+    }
+    const int rn = min(RC, K - r0);
+    double *cp = C + (int64_t) r0 * ldc + e0;
+    const bool full = e0 + EPT <= nnz;
+    if (VEC && full && rn == RC)
+    {
+#pragma unroll
+        for (int r = 0; r < RC; r++)
+        {
+            if (EPT == 4)
+            {
+                dvec4 v;
+                v.x = out[0][r];
+                v.y = out[1 % EPT][r];
+                v.z = out[2 % EPT][r];
+                v.w = out[3 % EPT][r];
+                __builtin_nontemporal_store(v, (dvec4 *) (cp + (int64_t) r * ldc));
+            }
+            else if (EPT == 2)
+            {
+                dvec2 v;
+                v.x = out[0][r];
+                v.y = out[1 % EPT][r];
+                __builtin_nontemporal_store(v, (dvec2 *) (cp + (int64_t) r * ldc));
+            }
+            else
+                __builtin_nontemporal_store(out[0][r], cp + (int64_t) r * ldc);
+        }
+    }
+    else
+    {
+        for (int r = 0; r < rn; r++)
+#pragma unroll
+            for (int u = 0; u < EPT; u++)
+                if (e0 + u < nnz) __builtin_nontemporal_store(out[u][r], cp + (int64_t) r * ldc + u);
     }
 }
 
@@ -845,23 +1138,167 @@ void product_numeric(const ProductPlan &P, const double *Lval, const double *d,
     SDL_HIP(hipGetLastError());
 }
 
+void product_bind_values(const ProductPlan &P, const double *Lval, const double *Ax,
+                         hipStream_t s)
+{
+    if (!P.pull || P.terms == 0 || !P.tv.p) return;
+    hipLaunchKernelGGL(k_gather_terms, dim3((unsigned) ((P.terms + 255) / 256)), dim3(256), 0, s,
+                       P.terms, P.tA.p, P.tB.p, Lval, Ax, const_cast<double *>(P.tv.p));
+    SDL_HIP(hipGetLastError());
+}
+
 void product_numeric_batched(const ProductPlan &P, const double *Lval, const double *D,
                              int64_t ldd, int K, const double *Ax, double *Cx, int64_t ldc,
-                             hipStream_t s)
+                             hipStream_t s, const RefillCfg *cfg, unsigned opts, void *ev_start,
+                             void *ev_stop)
 {
     if (P.nnz == 0 || K <= 0) return;
-    if (P.pull)
+    if (!P.pull)
+    {
+        for (int r = 0; r < K; r++)
+            product_numeric(P, Lval, D + (int64_t) r * ldd, Ax, Cx + (int64_t) r * ldc, s);
+        return;
+    }
+    RefillCfg c = cfg ? *cfg : (P.tuned.variant >= 0 ? P.tuned : default_refill_cfg(P, K));
+    if (c.variant >= 3 && K > P.batch_cap) c = RefillCfg{2, 8};
+    if (c.variant >= 3)
+    {
+        const int rc = c.rchunk == 8 ? 8 : 16;
+        const int ldt = (int) P.batch_ld;
+        if (!(opts & BATCH_VALUES_BOUND)) product_bind_values(P, Lval, Ax, s);
+        const double *DT = D;
+        int ldk = (int) ldd;
+        if (!(opts & BATCH_D_KMAJOR))
+        {
+            const int nkt = (int) ((P.A.Am + TR_K - 1) / TR_K);
+            const int nrt = (K + TR_R - 1) / TR_R;
+            hipLaunchKernelGGL(k_transpose_d, dim3((unsigned) (nkt * nrt)), dim3(256), 0, s,
+                               (int) P.A.Am, K, D, ldd, P.dT.p, ldt);
+            SDL_HIP(hipGetLastError());
+            DT = P.dT.p;
+            ldk = ldt;
+        }
+        if (ev_start) SDL_HIP(hipEventRecord((hipEvent_t) ev_start, s));
+        const int ept = c.variant == 4 ? 2 : (c.variant == 5 ? 4 : 1);
+        const int ntiles = (int) ((P.nnz + 256 * ept - 1) / (256 * ept));
+        const int nchunks = (K + rc - 1) / rc;
+        const int64_t blocks = ((int64_t) ntiles + 7) / 8 * 8 * nchunks;
+        const bool vec = ((uintptr_t) Cx % (8 * ept) == 0) && (ldc % ept == 0);
+#define SDL_IV(RC_, EPT_, VEC_)                                                                 \
+    if (rc == RC_ && ept == EPT_ && vec == VEC_)                                              \
+    {                                                                                         \
+        hipLaunchKernelGGL((k_num_pull_interleaved_v<RC_, EPT_, VEC_>), dim3((unsigned) blocks), \
+                           dim3(256), 0, s, P.nnz, ntiles, nchunks, P.ep.p, P.tK.p, P.tv.p,   \
+                           DT, ldk, K, Cx, ldc);                                          \
+        SDL_HIP(hipGetLastError());                                                           \
+        if (ev_stop) SDL_HIP(hipEventRecord((hipEvent_t) ev_stop, s));                       \
+        return;                                                                               \
+    }
+        if (ept > 1)
+        {
+            SDL_IV(8, 2, true)
+            SDL_IV(8, 2, false)
+            SDL_IV(16, 2, true)
+            SDL_IV(16, 2, false)
+            SDL_IV(8, 4, true)
+            SDL_IV(8, 4, false)
+            SDL_IV(16, 4, true)
+            SDL_IV(16, 4, false)
+        }
+#undef SDL_IV
+        if (rc == 8)
+            hipLaunchKernelGGL(k_num_pull_interleaved<8>, dim3((unsigned) blocks), dim3(256), 0, s,
+                               P.nnz, ntiles, nchunks, P.ep.p, P.tK.p, P.tv.p, DT, ldk, K, Cx,
+                               ldc);
+        else
+            hipLaunchKernelGGL(k_num_pull_interleaved<16>, dim3((unsigned) blocks), dim3(256), 0,
+                               s, P.nnz, ntiles, nchunks, P.ep.p, P.tK.p, P.tv.p, DT, ldk, K,
+                               Cx, ldc);
+        SDL_HIP(hipGetLastError());
+        if (ev_stop) SDL_HIP(hipEventRecord((hipEvent_t) ev_stop, s));
+        return;
+    }
+    if (c.variant == 0)
     {
         hipLaunchKernelGGL(k_num_pull_batched, dim3((unsigned) ((P.nnz + 255) / 256)),
                            dim3(256), 0, s, P.nnz, P.ep.p, P.tA.p, P.tB.p, P.tK.p, Lval, D,
                            ldd, K, Ax, Cx, ldc);
         SDL_HIP(hipGetLastError());
+        return;
     }
-    else
+    const int ept = c.variant == 2 ? 2 : 1;
+    const int rc = c.rchunk;
+    const int64_t per_tile = (int64_t) 256 * ept;
+    const int ntiles = (int) ((P.nnz + per_tile - 1) / per_tile);
+    const int nchunks = (K + rc - 1) / rc;
+    const int64_t tiles_pad = ((int64_t) ntiles + 7) / 8 * 8;
+    const int64_t blocks = tiles_pad * nchunks;
+    SDL_REQUIRE(blocks < (1ll << 31), "refill grid too large");
+#define SDL_RC_CASE(EPT, RC)                                                                 \
+    if (ept == EPT && rc == RC)                                                              \
+    {                                                                                        \
+        hipLaunchKernelGGL((k_num_pull_rchunk<EPT, RC>), dim3((unsigned) blocks), dim3(256), 0, s, \
+                           P.nnz, ntiles, nchunks, P.ep.p, P.tA.p, P.tB.p, P.tK.p, Lval, D, ldd, K, \
+                           Ax, Cx, ldc);                                                     \
+        SDL_HIP(hipGetLastError());                                                          \
+        return;                                                                              \
+    }
+    SDL_RC_CASE(1, 4)
+    SDL_RC_CASE(1, 8)
+    SDL_RC_CASE(1, 16)
+    SDL_RC_CASE(2, 4)
+    SDL_RC_CASE(2, 8)
+    SDL_RC_CASE(2, 16)
+#undef SDL_RC_CASE
+    SDL_REQUIRE(false, "unknown refill configuration");
+}
+
+RefillCfg default_refill_cfg(const ProductPlan &P, int K)
+{
+    RefillCfg c;
+    if (K <= P.batch_cap)
     {
-        for (int r = 0; r < K; r++)
-            product_numeric(P, Lval, D + (int64_t) r * ldd, Ax, Cx + (int64_t) r * ldc, s);
+        c.variant = 3;
+        c.rchunk = 16;
+        return c;
     }
+    c.variant = 2;
+    c.rchunk = K >= 16 ? 8 : 4;
+    return c;
+}
+
+void product_reserve_batch(ProductPlan &P, int Kmax)
+{
+    if (!P.pull || Kmax <= P.batch_cap) return;
+    if (!P.tv.p) P.tv.alloc((size_t) 2 * std::max<int64_t>(P.terms, 1));
+    int64_t ld = ((int64_t) Kmax + 15) / 16 * 16;
+    P.dT.alloc((size_t) std::max<int64_t>(P.A.Am, 1) * ld);
+    P.batch_ld = ld;
+    P.batch_cap = (int) ld;
+}
+
+std::vector<RefillCfg> refill_candidates()
+{
+    std::vector<RefillCfg> v;
+    RefillCfg c;
+    c.variant = 0;
+    c.rchunk = 0;
+    v.push_back(c);
+    for (int var = 1; var <= 2; var++)
+        for (int rc : {4, 8, 16})
+        {
+            c.variant = var;
+            c.rchunk = rc;
+            v.push_back(c);
+        }
+    for (int var = 3; var <= 5; var++)
+        for (int rc : {8, 16})
+        {
+            c.variant = var;
+            c.rchunk = rc;
+            v.push_back(c);
+        }
+    return v;
 }
 
 } // namespace sdl

@@ -11,6 +11,17 @@
 namespace sdl {
 
 namespace {
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+// Plain streaming copy (16 bytes per lane), the calibration kernel for the
+// HBM traffic counters: it moves exactly `bytes` in and `bytes` out.
+__global__ void k_stream_copy(const u32x4 *__restrict__ src, u32x4 *__restrict__ dst, size_t n)
+{
+    size_t i = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
+    size_t st = (size_t) gridDim.x * blockDim.x;
+    for (; i < n; i += st) dst[i] = src[i];
+}
+
 std::once_flag g_once;
 hipStream_t g_stream = nullptr;
 std::atomic<long long> g_dev_allocs{0};
@@ -171,6 +182,21 @@ float sdl_event_elapsed_ms(void *start, void *stop)
 void sdl_event_destroy(void *ev)
 {
     if (ev) SDL_HIP(hipEventDestroy((hipEvent_t) ev));
+}
+
+int sdl_copy_kernel(void *dst, const void *src, size_t bytes, void *stream)
+{
+    if (bytes % 16 != 0 || ((uintptr_t) dst % 16) || ((uintptr_t) src % 16))
+    {
+        sdl::set_error("sdl_copy_kernel: 16-byte aligned sizes and pointers only");
+        return 1;
+    }
+    size_t n = bytes / 16;
+    hipLaunchKernelGGL(sdl::k_stream_copy, dim3(8192), dim3(256), 0,
+                       stream ? (hipStream_t) stream : sdl::stream(), (const sdl::u32x4 *) src,
+                       (sdl::u32x4 *) dst, n);
+    SDL_HIP(hipGetLastError());
+    return 0;
 }
 
 int sdl_device_name(char *buf, int len)

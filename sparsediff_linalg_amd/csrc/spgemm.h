@@ -5,6 +5,8 @@
 
 #include "sdl_internal.h"
 
+#include <vector>
+
 namespace sdl {
 
 // Rows of a compressed matrix on the device: row r has entries
@@ -42,6 +44,19 @@ struct SymStats
     long long predicted_rows = 0;      // predicted path: rows served by the prediction
 };
 
+// Launch configuration of the batched pull refill (picked by the autotuner):
+// variant 0 = one thread per entry over all K refills; 1 = r-chunked grid, one
+// entry per thread; 2 = r-chunked grid, two entries per thread (16-byte stores);
+// 3 = d transposed to k-major and term operands pre-gathered (needs the
+// reserved workspace), r-chunked grid.
+struct RefillCfg
+{
+    int variant = -1;
+    int rchunk = 8;
+    RefillCfg() = default;
+    RefillCfg(int v, int r) : variant(v), rchunk(r) {}
+};
+
 struct ProductPlan
 {
     LView L;
@@ -58,7 +73,16 @@ struct ProductPlan
     DBuf<int> rbins;                 // row ids grouped by class
     int rbin_off[8] = {0};           // class c is rbins[rbin_off[c] .. rbin_off[c+1])
     SymStats stats;
+    RefillCfg tuned;   // variant < 0: not tuned, use the default heuristic
+    // Batched-refill workspace (reserved in the alloc phase): term operand pairs
+    // in term order, and D transposed to k-major with row stride batch_ld.
+    DBuf<double> tv, dT;
+    int batch_cap = 0;
+    int64_t batch_ld = 0;
 };
+
+// Reserve the interleaved batched-refill workspace for up to Kmax refills.
+void product_reserve_batch(ProductPlan &P, int Kmax);
 
 // Build the exact structure and the refill schedule.  pred (optional, device,
 // m_out floats) holds predicted per-row nnz for SymMode::Predicted.
@@ -71,9 +95,25 @@ void product_numeric(const ProductPlan &P, const double *Lval, const double *d,
                      const double *Ax, double *Cx, hipStream_t s);
 
 // K refills sharing operands: refill r reads d + r*ldd and writes Cx + r*ldc.
+// opts: BATCH_VALUES_BOUND = the term operand pairs were gathered by
+// product_bind_values for the current operand values (skip the gather);
+// BATCH_D_KMAJOR = D is already k-major (D[k*ldd + r]).  ev_start / ev_stop
+// (hipEvent_t, optional) are recorded around the main refill kernel.
+enum : unsigned
+{
+    BATCH_VALUES_BOUND = 1u,
+    BATCH_D_KMAJOR = 2u
+};
 void product_numeric_batched(const ProductPlan &P, const double *Lval, const double *D,
                              int64_t ldd, int K, const double *Ax, double *Cx, int64_t ldc,
-                             hipStream_t s);
+                             hipStream_t s, const RefillCfg *cfg = nullptr, unsigned opts = 0,
+                             void *ev_start = nullptr, void *ev_stop = nullptr);
+// Gather the term operand pairs for the interleaved refill (needs the reserved
+// workspace; no-op otherwise).
+void product_bind_values(const ProductPlan &P, const double *Lval, const double *Ax,
+                         hipStream_t s);
+RefillCfg default_refill_cfg(const ProductPlan &P, int K);
+std::vector<RefillCfg> refill_candidates();
 
 // Number of terms sum_r sum_{q in L_r} len(A_k); device pass, synchronises.
 int64_t product_count_terms(const LView &L, const AView &A, int64_t *row_terms_out);
