@@ -46,6 +46,9 @@ def parse():
                     help="approximate CPU time of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--plain-steps", type=int, default=10)
+    ap.add_argument("--lean", action="store_true",
+                    help="counter-collection mode: pinned refill configuration, no autotune, "
+                         "no plain path, no spot check")
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="also run the streaming copy kernel on this many MiB (HBM counter "
                          "calibration under rocprofv3 --pmc)")
@@ -91,8 +94,13 @@ def main():
     # autotune the refill launch shape on this plan (alloc-phase work)
     best = sdl_refill_cfg()
     best_us = C.c_float()
-    check(lib.sdl_btda_autotune(h, dAx.ptr, dBx.ptr, pools[0].ptr, m, K, dC.ptr, ldc, 10,
-                                C.byref(best), C.byref(best_us)), "autotune")
+    if args.lean:
+        best.variant, best.rchunk = 3, 8
+        check(lib.sdl_btda_set_refill_cfg(h, C.byref(best)), "set cfg")
+        args.plain_steps = 0
+    else:
+        check(lib.sdl_btda_autotune(h, dAx.ptr, dBx.ptr, pools[0].ptr, m, K, dC.ptr, ldc, 10,
+                                    C.byref(best), C.byref(best_us)), "autotune")
 
     stream = lib.sdl_library_stream()
     ev = [(lib.sdl_event_create(), lib.sdl_event_create()) for _ in range(args.steps)]
@@ -122,14 +130,14 @@ def main():
     D_last = pools[(args.steps - 1) % NPOOL]
     out_batch = dC.numpy()[:, :nnz]
     one = DeviceArray(max(nnz, 1), np.float64)
-    for r in (0, K - 1):
+    for r in (() if args.lean else (0, K - 1)):
         check(lib.sdl_btda_refill(h, dAx.ptr, dBx.ptr, D_last.offset(r * m), one.ptr, stream))
         lib.sdl_synchronize()
         assert np.array_equal(one.numpy()[:nnz].view(np.int64), out_batch[r].view(np.int64)), \
             "batched refill differs from the single refill"
 
     # plain per-call GPU path: one launch per d
-    for i in range(2):
+    for i in range(2 if args.plain_steps > 0 else 0):
         for r in range(K):
             lib.sdl_btda_refill(h, dAx.ptr, dBx.ptr, pools[0].offset(r * m), dC.offset(r * ldc),
                                 stream)
@@ -140,7 +148,7 @@ def main():
             lib.sdl_btda_refill(h, dAx.ptr, dBx.ptr, D.offset(r * m), dC.offset(r * ldc), stream)
     lib.sdl_event_record(e_all1, stream)
     lib.sdl_synchronize()
-    plain_ms_batch = lib.sdl_event_elapsed_ms(e_all0, e_all1) / args.plain_steps
+    plain_ms_batch = lib.sdl_event_elapsed_ms(e_all0, e_all1) / max(args.plain_steps, 1)
 
     if args.calib_copy_mb > 0:
         nb = args.calib_copy_mb << 20

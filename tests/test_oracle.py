@@ -1,0 +1,270 @@
+"""CPU pinning of the oracle (oracle/sdl_oracle.c):
+
+* every known-answer vector of the reference's own tests (tests/golden),
+* bit-exact agreement with the reference's sparse sources built in
+  oracle/_ref (same inputs, its Gustavson / transposition / row_reduce),
+* closed forms on tiny inputs: a dense boolean product for the structure and
+  a per-entry left fold in ascending k for the values (Python floats are
+  individually rounded IEEE doubles, so the fold is exact), a dense
+  scatter-add for coalescing, double transposition = identity.
+"""
+import ctypes as C
+import json
+import os
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tests.helpers import bits_equal, random_csr
+
+GOLD = json.load(open(os.path.join(os.path.dirname(__file__), "golden",
+                                   "reference_literals.json")))
+
+
+def _arr(v, dt):
+    return np.array(v, dt)
+
+
+# ------------------------------------------------------------ literals ----
+@pytest.mark.parametrize("case", GOLD["spgemm"], ids=lambda c: c["name"])
+def test_spgemm_literals(case):
+    L, A = case["L"], case["A"]
+    Cp, Ci, Cx, _ = O.spgemm(L["m"], L["p"], L["i"], L["x"], case["blocks"], case["d"],
+                             A["m"], A["n"], A["p"], A["i"], A["x"])
+    assert list(Cp) == case["C"]["p"]
+    assert list(Ci) == case["C"]["i"]
+    if case["C"]["x"] is not None:
+        assert list(Cx) == case["C"]["x"]
+
+
+@pytest.mark.parametrize("case", GOLD["ata"], ids=lambda c: c["name"])
+def test_ata_literals(case):
+    Ac = case["A_csc"]
+    m, n = Ac["m"], Ac["n"]
+    # A's CSR from its CSC (transpose of the n x m compressed columns)
+    Ap, Ai, Ax = O.transpose(n, m, Ac["p"], Ac["i"], np.array(Ac["x"], float))
+    Cp, Ci, Cx, _ = O.spgemm(n, Ac["p"], Ac["i"], np.array(Ac["x"], float), 1, case["d"], m, n,
+                             Ap, Ai, Ax)
+    assert list(Cp) == case["C"]["p"]
+    assert list(Ci) == case["C"]["i"]
+    if case["C"]["x"] is not None:
+        exp = np.array(case["C"]["x"])
+        if case["tol"] == 0:
+            assert list(Cx) == list(exp)
+        else:  # reference's is_equal_double (tests/test_helpers.c:10-18)
+            assert np.all(np.abs(Cx - exp) <= np.maximum(case["tol"],
+                                                         case["tol"] * np.maximum(abs(Cx), abs(exp))))
+
+
+def test_btda_literal():
+    case = GOLD["btda"][0]
+    A, B = case["A"], case["B"]
+    Cp, Ci, Cx, _ = O.btda(A["m"], A["n"], B["n"], A["p"], A["i"], A["x"], B["p"], B["i"],
+                           B["x"], case["d"])
+    assert list(Cp) == case["C"]["p"] and list(Ci) == case["C"]["i"]
+    assert list(Cx) == case["C"]["x"]
+
+
+@pytest.mark.parametrize("case", GOLD["transpose"], ids=lambda c: c["name"])
+def test_transpose_literals(case):
+    A = case["A"]
+    Tp, Ti, Tx = O.transpose(A["m"], A["n"], A["p"], A["i"], np.array(A["x"], float))
+    if case["AT"] is not None:
+        assert list(Tp) == case["AT"]["p"] and list(Ti) == case["AT"]["i"]
+        assert list(Tx) == case["AT"]["x"]
+    # round trip (test_csr_csc_conversion.h:127-171)
+    Rp, Ri, Rx = O.transpose(A["n"], A["m"], Tp, Ti, Tx)
+    assert list(Rp) == A["p"] and list(Ri) == A["i"] and list(Rx) == list(map(float, A["x"]))
+
+
+def test_coo_literals():
+    c0 = GOLD["coo"][0]
+    A = c0["A"]
+    # new_COO_matrix is an expansion: rows from p, cols/x copied
+    rows = np.repeat(np.arange(A["m"]), np.diff(A["p"]))
+    assert list(rows) == c0["rows"]
+    c1 = GOLD["coo"][1]
+    A = c1["A"]
+    r, c, x, vm = O.coo_lower(A["m"], A["p"], A["i"], np.array(A["x"], float))
+    low = c1["lower"]
+    assert list(r) == low["rows"] and list(c) == low["cols"]
+    assert list(x) == list(map(float, low["x"])) and list(vm) == low["value_map"]
+    assert list(np.array(c1["refresh_vals"], float)[vm]) == list(map(float, c1["refresh_x"]))
+
+
+def test_row_reduce_literal():
+    case = GOLD["row_reduce"][0]
+    A = case["A"]
+    x = np.array(A["x"], float)
+    Cp, Ci, Cx, mp = O.row_reduce(A["m"], A["n"], A["p"], A["i"], x, case["group"],
+                                  case["m_out"])
+    assert list(Cp) == case["C"]["p"] and list(Ci) == case["C"]["i"]
+    assert list(Cx) == case["C"]["x"]
+    _, _, Cx2, _ = O.row_reduce(A["m"], A["n"], A["p"], A["i"], x * case["refill_scale"],
+                                case["group"], case["m_out"])
+    assert list(Cx2) == [v * case["refill_scale"] for v in case["C"]["x"]]
+
+
+# ------------------------------------------------- closed forms (tiny) ----
+def dense_fold(Lm, Lp, Li, Lx, blocks, d, Am, An, Ap, Ai, Ax):
+    """C[i, j] = left fold over ascending k of (L[i,k] d[k]) A[k,j], built from
+    the definition with a per-entry list; structure = boolean product."""
+    nL = Am // blocks
+    out = {}
+    for b in range(blocks):
+        for r in range(Lm):
+            row = b * Lm + r
+            terms = {}
+            for q in range(Lp[r], Lp[r + 1]):
+                k = Li[q] + b * nL
+                v = float(Lx[q]) if d is None else float(Lx[q]) * float(d[k])
+                for t in range(Ap[k], Ap[k + 1]):
+                    terms.setdefault(int(Ai[t]), []).append((k, v * float(Ax[t])))
+            for j, lst in terms.items():
+                acc = 0.0
+                for _, p in sorted(lst):
+                    acc = acc + p
+                out[(row, j)] = acc
+    return out
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_spgemm_closed_form(seed):
+    rng = np.random.default_rng(seed)
+    blocks = 1 + seed % 3
+    Lm, nL, An = 7 + seed, 6 + seed % 4, 9
+    Lp, Li, Lx = random_csr(rng, Lm, nL, 0.4, dense_col=1, empty_row=2)
+    Ap, Ai, Ax = random_csr(rng, nL * blocks, An, 0.35, empty_col=3)
+    d = rng.uniform(0.1, 10.0, nL * blocks) if seed % 2 else None
+    Cp, Ci, Cx, _ = O.spgemm(Lm, Lp, Li, Lx, blocks, d, nL * blocks, An, Ap, Ai, Ax)
+    ref = dense_fold(Lm, Lp, Li, Lx, blocks, d, nL * blocks, An, Ap, Ai, Ax)
+    got = {}
+    for row in range(Lm * blocks):
+        cols = list(Ci[Cp[row]:Cp[row + 1]])
+        assert cols == sorted(set(cols))
+        for e in range(Cp[row], Cp[row + 1]):
+            got[(row, int(Ci[e]))] = float(Cx[e])
+    assert set(got) == set(ref)
+    for key, v in ref.items():
+        assert np.float64(got[key]).view(np.int64) == np.float64(v).view(np.int64), key
+
+
+def test_plan_and_accumulator_bit_identical():
+    # gustavson.h:60-61 / test_linalg_sparse_matmuls.h:663-755
+    rng = np.random.default_rng(3)
+    Lp, Li, Lx = random_csr(rng, 30, 25, 0.3, dense_col=0)
+    Ap, Ai, Ax = random_csr(rng, 25, 40, 0.2, dense_col=5)
+    d = rng.uniform(0.1, 10, 25)
+    _, _, X1, _ = O.spgemm(30, Lp, Li, Lx, 1, d, 25, 40, Ap, Ai, Ax, planned=True)
+    _, _, X2, _ = O.spgemm(30, Lp, Li, Lx, 1, d, 25, 40, Ap, Ai, Ax, planned=False)
+    assert bits_equal(X1, X2)
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_coo_coalesce_closed_form(seed):
+    rng = np.random.default_rng(seed)
+    m, n, nnz = 13, 11, 200
+    rows = rng.integers(0, m, nnz).astype(np.int32)
+    cols = rng.integers(0, n, nnz).astype(np.int32)
+    x = rng.standard_normal(nnz)
+    Cp, Ci, Cx = O.coo_coalesce(rows, cols, x, m, n)
+    sums = {}
+    for t in range(nnz):  # input order, starting from +0.0
+        key = (int(rows[t]), int(cols[t]))
+        sums[key] = sums.get(key, 0.0) + float(x[t])
+    got = {(r, int(Ci[e])): float(Cx[e]) for r in range(m) for e in range(Cp[r], Cp[r + 1])}
+    assert got == sums
+    for r in range(m):
+        seg = list(Ci[Cp[r]:Cp[r + 1]])
+        assert seg == sorted(set(seg))
+    # equals row_reduce with every triplet a one-entry row grouped by its row
+    Ap = np.arange(nnz + 1, dtype=np.int32)
+    Rp, Ri, Rx, _ = O.row_reduce(nnz, n, Ap, cols, x, rows, m)
+    assert list(Rp) == list(Cp) and list(Ri) == list(Ci) and bits_equal(Rx, Cx)
+
+
+def test_kron_spmv_closed_form():
+    rng = np.random.default_rng(5)
+    m, n, p = 9, 7, 3
+    Ap, Ai, Ax = random_csr(rng, m, n, 0.4)
+    x = rng.standard_normal(n * p)
+    y = O.kron_spmv(m, n, Ap, Ai, Ax, x, p)
+    for b in range(p):
+        for i in range(m):
+            s = 0.0
+            for t in range(Ap[i], Ap[i + 1]):
+                s = s + float(Ax[t]) * float(x[b * n + Ai[t]])
+            assert y[b * m + i] == s
+
+
+def test_da_and_gather_closed_form():
+    rng = np.random.default_rng(8)
+    Ap, Ai, Ax = random_csr(rng, 10, 6, 0.5)
+    d = rng.uniform(0.1, 10, 10)
+    Cx = O.da(10, Ap, Ax, d)
+    rows = np.repeat(np.arange(10), np.diff(Ap))
+    assert bits_equal(Cx, Ax * d[rows])
+    rmap = np.array([3, 3, 0, 9, 5], np.int32)
+    Gp, Gi, Gx = O.row_gather(Ap, Ai, Ax, rmap)
+    ref_i = np.concatenate([Ai[Ap[r]:Ap[r + 1]] for r in rmap])
+    ref_x = np.concatenate([Ax[Ap[r]:Ap[r + 1]] for r in rmap])
+    assert list(Gi) == list(ref_i) and bits_equal(Gx, ref_x)
+
+
+# -------------------------------------------------- vs oracle/_ref ----
+needs_ref = pytest.mark.skipif(not O.ref_available(),
+                               reason="oracle/_ref not built (reference tree absent)")
+
+
+@needs_ref
+@pytest.mark.parametrize("seed", range(5))
+def test_btda_matches_reference_build(seed):
+    rng = np.random.default_rng(100 + seed)
+    m, nA, nB = 60 + 20 * seed, 40, 35
+    Ap, Ai, Ax = random_csr(rng, m, nA, 0.08 + 0.04 * seed, dense_col=2, empty_row=1)
+    Bp, Bi, Bx = random_csr(rng, m, nB, 0.1, empty_col=4)
+    d = rng.uniform(0.1, 10, m)
+    Cp, Ci, Cx, _ = O.btda(m, nA, nB, Ap, Ai, Ax, Bp, Bi, Bx, d)
+    Rp, Ri, Rx = O.ref_btda(m, nA, nB, Ap, Ai, Ax, Bp, Bi, Bx, d)
+    assert list(Cp) == list(Rp) and list(Ci) == list(Ri) and bits_equal(Cx, Rx)
+
+
+@needs_ref
+def test_config1_matches_reference_build():
+    from sparsediff_linalg_amd import workloads
+    w = workloads.btda_config1(1)
+    Cp, Ci, Cx, terms = O.btda(w["m"], w["nA"], w["nB"], w["Ap"], w["Ai"], w["Ax"], w["Bp"],
+                               w["Bi"], w["Bx"], w["d"])
+    Rp, Ri, Rx = O.ref_btda(w["m"], w["nA"], w["nB"], w["Ap"], w["Ai"], w["Ax"], w["Bp"],
+                            w["Bi"], w["Bx"], w["d"])
+    assert terms == 720000 and len(Ci) == 709850
+    assert np.array_equal(Cp, Rp) and np.array_equal(Ci, Ri) and bits_equal(Cx, Rx)
+
+
+# ------------------------------------------------ the product library ----
+def _declared_functions():
+    import re
+    inc = os.path.join(os.path.dirname(os.path.dirname(__file__)), "include")
+    names = set()
+    for root, _, files in os.walk(inc):
+        for f in files:
+            if not f.endswith(".h"):
+                continue
+            txt = open(os.path.join(root, f)).read()
+            txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+            for mt in re.finditer(r"^[A-Za-z_][\w \*]*?\b([A-Za-z_]\w*)\s*\(", txt, flags=re.M):
+                name = mt.group(1)
+                if name in ("if", "return", "sizeof") or txt[mt.start():].startswith("static"):
+                    continue
+                if txt[mt.start():mt.start() + 7] == "typedef":
+                    continue
+                names.add(name)
+    return names
+
+
+def test_library_exports_every_declared_symbol():
+    from sparsediff_linalg_amd import LIB_PATH
+    lib = C.CDLL(LIB_PATH, mode=os.RTLD_LOCAL)
+    missing = [n for n in sorted(_declared_functions()) if not hasattr(lib, n)]
+    assert not missing, missing
