@@ -46,12 +46,5 @@ template <typename T> T *attached_as(const void *owner, const char *fn)
 int *host_ints(int64_t n, bool zero);
 double *host_doubles(int64_t n);
 
-// Device copy of host data (alloc phase).
-template <typename T> DBuf<T> upload(const T *h, int64_t n)
-{
-    DBuf<T> d(n > 0 ? n : 0);
-    if (n > 0) h2d(d.p, h, (size_t) n * sizeof(T));
-    return d;
-}
 
 } // namespace sdl

@@ -47,6 +47,10 @@ struct SparseM
     bool has_csc = false;
     DBuf<int> cp, ci, cperm;
     std::unique_ptr<Bound> bound;
+    // CSR view of a permuted_dense / stacked_pd owner: device values come
+    // from the owner (refreshed by ext_values), the view is freed with it.
+    const double *(*ext_values)(SparseM *) = nullptr;
+    void *ext_owner = nullptr;
 };
 
 SparseM *as_sparse(const matrix *A);

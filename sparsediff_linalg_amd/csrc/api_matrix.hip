@@ -242,7 +242,7 @@ void sparse_ensure_dev(SparseM *S)
     const CSR_matrix *A = S->csr;
     S->dp = upload(A->p, (int64_t) A->m + 1);
     S->di = upload(A->i, A->nnz);
-    S->val.x.alloc(A->nnz);
+    S->val.x.alloc(S->ext_values ? 0 : A->nnz);
     S->val.valid = false;
     S->dev = true;
 }
@@ -270,6 +270,7 @@ void sparse_ensure_rows(SparseM *S)
 
 const double *sparse_dev_values(SparseM *S)
 {
+    if (S->ext_values) return S->ext_values(S);
     if (!S->dev) api_exit("fill", "operand has no device mirror (was it given to the alloc?)");
     if (!S->val.valid || S->val.seen != S->base.values_version)
     {
@@ -346,7 +347,6 @@ void DA_matrices_fill_values(const double *d, const matrix *A, matrix *C)
     if (A->kind != MATRIX_SPARSE)
     {
         pd_DA_fill(d, A, C);
-        matrix_values_changed(C);
         return;
     }
     SparseM *SA = as_sparse(A), *SC = as_sparse(C);
@@ -382,7 +382,6 @@ void transpose_matrices_fill_values(const matrix *A, matrix *AT)
     if (A->kind != MATRIX_SPARSE)
     {
         pd_transpose_fill(A, AT);
-        matrix_values_changed(AT);
         return;
     }
     SparseM *S = as_sparse(A), *C = as_sparse(AT);
@@ -416,7 +415,6 @@ static void btda_fill(const char *fn, matrix *A, const double *d, matrix *B, mat
         return;
     }
     pd_BTDA_fill(A, d, B, C);
-    matrix_values_changed(C);
 }
 
 void BTDA_matrices_fill_values(matrix *A, const double *d, matrix *B, matrix *C)
@@ -444,7 +442,6 @@ void ATDA_matrices_fill_values(const matrix *A, const double *d, matrix *C)
     if (A->kind != MATRIX_SPARSE)
     {
         pd_ATDA_fill(A, d, C);
-        matrix_values_changed(C);
         return;
     }
     SparseM *S = as_sparse(A);
@@ -479,7 +476,6 @@ void BA_matrices_fill_values(const matrix *B, const matrix *A, matrix *C)
         return;
     }
     pd_BA_fill(B, A, C);
-    matrix_values_changed(C);
 }
 
 // -------------------------------------------------------------- kron ----
@@ -522,7 +518,6 @@ void BA_kron_matrices_fill_values(const matrix *A, int p, matrix *J, matrix *C)
     if (A->kind == MATRIX_PERMUTED_DENSE)
     {
         pd_dense_kron_fill(A, p, J, C, false);
-        matrix_values_changed(C);
         return;
     }
     SparseM *SA = as_sparse(A);
@@ -539,7 +534,6 @@ matrix *BA_dense_kron_I_matrices_alloc(const matrix *A, int m, matrix *J)
 void BA_dense_kron_I_matrices_fill_values(const matrix *A, int m, const matrix *J, matrix *C)
 {
     pd_dense_kron_fill(A, m, const_cast<matrix *>(J), C, true);
-    matrix_values_changed(C);
 }
 
 // ------------------------------------------------------------- vstack ----

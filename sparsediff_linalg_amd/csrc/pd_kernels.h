@@ -1,0 +1,97 @@
+// Generic device kernels behind the permuted_dense / stacked_pd kinds.  Every
+// PD/SPD operation is an alloc-time descriptor set run by one of them:
+//   * pull_sum:  out[e] = ((0 + src[idx[s0]]) + src[idx[s0+1]]) + ... (ordered);
+//   * sdmm:      sparse x dense with per-list ordered sums (pd x csc, csc x pd,
+//                pd x csr families);
+//   * ggemm:     grouped gathered GEMM on FP64 MFMA (every dense-block product).
+#pragma once
+
+#include "sdl_internal.h"
+
+#include <vector>
+
+namespace sdl {
+
+// ---------------------------------------------------------------- pull ----
+struct PullList
+{
+    int n = 0;           // outputs
+    DBuf<int> seg, idx;  // seg[n+1]; idx: source positions in order
+};
+void pull_build(PullList &P, const std::vector<int> &seg, const std::vector<int> &idx);
+void pull_sum(const PullList &P, const double *src, double *out, hipStream_t s);
+
+// ---------------------------------------------------------------- sdmm ----
+// Output list L (a row or a column of the output block) sums its entries in
+// order: entry t = (dense index dr, sparse value position vp, d index dk or -1).
+// dense(c) = D[dr*ld + c] (or D[c*ld + dr] when dense_trans).
+// assoc 0: term = (d[dk] * S[vp]) * dense(c);  assoc 1: term = S[vp] * (d[dk] * dense(c)).
+// C[L*sL + c*sC] for c < NC.
+struct SdmmPlan
+{
+    int NL = 0, NC = 0;
+    int ld = 0;
+    bool dense_trans = false;
+    int assoc = 0;
+    long long sL = 0, sC = 0;
+    DBuf<int> lp;                 // NL + 1
+    DBuf<int> dr, vp, dk;         // entries
+    bool has_dk = false;
+};
+void sdmm_build(SdmmPlan &P, const std::vector<int> &lp, const std::vector<int> &dr,
+                const std::vector<int> &vp, const std::vector<int> &dk);
+void sdmm_run(const SdmmPlan &P, const double *Sx, const double *D, const double *d,
+              double *C, hipStream_t s);
+
+// ---------------------------------------------------------------- ggemm ----
+// C[i][j] = sum over the job's tasks (in order) of sum_s P_t(s, i) * Q_t(s, j),
+//   P_t(s,i) = p_mode 0: P[p_off + prow[s]*p_ld + pmap(i)]  |  1: P[p_off + pmap(i)*p_ld + prow[s]]
+//   Q_t(s,j) = w(s) * (q_mode 0: Q[q_off + qrow[s]*q_ld + qmap(j)] | 1: Q[q_off + qmap(j)*q_ld + qrow[s]])
+// with w(s) = d[widx[s]] (scaled before the product) when widx >= 0; map
+// entries < 0 contribute zero; a map offset < 0 is the identity.  P and Q
+// read source buffer 0 or 1 (p_buf / q_buf).
+struct GTask
+{
+    long long p_off, q_off;
+    int p_buf, q_buf;
+    int p_ld, q_ld;
+    int p_mode, q_mode;
+    int S;
+    int prow, pmap;
+    int qrow, qmap;
+    int widx;
+};
+
+struct GJob
+{
+    long long c_off;
+    int ldc, M, N;
+    int i0, j0;
+    int t0, t1;
+};
+
+struct GemmPlan
+{
+    int njobs = 0;
+    long long flops = 0;   // 2 * sum over tasks of S * M * N (algorithmic)
+    DBuf<GTask> tasks;
+    DBuf<GJob> jobs;
+    DBuf<int> pool;
+};
+
+// Host-side builder: add output blocks and their tasks, then upload.
+struct GemmBuilder
+{
+    std::vector<GTask> tasks;
+    std::vector<GJob> jobs;
+    std::vector<int> pool;
+    long long flops = 0;
+    int add_ints(const std::vector<int> &v);  // returns the offset
+    // One output block (M x N at c_off, row stride ldc) fed by tasks [t0, t1).
+    void add_output(long long c_off, int ldc, int M, int N, int t0, int t1);
+};
+void gemm_upload(GemmPlan &P, const GemmBuilder &B);
+void gemm_run(const GemmPlan &P, const double *buf0, const double *buf1, const double *d,
+              double *C, hipStream_t s);
+
+} // namespace sdl

@@ -80,6 +80,14 @@ template <typename T> struct DBuf
     size_t bytes() const { return n * sizeof(T); }
 };
 
+// Device copy of host data (alloc phase).
+template <typename T> DBuf<T> upload(const T *h, int64_t n)
+{
+    DBuf<T> d(n > 0 ? n : 0);
+    if (n > 0) h2d(d.p, h, (size_t) n * sizeof(T));
+    return d;
+}
+
 // ----------------------------------------------------------- primitives -----
 // Exclusive scan of n int32 or int64 counts into int64 offsets; out has n+1
 // entries and out[n] is the total.  Scratch is allocated internally (alloc
