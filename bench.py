@@ -49,6 +49,9 @@ def parse():
     ap.add_argument("--lean", action="store_true",
                     help="counter-collection mode: pinned refill configuration, no autotune, "
                          "no plain path, no spot check")
+    ap.add_argument("--workload", default="btda", choices=["btda", "blocks-atda", "blocks-btda"],
+                    help="btda: the sparse refill (default line); blocks-*: the stacked "
+                         "permuted-dense products (A^T D A, or B^T D A of two stacks)")
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="also run the streaming copy kernel on this many MiB (HBM counter "
                          "calibration under rocprofv3 --pmc)")
@@ -62,6 +65,8 @@ def main():
               "the sparse product is sharded by rows/instances in the design but no multi-GPU "
               "path is built.", file=sys.stderr)
         sys.exit(2)
+    if args.workload != "btda":
+        return main_blocks(args)
     import sparsediff_linalg_amd as sdl
     from sparsediff_linalg_amd import DeviceArray, check, workloads
     from sparsediff_linalg_amd._lib import sdl_refill_cfg
@@ -223,6 +228,164 @@ def main():
     }
     lib.sdl_btda_plan_destroy(h)
     print(json.dumps(out), flush=True)
+
+
+FP64_MFMA_PEAK_TFS = 78.6
+
+
+def main_blocks(args):
+    """Stacked permuted-dense products: 256 dense FP64 blocks, m0 and n0
+    log-uniform on [64, 512], row partition, col_perms n0-subsets of [0, 4096).
+    One step = one device-resident refill of the product for a fresh d (the
+    grouped MFMA GEMM over all block pairs plus the ordered coalesce)."""
+    import sparsediff_linalg_amd as sdl
+    from sparsediff_linalg_amd import DeviceArray, check, workloads
+    from sparsediff_linalg_amd._lib import MAT_p
+
+    lib = sdl.load()
+    w = workloads.blocks_config3(3)
+    m, N = w["m"], w["n"]
+    btda = args.workload == "blocks-btda"
+
+    def stack(blocks):
+        hs = []
+        for rp, cp, X in blocks:
+            X = np.ascontiguousarray(X)
+            hs.append(lib.new_permuted_dense(m, N, len(rp), len(cp), sdl.ptr(rp), sdl.ptr(cp),
+                                             sdl.ptr(X)))
+        arr = (MAT_p * len(hs))(*hs)
+        return lib.new_stacked_pd(m, N, len(hs), C.cast(arr, C.POINTER(MAT_p)), None, None)
+
+    A = stack(w["A"])
+    B = stack(w["B"]) if btda else None
+    d = w["d"]
+    t0 = time.perf_counter()
+    Cm = lib.BTA_matrices_alloc(A, B) if btda else lib.ATA_matrices_alloc(A)
+    t_alloc = time.perf_counter() - t0
+    left = B if btda else A
+
+    def dropin_fill(dv):
+        if btda:
+            lib.BTDA_matrices_fill_values(A, sdl.ptr(dv), B, Cm)
+        else:
+            lib.ATDA_matrices_fill_values(A, sdl.ptr(dv), Cm)
+
+    dropin_fill(d)  # uploads the operand mirrors once
+    t0 = time.perf_counter()
+    dropin_fill(d)
+    t_dropin = time.perf_counter() - t0
+
+    stats = (C.c_double * 8)()
+    check(lib.sdl_block_stats(Cm, stats), "stats")
+    flops = stats[0] + stats[1]
+    out_cells = int(stats[5])
+    NPOOL = 4
+    pools = [DeviceArray.from_numpy(workloads.uniform(20_000 + p, m, 0.1, 10.0)) for p in range(NPOOL)]
+    stream = lib.sdl_library_stream()
+    ev = [(lib.sdl_event_create(), lib.sdl_event_create()) for _ in range(args.steps)]
+    e0, e1 = lib.sdl_event_create(), lib.sdl_event_create()
+    for i in range(args.warmup):
+        check(lib.sdl_block_refill_device(Cm, left, A, pools[i % NPOOL].ptr, None, None), "refill")
+    lib.sdl_synchronize()
+    lib.sdl_event_record(e0, stream)
+    for i in range(args.steps):
+        a, b = ev[i]
+        check(lib.sdl_block_refill_device(Cm, left, A, pools[i % NPOOL].ptr, a, b), "refill")
+    lib.sdl_event_record(e1, stream)
+    lib.sdl_synchronize()
+    ms_step = lib.sdl_event_elapsed_ms(e0, e1) / args.steps
+
+    # sampled check of the last refill against the blocks (numpy, FP64)
+    d_last = pools[(args.steps - 1) % NPOOL].numpy()
+    check(lib.sdl_block_publish(Cm), "publish")
+    csr = Cm.contents.to_csr(Cm).contents
+    nnz = csr.nnz
+    p = np.ctypeslib.as_array(csr.p, (csr.m + 1,)).copy()
+    ci = np.ctypeslib.as_array(csr.i, (max(nnz, 1),))[:nnz].copy()
+    cx = np.ctypeslib.as_array(csr.x, (max(nnz, 1),))[:nnz].copy()
+    rng = np.random.default_rng(0)
+    Lb = w["B"] if btda else w["A"]
+    inv = []
+    for (rpL, cpL, XL), (rpA, cpA, XA) in zip(Lb, w["A"]):
+        iL = np.full(N, -1, np.int64)
+        iL[cpL] = np.arange(len(cpL))
+        iA = np.full(N, -1, np.int64)
+        iA[cpA] = np.arange(len(cpA))
+        inv.append((iL, iA))
+    worst = 0.0
+    rows_of = np.repeat(np.arange(csr.m), np.diff(p))
+    for q in rng.choice(nnz, min(nnz, 300), replace=False):
+        r, c = rows_of[q], ci[q]
+        ref, mag = 0.0, 0.0
+        for k, ((rpL, cpL, XL), (rpA, cpA, XA)) in enumerate(zip(Lb, w["A"])):
+            iL, iA = inv[k]
+            if iL[r] >= 0 and iA[c] >= 0:
+                t = XL[:, iL[r]] * d_last[rpA] * XA[:, iA[c]]
+                ref += t.sum()
+                mag += np.abs(t).sum()
+        err = abs(cx[q] - ref) / (mag + 1e-300)
+        worst = max(worst, err)
+    assert worst <= 1e-13, f"block product differs from the blocks: rel {worst}"
+
+    XA_cells = sum(X.size for _, _, X in w["A"])
+    XB_cells = sum(X.size for _, _, X in w["B"]) if btda else 0
+    bytes_alg = 8 * (XA_cells + XB_cells + m + out_cells)
+    tfs = flops / (ms_step * 1e-3) / 1e12
+    gbs = bytes_alg / (ms_step * 1e-3) / 1e9
+    cpu = None if args.no_cpu else blocks_cpu_baseline(w, btda, args.cpu_seconds)
+    out = {
+        "metric": "stacked permuted-dense B^T·D·A GFLOP/s + % FP64 MFMA peak",
+        "value": flops / (ms_step * 1e-3) / 1e9,
+        "unit": "GFLOP/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_step,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": ("stacked permuted-dense: 256 FP64 blocks, m0,n0 log-uniform [64,512], "
+                         "row partition, col_perms n0-subsets of [0,4096), seed 3; "
+                         + ("C = B^T diag(d) A, B on A's rows with its own col_perms"
+                            if btda else "C = A^T diag(d) A") + "; one step = one refill"),
+            "m": m, "n": N, "out_cells": out_cells, "raw_cells": int(stats[4]),
+            "gemm_tiles": int(stats[2]), "gemm_tasks": int(stats[3]),
+            "coalesce_terms": int(stats[6]), "flops": flops, "alloc_s": t_alloc,
+            "dropin_fill_ms_incl_pcie": t_dropin * 1e3, "sampled_max_rel_err": worst,
+        },
+        "roofline": {
+            "bound": "mfma", "kernel": "k_ggemm + k_pull_sum (coalesce)",
+            "achieved": tfs, "peak": FP64_MFMA_PEAK_TFS, "unit": "TFLOP/s",
+            "frac": tfs / FP64_MFMA_PEAK_TFS, "traffic": None,
+            "alg_bytes_per_launch": bytes_alg, "alg_GBs": gbs, "hbm_frac": gbs / HBM_PEAK_GBS,
+        },
+        "cpu_baseline": cpu,
+    }
+    for M in (Cm, A, B):
+        if M is not None:
+            lib.free_matrix(M)
+    print(json.dumps(out), flush=True)
+
+
+def blocks_cpu_baseline(w, btda, seconds):
+    """Host BLAS (numpy) on a bounded sample of the block pairs: per block
+    X_L^T (d X_A), the dgemm the reference issues per pair; the coalesce is
+    not included."""
+    import threadpoolctl
+    d = w["d"]
+    Lb = w["B"] if btda else w["A"]
+    flops, t = 0.0, 0.0
+    k = 0
+    while t < seconds and k < 10 * len(Lb):
+        (rpL, _, XL), (rpA, _, XA) = Lb[k % len(Lb)], w["A"][k % len(Lb)]
+        t0 = time.perf_counter()
+        Y = d[rpA][:, None] * XA
+        _ = XL.T @ Y
+        t += time.perf_counter() - t0
+        flops += 2.0 * XL.shape[0] * XL.shape[1] * XA.shape[1]
+        k += 1
+    info = threadpoolctl.threadpool_info()
+    threads = max([i.get("num_threads", 1) for i in info if i.get("user_api") == "blas"] or [1])
+    return {"value": flops / t / 1e9, "unit": "GFLOP/s", "cores": threads, "kind": "port",
+            "sample": f"{k} block products X_L^T diag(d) X_A with numpy/OpenBLAS dgemm "
+                      f"({threads} threads), {t:.2f} s; coalesce not timed"}
 
 
 def cpu_baseline(w, nnz, seconds):

@@ -121,6 +121,26 @@ int sdl_btda_autotune(sdl_btda_plan *plan, const double *dAx, const double *dBx,
 /* Pin a configuration (NULL: back to the default heuristic). */
 int sdl_btda_set_refill_cfg(sdl_btda_plan *plan, const sdl_refill_cfg *cfg);
 
+/* ---- device refills of permuted_dense / stacked_pd product outputs ---- */
+struct matrix;
+/* Runs the kernels of a fill of C (an output of BTA_matrices_alloc,
+   ATA_matrices_alloc or BA_matrices_alloc of block kind) with d already in
+   device memory (dd: m doubles, or NULL for the d-free products): left is B
+   for BTA / BA outputs and A for ATA outputs, right is A.  C's device values
+   are written; its host values and values_version are left alone until
+   sdl_block_publish.  Operand device mirrors are refreshed iff their version
+   moved.  ev_start / ev_stop (hipEvent_t, may be NULL) bracket the kernels on
+   the library stream.  Allocates nothing. */
+int sdl_block_refill_device(struct matrix *C, struct matrix *left, struct matrix *right,
+                            const double *dd, void *ev_start, void *ev_stop);
+/* out[8]: GEMM flops, sparse x dense flops, GEMM tiles, GEMM tasks, raw
+   partial cells, output cells, coalesce terms, sparse x dense lists. */
+int sdl_block_stats(const struct matrix *C, double *out);
+/* Device pointer of C's values (block-major for stacked_pd). */
+double *sdl_block_device_values(struct matrix *C);
+/* Copies C's device values to its host values and bumps values_version. */
+int sdl_block_publish(struct matrix *C);
+
 /* ---- synthetic workload generators (host; deterministic in seed) ---- */
 /* Row lengths uniform in [lo, hi]; writes Ap[m+1]; returns nnz. */
 long long sdl_gen_row_lengths_uniform(uint64_t seed, int m, int lo, int hi, int *Ap);

@@ -113,6 +113,11 @@ SIGNATURES = {
     "sdl_btda_bind_values": (C.c_int, [vp, vp, vp, vp]),
     "sdl_btda_refill_batched_ex": (C.c_int, [vp, vp, vp, vp, C.c_longlong, C.c_int, vp,
                                              C.c_longlong, C.c_uint, vp, vp, vp]),
+    # device refills of block-kind product outputs (matrix * passed as pointers)
+    "sdl_block_refill_device": (C.c_int, [vp, vp, vp, vp, vp, vp]),
+    "sdl_block_stats": (C.c_int, [vp, c_double_p]),
+    "sdl_block_device_values": (vp, [vp]),
+    "sdl_block_publish": (C.c_int, [vp]),
     # generators
     "sdl_gen_row_lengths_uniform": (C.c_longlong, [C.c_uint64, C.c_int, C.c_int, C.c_int,
                                                    c_int_p]),

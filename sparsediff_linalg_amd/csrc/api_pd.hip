@@ -365,7 +365,6 @@ matrix *pd_row_gather_alloc(const matrix *self, const int *map, int m_out)
     std::vector<Blk> out;
     std::vector<int> src_idx;
     std::vector<int> perm;
-    long long total = 0;
     for (size_t k = 0; k < bl.size(); k++)
     {
         PDM *B = bl[k];
@@ -387,7 +386,6 @@ matrix *pd_row_gather_alloc(const matrix *self, const int *map, int m_out)
         for (size_t r = 0; r < src_rows.size(); r++)
             for (int jj = 0; jj < B->n0; jj++)
                 perm.push_back((int) (off[k] + (long long) src_rows[r] * B->n0 + jj));
-        total += b.size();
         out.push_back(b);
         src_idx.push_back((int) k);
     }

@@ -102,6 +102,8 @@ struct ProdB : Bound
     DBuf<double> d;
     int d_len = 0;
     matrix *left_view = nullptr;   // when the left operand is read through a sparse view
+    long long raw_cells = 0;       // per-block partial cells before the coalesce
+    long long coalesce_terms = 0;  // raw cells read by the coalesce
 };
 
 struct Program
@@ -399,6 +401,8 @@ matrix *finish(Program &pg, bool stack_out, int m, int n, const std::vector<Blk>
         C = &new_spdm(m, n, pds, co.src_p, co.src_idx)->base;
         b->to_raw = true;
         b->raw.alloc(raw_total);
+        b->raw_cells = raw_total;
+        b->coalesce_terms = (long long) co.idx.size();
         pull_build(b->coalesce, co.seg, co.idx);
     }
     gemm_upload(b->gemm, pg.gb);
@@ -418,20 +422,15 @@ ProdB *prod_of(matrix *C, const char *fn)
     return b;
 }
 
-// left = B (buf 0), right = A (buf 1); sparse operands are read through their
-// CSR values (the lists hold CSR positions).
-void run(const char *fn, matrix *B, matrix *A, const double *d, matrix *C)
+// The device part of a fill: every kernel of C's program on the library
+// stream, d already in device memory (or null), C's device values written.
+void launch(ProdB *b, matrix *B, matrix *A, const double *dd, matrix *C, hipEvent_t e0,
+            hipEvent_t e1)
 {
-    ProdB *b = prod_of(C, fn);
     const double *dB = mat_vals(b->left_view ? b->left_view : B);
     const double *dA = mat_vals(A);
-    const double *dd = nullptr;
-    if (d && b->d_len > 0)
-    {
-        h2d(b->d.p, d, (size_t) b->d_len * sizeof(double));
-        dd = b->d.p;
-    }
     double *target = b->to_raw ? b->raw.p : out_buf(C);
+    if (e0) SDL_HIP(hipEventRecord(e0, stream()));
     gemm_run(b->gemm, dB, dA, dd, target, stream());
     for (auto &j : b->sdmm)
     {
@@ -440,6 +439,21 @@ void run(const char *fn, matrix *B, matrix *A, const double *d, matrix *C)
         sdmm_run(j->plan, sv, dense, dd, target + j->out_off, stream());
     }
     if (b->to_raw) pull_sum(b->coalesce, b->raw.p, out_buf(C), stream());
+    if (e1) SDL_HIP(hipEventRecord(e1, stream()));
+}
+
+// left = B (buf 0), right = A (buf 1); sparse operands are read through their
+// CSR values (the lists hold CSR positions).
+void run(const char *fn, matrix *B, matrix *A, const double *d, matrix *C)
+{
+    ProdB *b = prod_of(C, fn);
+    const double *dd = nullptr;
+    if (d && b->d_len > 0)
+    {
+        h2d(b->d.p, d, (size_t) b->d_len * sizeof(double));
+        dd = b->d.p;
+    }
+    launch(b, B, A, dd, C, nullptr, nullptr);
     out_publish(C);
 }
 
@@ -770,3 +784,67 @@ void pd_dense_kron_fill(const matrix *A, int p, matrix *J, matrix *C, bool kron_
 }
 
 } // namespace sdl
+
+// ====================================================== device refills ====
+extern "C" {
+#pragma GCC visibility push(default)
+
+int sdl_block_refill_device(matrix *C, matrix *left, matrix *right, const double *dd,
+                            void *ev_start, void *ev_stop)
+{
+    using namespace sdl;
+    if (!C || !left || !right || (C->kind != MATRIX_PERMUTED_DENSE && C->kind != MATRIX_STACKED_PD))
+    {
+        set_error("sdl_block_refill_device: C must be a permuted_dense / stacked_pd product output");
+        return 1;
+    }
+    std::unique_ptr<Bound> &slot = C->kind == MATRIX_PERMUTED_DENSE ? as_pd(C)->bound : as_spd(C)->bound;
+    ProdB *b = slot ? dynamic_cast<ProdB *>(slot.get()) : nullptr;
+    if (!b)
+    {
+        set_error("sdl_block_refill_device: C was not produced by a product alloc");
+        return 1;
+    }
+    launch(b, left, right, b->d_len > 0 ? dd : nullptr, C, (hipEvent_t) ev_start,
+           (hipEvent_t) ev_stop);
+    return 0;
+}
+
+int sdl_block_stats(const matrix *C, double *out)
+{
+    using namespace sdl;
+    if (!C || !out || (C->kind != MATRIX_PERMUTED_DENSE && C->kind != MATRIX_STACKED_PD)) return 1;
+    const std::unique_ptr<Bound> &slot =
+        C->kind == MATRIX_PERMUTED_DENSE ? as_pd(C)->bound : as_spd(C)->bound;
+    const ProdB *b = slot ? dynamic_cast<const ProdB *>(slot.get()) : nullptr;
+    if (!b) return 1;
+    long long sflops = 0;
+    for (const auto &j : b->sdmm) sflops += 2ll * j->plan.nent * j->plan.NC;
+    out[0] = (double) b->gemm.flops;
+    out[1] = (double) sflops;
+    out[2] = (double) b->gemm.njobs;
+    out[3] = (double) (b->gemm.tasks.n);
+    out[4] = (double) b->raw_cells;
+    out[5] = (double) C->nnz;
+    out[6] = (double) b->coalesce_terms;
+    out[7] = (double) b->sdmm.size();
+    return 0;
+}
+
+double *sdl_block_device_values(matrix *C)
+{
+    using namespace sdl;
+    if (!C || (C->kind != MATRIX_PERMUTED_DENSE && C->kind != MATRIX_STACKED_PD)) return nullptr;
+    return out_buf(C);
+}
+
+int sdl_block_publish(matrix *C)
+{
+    using namespace sdl;
+    if (!C || (C->kind != MATRIX_PERMUTED_DENSE && C->kind != MATRIX_STACKED_PD)) return 1;
+    out_publish(C);
+    return 0;
+}
+
+#pragma GCC visibility pop
+} // extern "C"

@@ -181,6 +181,7 @@ void sdmm_build(SdmmPlan &P, const std::vector<int> &lp, const std::vector<int> 
                 const std::vector<int> &vp, const std::vector<int> &dk)
 {
     P.NL = (int) lp.size() - 1;
+    P.nent = (long long) dr.size();
     P.lp = upload(lp.data(), (int64_t) lp.size());
     P.dr = upload(dr.data(), (int64_t) dr.size());
     P.vp = upload(vp.data(), (int64_t) vp.size());

@@ -36,6 +36,7 @@ struct SdmmPlan
     long long sL = 0, sC = 0;
     DBuf<int> lp;                 // NL + 1
     DBuf<int> dr, vp, dk;         // entries
+    long long nent = 0;           // number of entries
     bool has_dk = false;
 };
 void sdmm_build(SdmmPlan &P, const std::vector<int> &lp, const std::vector<int> &dr,

@@ -38,3 +38,29 @@ def btda_config1(seed: int = 1, m: int = 20000, n: int = 5000, nnz_row: int = 6)
     Bp, Bi, Bx = csr_uniform(seed * 1000 + 2, m, n, nnz_row, nnz_row)
     d = uniform(seed * 1000 + 3, m, 0.1, 10.0)
     return dict(m=m, nA=n, nB=n, Ap=Ap, Ai=Ai, Ax=Ax, Bp=Bp, Bi=Bi, Bx=Bx, d=d)
+
+
+def blocks_config3(seed: int = 3, nblocks: int = 256, lo: int = 64, hi: int = 512,
+                   ncols: int = 4096):
+    """Stacked permuted-dense operands: nblocks dense FP64 N(0,1) blocks, m0 and
+    n0 independently log-uniform on [lo, hi] (rounded); the rows are a random
+    partition of [0, sum m0) into the blocks' sorted row_perms; each col_perm
+    is a sorted random n0-subset of [0, ncols).  B shares A's row partition
+    with independent col_perms and values.  d uniform [0.1, 10]."""
+    g = np.random.Generator(np.random.PCG64(seed))
+    m0 = np.rint(np.exp(g.uniform(np.log(lo), np.log(hi), nblocks))).astype(int)
+    n0A = np.rint(np.exp(g.uniform(np.log(lo), np.log(hi), nblocks))).astype(int)
+    n0B = np.rint(np.exp(g.uniform(np.log(lo), np.log(hi), nblocks))).astype(int)
+    m = int(m0.sum())
+    rows = g.permutation(m)
+    A, B = [], []
+    start = 0
+    for k in range(nblocks):
+        rp = np.sort(rows[start:start + m0[k]]).astype(np.int32)
+        start += m0[k]
+        cpA = np.sort(g.choice(ncols, n0A[k], replace=False)).astype(np.int32)
+        cpB = np.sort(g.choice(ncols, n0B[k], replace=False)).astype(np.int32)
+        A.append((rp, cpA, g.standard_normal((m0[k], n0A[k]))))
+        B.append((rp, cpB, g.standard_normal((m0[k], n0B[k]))))
+    d = g.uniform(0.1, 10.0, m)
+    return dict(m=m, n=ncols, A=A, B=B, d=d)

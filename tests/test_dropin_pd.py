@@ -354,3 +354,55 @@ def test_pd_reference_literal(c):
     for M in (Cm, A, B):
         if M is not None:
             L.free_matrix(M)
+
+
+@pytest.mark.parametrize("case", ["atda", "btda"])
+def test_config3_full_size(case):
+    """The full stacked permuted-dense workload (256 blocks, m0/n0 log-uniform
+    64..512, col_perms in [0, 4096)): output structure exactly the oracle's,
+    values on 2000 sampled cells within 1e-13 * sum |terms|."""
+    from sparsediff_linalg_amd import workloads
+    L = D.lib()
+    w = workloads.blocks_config3(3)
+    m, N, d = w["m"], w["n"], w["d"]
+    Ao = PO.spd(m, N, [PO.pd(m, N, rp.tolist(), cp.tolist(), X) for rp, cp, X in w["A"]])
+    Ah = make(Ao)
+    if case == "btda":
+        Bo = PO.spd(m, N, [PO.pd(m, N, rp.tolist(), cp.tolist(), X) for rp, cp, X in w["B"]])
+        Bh = make(Bo)
+        Cm = L.BTA_matrices_alloc(Ah, Bh)
+        L.BTDA_matrices_fill_values(Ah, D.dptr(d), Bh, Cm)
+        S = PO.bta(Ao, Bo)
+        Lb = w["B"]
+    else:
+        Bh = None
+        Cm = L.ATA_matrices_alloc(Ah)
+        L.ATDA_matrices_fill_values(Ah, D.dptr(d), Cm)
+        S = PO.ata(Ao)
+        Lb = w["A"]
+    p, i, x = D.mat_csr(Cm)
+    ep, ei = PO.csr_pattern(S)
+    assert np.array_equal(p, np.asarray(ep)), "row pointers differ"
+    assert np.array_equal(i, np.asarray(ei)), "column indices differ"
+    rows_of = np.repeat(np.arange(len(p) - 1), np.diff(p))
+    rng = np.random.default_rng(1)
+    invs = []
+    for (_, cpL, _), (_, cpA, _) in zip(Lb, w["A"]):
+        iL = np.full(N, -1)
+        iL[cpL] = np.arange(len(cpL))
+        iA = np.full(N, -1)
+        iA[cpA] = np.arange(len(cpA))
+        invs.append((iL, iA))
+    for q in rng.choice(len(x), 2000, replace=False):
+        r, c = rows_of[q], i[q]
+        ref = mag = 0.0
+        for k, ((_, _, XL), (rpA, _, XA)) in enumerate(zip(Lb, w["A"])):
+            iL, iA = invs[k]
+            if iL[r] >= 0 and iA[c] >= 0:
+                t = XL[:, iL[r]] * d[rpA] * XA[:, iA[c]]
+                ref += t.sum()
+                mag += np.abs(t).sum()
+        assert abs(x[q] - ref) <= 1e-13 * mag + 1e-300, (r, c, x[q], ref)
+    for M in (Cm, Ah, Bh):
+        if M is not None:
+            L.free_matrix(M)
