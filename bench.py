@@ -52,6 +52,9 @@ def parse():
     ap.add_argument("--workload", default="btda", choices=["btda", "blocks-atda", "blocks-btda"],
                     help="btda: the sparse refill (default line); blocks-*: the stacked "
                          "permuted-dense products (A^T D A, or B^T D A of two stacks)")
+    ap.add_argument("--gemm-slice", type=int, default=0,
+                    help="blocks-*: reduction slice depth of the block GEMM (16 or 32; 0 = "
+                         "library default)")
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="also run the streaming copy kernel on this many MiB (HBM counter "
                          "calibration under rocprofv3 --pmc)")
@@ -243,6 +246,8 @@ def main_blocks(args):
     from sparsediff_linalg_amd._lib import MAT_p
 
     lib = sdl.load()
+    if args.gemm_slice:
+        check(0 if lib.sdl_block_gemm_slice(args.gemm_slice) > 0 else 1, "gemm slice")
     w = workloads.blocks_config3(3)
     m, N = w["m"], w["n"]
     btda = args.workload == "blocks-btda"
@@ -348,6 +353,7 @@ def main_blocks(args):
             "m": m, "n": N, "out_cells": out_cells, "raw_cells": int(stats[4]),
             "gemm_tiles": int(stats[2]), "gemm_tasks": int(stats[3]),
             "coalesce_terms": int(stats[6]), "flops": flops, "alloc_s": t_alloc,
+            "gemm_slice": args.gemm_slice or 16,
             "dropin_fill_ms_incl_pcie": t_dropin * 1e3, "sampled_max_rel_err": worst,
         },
         "roofline": {

@@ -140,6 +140,9 @@ int sdl_block_stats(const struct matrix *C, double *out);
 double *sdl_block_device_values(struct matrix *C);
 /* Copies C's device values to its host values and bumps values_version. */
 int sdl_block_publish(struct matrix *C);
+/* Reduction slice depth of the block-product GEMM (16 or 32, default 16);
+   returns the previous depth, or -1 for an unsupported value. */
+int sdl_block_gemm_slice(int sk);
 
 /* ---- synthetic workload generators (host; deterministic in seed) ---- */
 /* Row lengths uniform in [lo, hi]; writes Ap[m+1]; returns nnz. */

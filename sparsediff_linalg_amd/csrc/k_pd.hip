@@ -67,84 +67,154 @@ __global__ void __launch_bounds__(256)
     }
 }
 
-constexpr int GT = 64;   // output tile
-constexpr int SK = 16;   // reduction slice
+constexpr int GT = 64;      // output tile
 constexpr int PAD = 2;
+constexpr int XCD = 8;      // workgroups are dealt round-robin to the XCDs
+constexpr int XCD_RUN = 4;  // consecutive tiles kept on one XCD (shared panels)
 
+// One SK x 64 slice of a task's P and Q operands into registers: thread tid
+// owns tile column ii = tid % 64 and slice rows sb, sb + 4, ... (sb = tid / 64),
+// so each wavefront reads 64 consecutive columns of a row.
+template <int SK>
+__device__ __forceinline__ void load_slice(const GJob &job, const GTask &T, int s0,
+                                           const int *__restrict__ pool,
+                                           const double *__restrict__ buf0,
+                                           const double *__restrict__ buf1,
+                                           const double *__restrict__ w, int tid,
+                                           double pr[SK / 4], double qr[SK / 4],
+                                           double wr[SK / 4])
+{
+    const double *P = (T.p_buf ? buf1 : buf0) + T.p_off;
+    const double *Q = (T.q_buf ? buf1 : buf0) + T.q_off;
+    const int ii = tid & 63, sb = tid >> 6;
+    const int i = job.i0 + ii, j = job.j0 + ii;
+    int mi = -1, mj = -1;
+    if (i < job.M) mi = T.pmap < 0 ? i : pool[T.pmap + i];
+    if (j < job.N) mj = T.qmap < 0 ? j : pool[T.qmap + j];
+#pragma unroll
+    for (int u = 0; u < SK / 4; u++)
+    {
+        const int s = s0 + sb + 4 * u;
+        double pv = 0.0, qv = 0.0, wv = 1.0;
+        if (s < T.S)
+        {
+            if (mi >= 0)
+            {
+                const long long r = (T.direct & 1) ? T.p_r0 + s : pool[T.prow + s];
+                pv = T.p_mode == 0 ? P[r * T.p_ld + mi] : P[(long long) mi * T.p_ld + r];
+            }
+            if (mj >= 0)
+            {
+                const long long r = (T.direct & 2) ? T.q_r0 + s : pool[T.qrow + s];
+                qv = T.q_mode == 0 ? Q[r * T.q_ld + mj] : Q[(long long) mj * T.q_ld + r];
+                if (w != nullptr && T.widx >= 0) wv = w[T.widx + s];
+            }
+        }
+        pr[u] = pv;
+        qr[u] = qv;
+        wr[u] = wv;
+    }
+}
+
+__global__ void k_gather_w(int n, const int *__restrict__ src, const double *__restrict__ d,
+                           double *__restrict__ w)
+{
+    int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < n) w[q] = d[src[q]];
+}
+
+// next (task, slice) after (t, s0) in the job, skipping empty tasks
+template <int SK>
+__device__ __forceinline__ void next_slice(const GTask *__restrict__ tasks, int t1, int &t, int &s0,
+                                           int S)
+{
+    s0 += SK;
+    if (s0 < S) return;
+    s0 = 0;
+    for (t = t + 1; t < t1 && tasks[t].S <= 0; t++) {}
+}
+
+// Grid position of job index j (jobs sorted by decreasing work): runs of
+// XCD_RUN consecutive jobs go to one XCD, runs dealt round-robin.
+__host__ __device__ inline int job_of_block(int bid)
+{
+    const int xcd = bid % XCD, slot = bid / XCD;
+    return ((slot / XCD_RUN) * XCD + xcd) * XCD_RUN + slot % XCD_RUN;
+}
+
+inline int grid_for_jobs(int njobs)
+{
+    const int runs = (njobs + XCD_RUN - 1) / XCD_RUN;
+    return (runs + XCD - 1) / XCD * XCD * XCD_RUN;
+}
+
+template <int SK>
 __global__ void __launch_bounds__(256)
-    k_ggemm(const GJob *__restrict__ jobs, const GTask *__restrict__ tasks,
+    k_ggemm(int njobs, const GJob *__restrict__ jobs, const GTask *__restrict__ tasks,
             const int *__restrict__ pool, const double *__restrict__ buf0,
-            const double *__restrict__ buf1, const double *__restrict__ d,
+            const double *__restrict__ buf1, const double *__restrict__ w,
             double *__restrict__ C)
 {
     __shared__ double Ps[SK][GT + PAD];
     __shared__ double Qs[SK][GT + PAD];
-    const GJob job = jobs[blockIdx.x];
+    const int jb = job_of_block(blockIdx.x);
+    if (jb >= njobs) return;
+    const GJob job = jobs[jb];
     const int tid = threadIdx.x;
-    const int lane = tid & 63, w = tid >> 6;
-    const int wr = w >> 1, wc = w & 1;
+    const int lane = tid & 63, wave = tid >> 6;
+    const int wrow = wave >> 1, wcol = wave & 1;
+    const int ii = tid & 63, sb = tid >> 6;
+    // 16 x 16 sub-tiles of this wavefront that intersect the output block
+    // (wave-uniform; the others are skipped)
+    bool live[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int b = 0; b < 2; b++)
+            live[a][b] = job.i0 + wrow * 32 + a * 16 < job.M && job.j0 + wcol * 32 + b * 16 < job.N;
     d4 acc[2][2];
 #pragma unroll
     for (int a = 0; a < 2; a++)
 #pragma unroll
         for (int b = 0; b < 2; b++) acc[a][b] = d4{0.0, 0.0, 0.0, 0.0};
 
-    for (int t = job.t0; t < job.t1; t++)
+    int t = job.t0;
+    while (t < job.t1 && tasks[t].S <= 0) t++;
+    int s0 = 0;
+    double pr[SK / 4], qr[SK / 4], wr[SK / 4];
+    if (t < job.t1) load_slice<SK>(job, tasks[t], 0, pool, buf0, buf1, w, tid, pr, qr, wr);
+    while (t < job.t1)
     {
-        const GTask T = tasks[t];
-        const double *P = (T.p_buf ? buf1 : buf0) + T.p_off;
-        const double *Q = (T.q_buf ? buf1 : buf0) + T.q_off;
-        for (int s0 = 0; s0 < T.S; s0 += SK)
+        // the d factor scales Q's row before the product: ((d Q) entries)
+#pragma unroll
+        for (int u = 0; u < SK / 4; u++)
         {
-            for (int e = tid; e < SK * GT; e += 256)
-            {
-                const int ss = e / GT, ii = e % GT;
-                const int s = s0 + ss;
-                const int i = job.i0 + ii;
-                double pv = 0.0, qv = 0.0;
-                if (s < T.S)
-                {
-                    if (i < job.M)
-                    {
-                        const int mi = T.pmap < 0 ? i : pool[T.pmap + i];
-                        if (mi >= 0)
-                        {
-                            const long long r = pool[T.prow + s];
-                            pv = T.p_mode == 0 ? P[r * T.p_ld + mi] : P[(long long) mi * T.p_ld + r];
-                        }
-                    }
-                    const int j = job.j0 + ii;
-                    if (j < job.N)
-                    {
-                        const int mj = T.qmap < 0 ? j : pool[T.qmap + j];
-                        if (mj >= 0)
-                        {
-                            const long long r = pool[T.qrow + s];
-                            qv = T.q_mode == 0 ? Q[r * T.q_ld + mj] : Q[(long long) mj * T.q_ld + r];
-                            if (d != nullptr && T.widx >= 0) qv = d[pool[T.widx + s]] * qv;
-                        }
-                    }
-                }
-                Ps[ss][ii] = pv;
-                Qs[ss][ii] = qv;
-            }
-            __syncthreads();
-#pragma unroll
-            for (int kk = 0; kk < SK; kk += 4)
-            {
-                double af[2], bf[2];
-#pragma unroll
-                for (int a = 0; a < 2; a++) af[a] = Ps[kk + (lane >> 4)][wr * 32 + a * 16 + (lane & 15)];
-#pragma unroll
-                for (int b = 0; b < 2; b++) bf[b] = Qs[kk + (lane >> 4)][wc * 32 + b * 16 + (lane & 15)];
-#pragma unroll
-                for (int a = 0; a < 2; a++)
-#pragma unroll
-                    for (int b = 0; b < 2; b++)
-                        acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[a], bf[b], acc[a][b], 0, 0, 0);
-            }
-            __syncthreads();
+            Ps[sb + 4 * u][ii] = pr[u];
+            Qs[sb + 4 * u][ii] = wr[u] * qr[u];
         }
+        __syncthreads();
+        int nt = t, ns0 = s0;
+        next_slice<SK>(tasks, job.t1, nt, ns0, tasks[t].S);
+        // the next slice's loads are in flight during this slice's MFMAs
+        if (nt < job.t1) load_slice<SK>(job, tasks[nt], ns0, pool, buf0, buf1, w, tid, pr, qr, wr);
+#pragma unroll
+        for (int kk = 0; kk < SK; kk += 4)
+        {
+            double af[2], bf[2];
+#pragma unroll
+            for (int a = 0; a < 2; a++) af[a] = Ps[kk + (lane >> 4)][wrow * 32 + a * 16 + (lane & 15)];
+#pragma unroll
+            for (int b = 0; b < 2; b++) bf[b] = Qs[kk + (lane >> 4)][wcol * 32 + b * 16 + (lane & 15)];
+#pragma unroll
+            for (int a = 0; a < 2; a++)
+#pragma unroll
+                for (int b = 0; b < 2; b++)
+                    if (live[a][b])
+                        acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[a], bf[b], acc[a][b], 0, 0, 0);
+        }
+        __syncthreads();
+        t = nt;
+        s0 = ns0;
     }
     double *Cb = C + job.c_off;
 #pragma unroll
@@ -154,11 +224,13 @@ __global__ void __launch_bounds__(256)
 #pragma unroll
             for (int v = 0; v < 4; v++)
             {
-                const int i = job.i0 + wr * 32 + a * 16 + (lane >> 4) + 4 * v;
-                const int j = job.j0 + wc * 32 + b * 16 + (lane & 15);
+                const int i = job.i0 + wrow * 32 + a * 16 + (lane >> 4) + 4 * v;
+                const int j = job.j0 + wcol * 32 + b * 16 + (lane & 15);
                 if (i < job.M && j < job.N) Cb[(long long) i * job.ldc + j] = acc[a][b][v];
             }
 }
+
+int g_gemm_slice = 16;  // reduction slice depth of k_ggemm (16 or 32)
 
 } // namespace
 
@@ -226,12 +298,68 @@ void GemmBuilder::add_output(long long c_off, int ldc, int M, int N, int t0, int
         }
 }
 
+namespace {
+// rows pool[off .. off + S) are r0, r0 + 1, ...
+bool contiguous(const std::vector<int> &pool, int off, int S, int &r0)
+{
+    if (S <= 0) return false;
+    r0 = pool[off];
+    for (int s = 1; s < S; s++)
+        if (pool[off + s] != r0 + s) return false;
+    return true;
+}
+} // namespace
+
 void gemm_upload(GemmPlan &P, const GemmBuilder &B)
 {
     P.njobs = (int) B.jobs.size();
     P.flops = B.flops;
-    P.tasks = upload(B.tasks.data(), (int64_t) B.tasks.size());
-    P.jobs = upload(B.jobs.data(), (int64_t) B.jobs.size());
+    std::vector<GTask> tasks = B.tasks;
+    // column maps that are the identity over every output tile they feed
+    std::vector<int> pm_len(tasks.size(), 0), qm_len(tasks.size(), 0);
+    for (const GJob &j : B.jobs)
+        for (int t = j.t0; t < j.t1; t++)
+        {
+            pm_len[t] = std::max(pm_len[t], j.M);
+            qm_len[t] = std::max(qm_len[t], j.N);
+        }
+    auto identity = [&](int off, int len) {
+        for (int i = 0; i < len; i++)
+            if (B.pool[off + i] != i) return false;
+        return true;
+    };
+    std::vector<int> wsrc;
+    for (size_t t = 0; t < tasks.size(); t++)
+    {
+        GTask &T = tasks[t];
+        if (T.pmap >= 0 && identity(T.pmap, pm_len[t])) T.pmap = -1;
+        if (T.qmap >= 0 && identity(T.qmap, qm_len[t])) T.qmap = -1;
+        T.direct = 0;
+        if (T.pmap < 0 && contiguous(B.pool, T.prow, T.S, T.p_r0)) T.direct |= 1;
+        if (T.qmap < 0 && contiguous(B.pool, T.qrow, T.S, T.q_r0)) T.direct |= 2;
+        if (T.widx >= 0)
+        {
+            const int pos = (int) wsrc.size();
+            for (int s = 0; s < T.S; s++) wsrc.push_back(B.pool[T.widx + s]);
+            T.widx = pos;
+        }
+    }
+    P.nw = (int) wsrc.size();
+    if (P.nw > 0)
+    {
+        P.wsrc = upload(wsrc.data(), (int64_t) wsrc.size());
+        P.w.alloc(wsrc.size());
+    }
+    P.tasks = upload(tasks.data(), (int64_t) tasks.size());
+    // longest jobs first (the grid drains evenly); ties keep block order so
+    // the tiles of one block stay adjacent
+    std::vector<GJob> jobs = B.jobs;
+    std::vector<long long> work(tasks.size() + 1, 0);
+    for (size_t t = 0; t < tasks.size(); t++) work[t + 1] = work[t] + std::max(tasks[t].S, 0);
+    std::stable_sort(jobs.begin(), jobs.end(), [&](const GJob &a, const GJob &b) {
+        return work[a.t1] - work[a.t0] > work[b.t1] - work[b.t0];
+    });
+    P.jobs = upload(jobs.data(), (int64_t) jobs.size());
     P.pool = upload(B.pool.data(), (int64_t) B.pool.size());
     sync();
 }
@@ -240,9 +368,36 @@ void gemm_run(const GemmPlan &P, const double *buf0, const double *buf1, const d
               double *C, hipStream_t s)
 {
     if (P.njobs <= 0) return;
-    hipLaunchKernelGGL(k_ggemm, dim3((unsigned) P.njobs), dim3(256), 0, s, P.jobs.p, P.tasks.p,
-                       P.pool.p, buf0, buf1, d, C);
+    const double *w = nullptr;
+    if (d != nullptr && P.nw > 0)
+    {
+        hipLaunchKernelGGL(k_gather_w, dim3(grid_for(P.nw)), dim3(256), 0, s, P.nw, P.wsrc.p, d,
+                           P.w.p);
+        SDL_HIP(hipGetLastError());
+        w = P.w.p;
+    }
+    const dim3 grid((unsigned) grid_for_jobs(P.njobs));
+    if (g_gemm_slice == 32)
+        hipLaunchKernelGGL(k_ggemm<32>, grid, dim3(256), 0, s, P.njobs, P.jobs.p, P.tasks.p,
+                           P.pool.p, buf0, buf1, w, C);
+    else
+        hipLaunchKernelGGL(k_ggemm<16>, grid, dim3(256), 0, s, P.njobs, P.jobs.p, P.tasks.p,
+                           P.pool.p, buf0, buf1, w, C);
     SDL_HIP(hipGetLastError());
 }
 
 } // namespace sdl
+
+extern "C" {
+#pragma GCC visibility push(default)
+// Reduction slice depth of the block-product GEMM (16 or 32); returns the
+// previous value, or -1 for an unsupported depth.
+int sdl_block_gemm_slice(int sk)
+{
+    if (sk != 16 && sk != 32) return -1;
+    const int prev = sdl::g_gemm_slice;
+    sdl::g_gemm_slice = sk;
+    return prev;
+}
+#pragma GCC visibility pop
+}

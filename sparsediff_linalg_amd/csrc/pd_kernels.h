@@ -48,7 +48,8 @@ void sdmm_run(const SdmmPlan &P, const double *Sx, const double *D, const double
 // C[i][j] = sum over the job's tasks (in order) of sum_s P_t(s, i) * Q_t(s, j),
 //   P_t(s,i) = p_mode 0: P[p_off + prow[s]*p_ld + pmap(i)]  |  1: P[p_off + pmap(i)*p_ld + prow[s]]
 //   Q_t(s,j) = w(s) * (q_mode 0: Q[q_off + qrow[s]*q_ld + qmap(j)] | 1: Q[q_off + qmap(j)*q_ld + qrow[s]])
-// with w(s) = d[widx[s]] (scaled before the product) when widx >= 0; map
+// with w(s) = d[widx[s]] (scaled before the product) when widx >= 0 (on the
+// device widx is rewritten to a position in the plan's gathered w); map
 // entries < 0 contribute zero; a map offset < 0 is the identity.  P and Q
 // read source buffer 0 or 1 (p_buf / q_buf).
 struct GTask
@@ -61,6 +62,10 @@ struct GTask
     int prow, pmap;
     int qrow, qmap;
     int widx;
+    // set by gemm_upload: bit 0 (1) = P's rows are p_r0 + s and pmap is the
+    // identity, so P loads need no pool lookup; same for Q
+    int direct = 0;
+    int p_r0 = 0, q_r0 = 0;
 };
 
 struct GJob
@@ -78,6 +83,11 @@ struct GemmPlan
     DBuf<GTask> tasks;
     DBuf<GJob> jobs;
     DBuf<int> pool;
+    // the d factors of every task, gathered once per refill (w[q] = d[wsrc[q]])
+    // so that the GEMM's operand loads do not wait on a d-index load
+    int nw = 0;
+    DBuf<int> wsrc;
+    DBuf<double> w;
 };
 
 // Host-side builder: add output blocks and their tasks, then upload.
