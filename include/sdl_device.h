@@ -121,6 +121,58 @@ int sdl_btda_autotune(sdl_btda_plan *plan, const double *dAx, const double *dBx,
 /* Pin a configuration (NULL: back to the default heuristic). */
 int sdl_btda_set_refill_cfg(sdl_btda_plan *plan, const sdl_refill_cfg *cfg);
 
+/* ---- learned row binning (no reference counterpart) ----
+   A 16 -> 32 -> 6 FP32 MLP predicts, from 16 structural features of an
+   output row, the size bin of the row's nnz (<= 32, 128, 512, 2048, 4096,
+   larger).  A predicted plan runs one symbolic pass with each row's hash
+   table sized at twice its bin's bound (or from the term bound when that is
+   smaller); rows whose table fills up take the exact count + fill passes, so
+   the structure is exact whatever the prediction.  Parameters: W1[32][16] |
+   b1[32] | W2[6][32] | b2[6] (742 floats).  dX: n x 16 features (row-major),
+   dY: bins (int32). */
+typedef struct sdl_mlp sdl_mlp;
+int sdl_mlp_create(uint64_t seed, sdl_mlp **out);
+void sdl_mlp_destroy(sdl_mlp *mlp);
+int sdl_mlp_num_params(void);
+int sdl_mlp_get_params(const sdl_mlp *mlp, float *params);
+int sdl_mlp_set_params(sdl_mlp *mlp, const float *params);
+/* Decision rule: the smallest bin whose cumulative probability >= tau
+   (default 0.95). */
+int sdl_mlp_set_tau(sdl_mlp *mlp, float tau);
+/* Logits dZ[n][6]. */
+int sdl_mlp_forward(const sdl_mlp *mlp, const float *dX, long long n, float *dZ);
+int sdl_mlp_predict(const sdl_mlp *mlp, const float *dX, long long n, unsigned char *dbins);
+/* Mean cross-entropy over the n rows and its gradient dG[742] (device). */
+int sdl_mlp_gradients(sdl_mlp *mlp, const float *dX, const int *dY, long long n, float *dG,
+                      float *loss);
+/* Adam (lr) over shuffled mini-batches of `batch` rows for `epochs` passes;
+   final_loss = mean batch loss of the last epoch. */
+int sdl_mlp_train(sdl_mlp *mlp, const float *dX, const int *dY, long long n, int epochs,
+                  int batch, float lr, uint64_t seed, float *final_loss);
+/* Parameters to / from a file (magic, count, tau, parameters, FNV-1a
+   checksum); load rejects a file whose checksum does not match. */
+int sdl_mlp_save(const sdl_mlp *mlp, const char *path);
+int sdl_mlp_load(sdl_mlp *mlp, const char *path);
+/* Predictor for the sparse product allocs of the reference API and for
+   sdl_btda_plan_create(..., SDL_SYM_PREDICTED, ...) (NULL: exact passes for
+   the allocs; term-bound tables for SDL_SYM_PREDICTED).
+   The predicted one-pass symbolic phase is currently switched off: binding
+   a non-NULL predictor, SDL_SYM_PREDICTED plans and
+   sdl_btda_plan_create_predicted return an error (sdl_last_error says so). */
+int sdl_set_row_predictor(const sdl_mlp *mlp);
+/* Plan with the one-pass predicted symbolic phase (mlp may be NULL). */
+int sdl_btda_plan_create_predicted(int m, int nA, int nB, const int *dAp, const int *dAi,
+                                   const int *dBp, const int *dBi, const sdl_mlp *mlp,
+                                   sdl_btda_plan **out);
+/* Features dF[nB][16] of the plan's output rows, and the bins of its exact
+   row counts (training labels). */
+int sdl_btda_plan_features(const sdl_btda_plan *plan, float *dF);
+int sdl_btda_plan_row_bins(const sdl_btda_plan *plan, int *dY);
+/* Predicted plans: rows served by the one pass, one-pass rows whose table
+   filled up (exact fallback), rows whose predicted bin differs from the bin
+   of their exact nnz, rows sent straight to the exact passes. */
+int sdl_btda_plan_pred_stats(const sdl_btda_plan *plan, long long *out4);
+
 /* ---- device refills of permuted_dense / stacked_pd product outputs ---- */
 struct matrix;
 /* Runs the kernels of a fill of C (an output of BTA_matrices_alloc,

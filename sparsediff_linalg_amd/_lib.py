@@ -113,6 +113,26 @@ SIGNATURES = {
     "sdl_btda_bind_values": (C.c_int, [vp, vp, vp, vp]),
     "sdl_btda_refill_batched_ex": (C.c_int, [vp, vp, vp, vp, C.c_longlong, C.c_int, vp,
                                              C.c_longlong, C.c_uint, vp, vp, vp]),
+    # learned row binning
+    "sdl_mlp_create": (C.c_int, [C.c_uint64, C.POINTER(vp)]),
+    "sdl_mlp_destroy": (None, [vp]),
+    "sdl_mlp_num_params": (C.c_int, []),
+    "sdl_mlp_get_params": (C.c_int, [vp, C.POINTER(C.c_float)]),
+    "sdl_mlp_set_params": (C.c_int, [vp, C.POINTER(C.c_float)]),
+    "sdl_mlp_set_tau": (C.c_int, [vp, C.c_float]),
+    "sdl_mlp_forward": (C.c_int, [vp, vp, C.c_longlong, vp]),
+    "sdl_mlp_predict": (C.c_int, [vp, vp, C.c_longlong, vp]),
+    "sdl_mlp_gradients": (C.c_int, [vp, vp, vp, C.c_longlong, vp, C.POINTER(C.c_float)]),
+    "sdl_mlp_train": (C.c_int, [vp, vp, vp, C.c_longlong, C.c_int, C.c_int, C.c_float,
+                                C.c_uint64, C.POINTER(C.c_float)]),
+    "sdl_mlp_save": (C.c_int, [vp, C.c_char_p]),
+    "sdl_mlp_load": (C.c_int, [vp, C.c_char_p]),
+    "sdl_set_row_predictor": (C.c_int, [vp]),
+    "sdl_btda_plan_create_predicted": (C.c_int, [C.c_int, C.c_int, C.c_int, vp, vp, vp, vp, vp,
+                                                 C.POINTER(vp)]),
+    "sdl_btda_plan_features": (C.c_int, [vp, vp]),
+    "sdl_btda_plan_row_bins": (C.c_int, [vp, vp]),
+    "sdl_btda_plan_pred_stats": (C.c_int, [vp, c_ll_p]),
     # device refills of block-kind product outputs (matrix * passed as pointers)
     "sdl_block_refill_device": (C.c_int, [vp, vp, vp, vp, vp, vp]),
     "sdl_block_stats": (C.c_int, [vp, c_double_p]),

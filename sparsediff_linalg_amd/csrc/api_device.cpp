@@ -1,6 +1,8 @@
 // Device-resident C ABI (include/sdl_device.h): B^T diag(d) A plans on HBM
 // operands, single and batched refills.
 #include "../../include/sdl_device.h"
+#include "btda_plan.h"
+#include "mlp.h"
 #include "sparse_ops.h"
 #include "spgemm.h"
 
@@ -8,13 +10,6 @@
 #include <new>
 #include <vector>
 
-struct sdl_btda_plan
-{
-    int m = 0, nA = 0, nB = 0;
-    sdl::DBuf<int> Ap, Ai;            // A's structure (copied)
-    sdl::DBuf<int> Lp, Li, Lperm;     // B's columns: CSC structure + CSR positions
-    sdl::ProductPlan P;
-};
 
 namespace {
 hipStream_t as_stream(void *s) { return s ? (hipStream_t) s : sdl::stream(); }
@@ -32,6 +27,23 @@ extern "C" {
 
 int sdl_btda_plan_create(int m, int nA, int nB, const int *dAp, const int *dAi,
                          const int *dBp, const int *dBi, int mode, sdl_btda_plan **out)
+{
+    if (mode == SDL_SYM_PREDICTED && !sdl::kPredictedSymbolicEnabled)
+    {
+        sdl::set_error("sdl_btda_plan_create: %s", sdl::kPredictedSymbolicOff);
+        return 1;
+    }
+    return sdl_btda_plan_create_mode(m, nA, nB, dAp, dAi, dBp, dBi,
+                                     mode == SDL_SYM_PREDICTED ? sdl::SymMode::Predicted
+                                                               : sdl::SymMode::Exact,
+                                     sdl::g_row_predictor, out);
+}
+#pragma GCC visibility pop
+} // extern "C"
+
+int sdl_btda_plan_create_mode(int m, int nA, int nB, const int *dAp, const int *dAi,
+                              const int *dBp, const int *dBi, sdl::SymMode mode,
+                              const sdl::RowPredictor *pred, sdl_btda_plan **out)
 {
     if (!out || m < 0 || nA < 0 || nB < 0 || !dAp || !dBp)
     {
@@ -67,12 +79,13 @@ int sdl_btda_plan_create(int m, int nA, int nB, const int *dAp, const int *dAi,
     A.An = nA;
     A.Ap = pl->Ap.p;
     A.Ai = pl->Ai.p;
-    sdl::product_plan(pl->P, L, A, mode == SDL_SYM_PREDICTED ? sdl::SymMode::Predicted
-                                                             : sdl::SymMode::Exact,
-                      nullptr);
+    sdl::product_plan(pl->P, L, A, mode, pred);
     *out = pl;
     return 0;
 }
+
+extern "C" {
+#pragma GCC visibility push(default)
 
 void sdl_btda_plan_destroy(sdl_btda_plan *plan)
 {

@@ -5,6 +5,7 @@
 // refreshes the operands' device values (version-guarded), runs the GPU
 // kernels, writes the output's host values and bumps its version once.
 #include "api_matrix.h"
+#include "mlp.h"
 #include "api_pd.h"
 
 #include <climits>
@@ -186,7 +187,8 @@ AView a_view(SparseM *A)
 matrix *product_alloc(const char *fn, const LView &L, SparseM *A, int d_len)
 {
     auto b = std::make_unique<ProductBound>();
-    product_plan(b->P, L, a_view(A), SymMode::Exact, nullptr, true);
+    product_plan(b->P, L, a_view(A), g_row_predictor ? SymMode::Predicted : SymMode::Exact,
+                 g_row_predictor, true);
     CSR_matrix *csr = csr_from_device(fn, (int) b->P.m_out, A->base.n, b->P.Cp.p, b->P.Ci.p,
                                       b->P.nnz);
     b->d.alloc(std::max(d_len, 1));

@@ -4,6 +4,7 @@
 // output; every fill uploads the operands' values, runs the GPU kernels and
 // writes the output's host values.
 #include "api_common.h"
+#include "mlp.h"
 
 #include <climits>
 #include <cstring>
@@ -176,7 +177,8 @@ CSR_matrix *product_alloc_impl(const char *fn, int Lm, const int *Lp, const int 
     A.An = An;
     A.Ap = st->Ap.p;
     A.Ai = st->Ai.p;
-    product_plan(st->P, L, A, SymMode::Exact, nullptr, allow_pull);
+    product_plan(st->P, L, A, g_row_predictor ? SymMode::Predicted : SymMode::Exact, g_row_predictor,
+                 allow_pull);
     int nnz = checked_nnz(fn, st->P.nnz);
     CSR_matrix *C = alloc_csr_struct(Lm, An, nnz);
     DBuf<int> p32((size_t) Lm + 1);

@@ -16,7 +16,10 @@
 // for bit.
 #include "spgemm.h"
 
+#include "mlp.h"
+
 #include <algorithm>
+#include <climits>
 #include <vector>
 
 namespace sdl {
@@ -250,7 +253,8 @@ __device__ __forceinline__ int classify(int64_t cap, int nwords, int lds_words)
     return CLS_GLOBAL;
 }
 
-// mode 0: symbolic accumulator classes; mode 1: 0 empty, 1 <= t1, 2 <= t2, 3 larger
+// mode 0: symbolic accumulator classes; mode 1: 0 empty, 1 <= t1, 2 <= t2, 3 larger;
+// mode 2: cap already holds the class
 template <typename T>
 __global__ void k_classify(int64_t m, const T *__restrict__ cap, int mode, int nwords,
                            int lds_words, int t1, int t2, unsigned char *__restrict__ cls,
@@ -264,7 +268,8 @@ __global__ void k_classify(int64_t m, const T *__restrict__ cap, int mode, int n
     {
         int64_t v = (int64_t) cap[row];
         int c = mode == 0 ? classify(v, nwords, lds_words)
-                          : (v <= 0 ? 0 : (v <= t1 ? 1 : (v <= t2 ? 2 : 3)));
+                : mode == 1 ? (v <= 0 ? 0 : (v <= t1 ? 1 : (v <= t2 ? 2 : 3)))
+                            : (int) v;  // mode 2: the value is the class
         cls[row] = (unsigned char) c;
         atomicAdd(&hist[c], 1);
     }
@@ -330,25 +335,36 @@ __global__ void __launch_bounds__(G)
     __shared__ Expand<G> ex;
     __shared__ long long red[G / 64];
     __shared__ int wsum[G / 64];
+    // In global scratch the bits are set by atomics (performed in L2), so the
+    // words are cleared and read with agent-scope atomic stores / loads and
+    // the phases are separated by device-scope fences.
     unsigned *bm = GLOBAL ? gscratch + (size_t) blockIdx.x * nwords : lds_bm;
+    auto word = [&](int w) -> unsigned {
+        if (!GLOBAL) return bm[w];
+        return __hip_atomic_load(bm + w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    };
     for (int ri = blockIdx.x; ri < nrows; ri += (GLOBAL ? gridDim.x : nrows))
     {
         const int row = rows[ri];
-        for (int i = threadIdx.x; i < nwords; i += G) bm[i] = 0u;
-        if (GLOBAL) __threadfence_block();
+        for (int i = threadIdx.x; i < nwords; i += G)
+        {
+            if (GLOBAL) __hip_atomic_store(bm + i, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            else bm[i] = 0u;
+        }
+        if (GLOBAL) __threadfence();
         __syncthreads();
         RowCtx c = row_ctx(row, Lm, Lp, nL);
         for_each_term<G>(ex, c.q0, c.q1, c.koff, Li, Ap, [&](int t, int, int) {
             int col = Ai[t];
             atomicOr(&bm[col >> 5], 1u << (col & 31));
         });
-        if (GLOBAL) __threadfence_block();
+        if (GLOBAL) __threadfence();
         __syncthreads();
         // contiguous word range per thread keeps the output ascending
         int per = (nwords + G - 1) / G;
         int w0 = min(nwords, (int) threadIdx.x * per), w1 = min(nwords, w0 + per);
         int cnt = 0;
-        for (int w = w0; w < w1; w++) cnt += __popc(bm[w]);
+        for (int w = w0; w < w1; w++) cnt += __popc(word(w));
         if (PASS == 0)
         {
             long long tot = block_sum_ll<G>((long long) cnt, red);
@@ -361,7 +377,7 @@ __global__ void __launch_bounds__(G)
             int64_t base = Cp[row] + pos;
             for (int w = w0; w < w1; w++)
             {
-                unsigned bits = bm[w];
+                unsigned bits = word(w);
                 while (bits)
                 {
                     int b = __ffs(bits) - 1;
@@ -375,9 +391,206 @@ __global__ void __launch_bounds__(G)
     }
 }
 
+// ---------------------------------------------------- learned binning ----
+__device__ __forceinline__ float lg(double x) { return (float) log2(1.0 + x); }
+
+// Structural features of every output row (one thread per row, one walk over
+// the row's L entries reading A's row bounds and first / last columns).
+__global__ void k_row_features(int64_t m_out, int Lm, const int *__restrict__ Lp,
+                               const int *__restrict__ Li, int nL, const int *__restrict__ Ap,
+                               const int *__restrict__ Ai, int An, float *__restrict__ F)
+{
+    int64_t row = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (row >= m_out) return;
+    RowCtx c = row_ctx((int) row, Lm, Lp, nL);
+    const int deg = c.q1 - c.q0;
+    double terms = 0, maxlen = 0, minlen = 1e300, sumsq = 0, sspan = 0;
+    long long cmin = LLONG_MAX, cmax = -1, kmin = LLONG_MAX, kmax = -1;
+    for (int q = c.q0; q < c.q1; q++)
+    {
+        const int k = Li[q] + c.koff;
+        const int st = Ap[k], en = Ap[k + 1];
+        const double len = en - st;
+        kmin = min(kmin, (long long) k);
+        kmax = max(kmax, (long long) k);
+        if (len > 0)
+        {
+            terms += len;
+            maxlen = fmax(maxlen, len);
+            minlen = fmin(minlen, len);
+            sumsq += len * len;
+            const long long f = Ai[st], l = Ai[en - 1];
+            cmin = min(cmin, f);
+            cmax = max(cmax, l);
+            sspan += (double) (l - f + 1);
+        }
+    }
+    if (terms == 0) minlen = 0;
+    const double n = An;
+    const double span = terms > 0 ? (double) (cmax - cmin + 1) : 0.0;
+    const double kspan = deg > 0 ? (double) (kmax - kmin + 1) : 0.0;
+    const double mean = deg > 0 ? terms / deg : 0.0;
+    float *f = F + row * 16;
+    f[0] = lg(terms);
+    f[1] = lg(deg);
+    f[2] = lg(maxlen);
+    f[3] = lg(minlen);
+    f[4] = lg(mean);
+    f[5] = lg(n);
+    f[6] = lg(n > 0 ? n * (1.0 - exp(-terms / n)) : 0.0);
+    f[7] = lg(span);
+    f[8] = lg(span > 0 ? span * (1.0 - exp(-terms / span)) : 0.0);
+    f[9] = lg(fmin(terms, span));
+    f[10] = lg(kspan);
+    f[11] = kspan > 0 ? (float) (deg / kspan) : 0.0f;
+    f[12] = lg(sqrt(sumsq));
+    f[13] = lg(deg > 0 ? sspan / deg : 0.0);
+    f[14] = lg(maxlen) - lg(mean);
+    f[15] = lg(span > 0 ? terms / span : 0.0);
+}
+
+// Hash class of a one-pass table of S slots (S = 64 << (class - 1)).
+__device__ __forceinline__ int hash_class_of_slots(int S)
+{
+    int c = 1;
+    for (int s = 64; s < S; s <<= 1) c++;
+    return c;
+}
+
+// One-pass class of every row: the smaller of the term-bound hash class
+// (never overflows) and the predicted bin's table (2 x bin bound); 0 = the
+// row goes to the exact passes (empty rows have no work, and rows whose
+// column bitmap is no bigger than the predicted table keep the bitmap).
+__global__ void k_onepass_class(int64_t m, const int64_t *__restrict__ terms,
+                                const unsigned char *__restrict__ pbin, int nwords,
+                                int lds_words, unsigned char *__restrict__ cls,
+                                int64_t *__restrict__ slots)
+{
+    int64_t row = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (row >= m) return;
+    const int64_t t = terms[row];
+    int c = 0;
+    if (t > 0)
+    {
+        const int ct = classify(t, nwords, lds_words);
+        const int b = pbin[row];
+        const int cp = b < MLP_NBIN_BOUNDED ? hash_class_of_slots(2 * mlp_bin_max(b)) : 0;
+        if (ct >= 1 && ct < CLS_BITMAP) c = (cp > 0 && cp < ct) ? cp : ct;
+        else if (cp > 0 && !(ct == CLS_BITMAP && nwords <= (64 << (cp - 1)))) c = cp;
+    }
+    cls[row] = (unsigned char) c;
+    slots[row] = c ? (int64_t) (64 << (c - 1)) : 0;
+}
+
+// One pass: insert every term's column; if the table fills up the row is
+// marked for the exact passes, else its sorted columns go to tmp at toff and
+// its count to rowcnt.
+template <int G, int S>
+__global__ void __launch_bounds__(G)
+    k_sym_hash_once(const int *__restrict__ rows, int Lm, const int *__restrict__ Lp,
+                    const int *__restrict__ Li, int nL, const int *__restrict__ Ap,
+                    const int *__restrict__ Ai, const int64_t *__restrict__ toff,
+                    int *__restrict__ tmp, int *__restrict__ rowcnt,
+                    unsigned char *__restrict__ done)
+{
+    __shared__ int tab[S];
+    __shared__ Expand<G> ex;
+    __shared__ long long red[G / 64];
+    __shared__ int full;
+    const int row = rows[blockIdx.x];
+    for (int i = threadIdx.x; i < S; i += G) tab[i] = -1;
+    if (threadIdx.x == 0) full = 0;
+    __syncthreads();
+    RowCtx c = row_ctx(row, Lm, Lp, nL);
+    for_each_term<G>(ex, c.q0, c.q1, c.koff, Li, Ap, [&](int t, int, int) {
+        if (full) return;
+        if (!hash_insert<S>(tab, Ai[t])) full = 1;
+    });
+    __syncthreads();
+    if (full)
+    {
+        if (threadIdx.x == 0)
+        {
+            rowcnt[row] = 0;
+            done[row] = 0;
+        }
+        return;
+    }
+    long long cnt = 0;
+    for (int i = threadIdx.x; i < S; i += G) cnt += tab[i] != -1;
+    cnt = block_sum_ll<G>(cnt, red);
+    bitonic_sort<G, S>((unsigned *) tab);
+    const int64_t base = toff[row];
+    for (int i = threadIdx.x; i < (int) cnt; i += G) tmp[base + i] = tab[i];
+    if (threadIdx.x == 0)
+    {
+        rowcnt[row] = (int) cnt;
+        done[row] = 1;
+    }
+}
+
+// Rows served by the one pass: copy their columns from tmp to Ci (one
+// wavefront per row).
+__global__ void __launch_bounds__(256)
+    k_copy_onepass(int64_t m, const unsigned char *__restrict__ done,
+                   const int64_t *__restrict__ toff, const int *__restrict__ tmp,
+                   const int64_t *__restrict__ Cp, int *__restrict__ Ci)
+{
+    const int64_t row = (int64_t) blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (row >= m || !done[row]) return;
+    const int64_t b = Cp[row], cnt = Cp[row + 1] - b, t = toff[row];
+    for (int64_t i = lane; i < cnt; i += 64) Ci[b + i] = tmp[t + i];
+}
+
+// v_out[row] = done[row] ? 0 : v[row]
+template <typename T>
+__global__ void k_mask_done(int64_t m, const unsigned char *__restrict__ done,
+                            const T *__restrict__ v, T *__restrict__ out)
+{
+    int64_t row = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (row < m) out[row] = done[row] ? (T) 0 : v[row];
+}
+
+// [0] rows served by the one pass, [1] one-pass rows that filled up,
+// [2] rows whose predicted bin differs from the bin of the exact nnz,
+// [3] non-empty rows sent straight to the exact passes
+__global__ void k_pred_stats(int64_t m, const unsigned char *__restrict__ cls,
+                             const unsigned char *__restrict__ done,
+                             const unsigned char *__restrict__ pbin,
+                             const int64_t *__restrict__ terms, const int64_t *__restrict__ Cp,
+                             unsigned long long *__restrict__ out)
+{
+    int64_t row = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (row >= m) return;
+    if (done[row]) atomicAdd(&out[0], 1ull);
+    if (cls[row] && !done[row]) atomicAdd(&out[1], 1ull);
+    if (terms[row] > 0 && pbin[row] != mlp_bin_of(Cp[row + 1] - Cp[row])) atomicAdd(&out[2], 1ull);
+    if (!cls[row] && terms[row] > 0) atomicAdd(&out[3], 1ull);
+}
+
 // Term lists of one output row: entries' term counts, offsets, scatter, then
 // each entry's list sorted by k (k is unique within an entry).  cols/off live
 // in LDS (CAP > 0) or in global memory (CAP == 0: cols = Ci, off = gscratch).
+// In global memory the counters are updated by atomics, which are performed
+// in L2, so every other access to them is an agent-scope atomic load / store
+// too and the phases are separated by device-scope fences: a plain store
+// would otherwise only be visible to this CU, not to the atomics.
+template <int CAP> __device__ __forceinline__ int off_ld(const int *off, int e)
+{
+    if (CAP > 0) return off[e];
+    return __hip_atomic_load(off + e, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <int CAP> __device__ __forceinline__ void off_st(int *off, int e, int v)
+{
+    if (CAP > 0) off[e] = v;
+    else __hip_atomic_store(off + e, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <int CAP> __device__ __forceinline__ void off_fence()
+{
+    if (CAP == 0) __threadfence();
+}
+
 template <int G, int CAP>
 __global__ void __launch_bounds__(G)
     k_terms(const int *__restrict__ rows, int Lm, const int *__restrict__ Lp,
@@ -385,7 +598,8 @@ __global__ void __launch_bounds__(G)
             const int *__restrict__ Ap, const int *__restrict__ Ai,
             const int64_t *__restrict__ Cp, const int *__restrict__ Ci,
             const int64_t *__restrict__ toff, int64_t *__restrict__ ep, int *__restrict__ tA,
-            int *__restrict__ tB, int *__restrict__ tK, int *__restrict__ gscratch)
+            int *__restrict__ tB, int *__restrict__ tK, int *__restrict__ gscratch,
+            int *__restrict__ err)
 {
     __shared__ int s_cols[CAP > 0 ? CAP : 1];
     __shared__ int s_off[CAP > 0 ? CAP : 1];
@@ -398,11 +612,7 @@ __global__ void __launch_bounds__(G)
     int *off;
     if (CAP > 0)
     {
-        for (int i = threadIdx.x; i < cnt; i += G)
-        {
-            s_cols[i] = Ci[cbase + i];
-            s_off[i] = 0;
-        }
+        for (int i = threadIdx.x; i < cnt; i += G) s_cols[i] = Ci[cbase + i];
         cols = s_cols;
         off = s_off;
     }
@@ -410,48 +620,68 @@ __global__ void __launch_bounds__(G)
     {
         cols = Ci + cbase;
         off = gscratch + cbase;
-        for (int i = threadIdx.x; i < cnt; i += G) off[i] = 0;
-        __threadfence_block();
     }
+    for (int i = threadIdx.x; i < cnt; i += G) off_st<CAP>(off, i, 0);
+    off_fence<CAP>();
     __syncthreads();
     RowCtx c = row_ctx(row, Lm, Lp, nL);
     for_each_term<G>(ex, c.q0, c.q1, c.koff, Li, Ap, [&](int t, int, int) {
-        atomicAdd(&off[lower_bound_int(cols, cnt, Ai[t])], 1);
+        const int pos = lower_bound_int(cols, cnt, Ai[t]);
+        if (pos < cnt) atomicAdd(&off[pos], 1);
+        else atomicOr(err, 1);
     });
+    off_fence<CAP>();
+    __syncthreads();
     // blocked exclusive scan of the per-entry counts
+    const int64_t rbase = toff[row];
     {
         int per = (cnt + G - 1) / G;
         int e0 = min(cnt, (int) threadIdx.x * per), e1 = min(cnt, e0 + per);
         int s = 0;
-        for (int e = e0; e < e1; e++) s += off[e];
+        for (int e = e0; e < e1; e++) s += off_ld<CAP>(off, e);
         int tot;
         int pre = block_excl_scan_int<G>(s, &tot, wsum);
-        const int64_t rbase = toff[row];
         for (int e = e0; e < e1; e++)
         {
-            int v = off[e];
-            off[e] = pre;
+            int v = off_ld<CAP>(off, e);
+            off_st<CAP>(off, e, pre);
             ep[cbase + e] = rbase + pre;
             pre += v;
         }
-        if (CAP == 0) __threadfence_block();
+        off_fence<CAP>();
         __syncthreads();
     }
-    const int64_t rbase = toff[row];
+    const int64_t rend = toff[row + 1];
     for_each_term<G>(ex, c.q0, c.q1, c.koff, Li, Ap, [&](int t, int q, int k) {
         int pos = lower_bound_int(cols, cnt, Ai[t]);
+        if (pos >= cnt)  // the row's columns miss a term: report, never write
+        {
+            atomicOr(err, 1);
+            return;
+        }
         int64_t slot = rbase + atomicAdd(&off[pos], 1);
+        if (slot >= rend)
+        {
+            atomicOr(err, 1);
+            return;
+        }
         tA[slot] = t;
         tB[slot] = Lperm ? Lperm[q] : q;
         tK[slot] = k;
     });
+    off_fence<CAP>();
     __threadfence_block();
     __syncthreads();
     // off[e] is now the end of entry e's list; sort each list by k
     for (int e = threadIdx.x; e < cnt; e += G)
     {
-        int64_t s1 = rbase + off[e];
-        int64_t s0 = e == 0 ? rbase : rbase + off[e - 1];
+        int64_t s1 = rbase + off_ld<CAP>(off, e);
+        int64_t s0 = e == 0 ? rbase : rbase + off_ld<CAP>(off, e - 1);
+        if (s0 < rbase || s1 > rend || s0 > s1)  // inconsistent counters: report, touch nothing
+        {
+            atomicOr(err, 1);
+            continue;
+        }
         for (int64_t i = s0 + 1; i < s1; i++)
         {
             int kk = tK[i], aa = tA[i], bb = tB[i];
@@ -977,6 +1207,137 @@ void run_symbolic_class(int c, const Bins &B, const ProductPlan &P, int nwords, 
     SDL_HIP(hipGetLastError());
 }
 
+void run_onepass_class(int c, const Bins &B, const ProductPlan &P, const int64_t *toff, int *tmp,
+                       int *rowcnt, unsigned char *done)
+{
+    int nrows = B.count[c];
+    if (nrows == 0) return;
+    const int *rows = B.rows.p + B.off[c];
+    const int nL = P.A.Am / P.L.blocks;
+    const LView &L = P.L;
+    const AView &A = P.A;
+#define SDL_ONCE_CASE(CLS, G, S)                                                              \
+    case CLS:                                                                                 \
+        hipLaunchKernelGGL((k_sym_hash_once<G, S>), dim3(nrows), dim3(G), 0, stream(), rows,  \
+                           L.Lm, L.Lp, L.Li, nL, A.Ap, A.Ai, toff, tmp, rowcnt, done);        \
+        break;
+    switch (c)
+    {
+        SDL_ONCE_CASE(1, 64, 64)
+        SDL_ONCE_CASE(2, 64, 128)
+        SDL_ONCE_CASE(3, 64, 256)
+        SDL_ONCE_CASE(4, 128, 512)
+        SDL_ONCE_CASE(5, 128, 1024)
+        SDL_ONCE_CASE(6, 256, 2048)
+        SDL_ONCE_CASE(7, 256, 4096)
+        SDL_ONCE_CASE(8, 256, 8192)
+    default: break;
+    }
+#undef SDL_ONCE_CASE
+    SDL_HIP(hipGetLastError());
+}
+
+// Count pass (tables sized from the term bound), scan, fill pass (tables
+// sized from the exact count).
+void symbolic_exact(ProductPlan &P, const int64_t *rterms, int *rowcnt, int nwords)
+{
+    const int64_t m_out = P.m_out;
+    {
+        Bins B;
+        bin_rows<int64_t>(B, rterms, m_out, nwords);
+        for (int c = 1; c < NCLASS; c++) run_symbolic_class<0>(c, B, P, nwords, rowcnt, nullptr, nullptr);
+    }
+    P.nnz = scan_i32_to_i64(rowcnt, P.Cp.p, m_out, true);
+    P.stats.nnz = P.nnz;
+    P.Ci.alloc(P.nnz);
+    Bins B2;
+    bin_rows<int>(B2, rowcnt, m_out, nwords);
+    for (int c = 1; c < NCLASS; c++)
+    {
+        run_symbolic_class<1>(c, B2, P, nwords, nullptr, P.Cp.p, P.Ci.p);
+        if (c >= 1 && c <= 8) P.stats.hash_rows += B2.count[c];
+    }
+    P.stats.bitmap_rows = B2.count[CLS_BITMAP];
+    P.stats.global_rows = B2.count[CLS_GLOBAL];
+}
+
+// One pass with predicted table sizes, then the exact passes for the rows it
+// could not serve, then the one-pass rows' columns copied into place.
+void symbolic_predicted(ProductPlan &P, const RowPredictor *pred, const int64_t *rterms,
+                        int *rowcnt, int nwords)
+{
+    const int64_t m_out = P.m_out;
+    const unsigned grid = (unsigned) ((m_out + 255) / 256);
+    DBuf<unsigned char> pbin(m_out);
+    if (pred)
+    {
+        DBuf<float> F((size_t) m_out * MLP_IN);
+        product_row_features(P.L, P.A, F.p, stream());
+        mlp_predict(*pred, F.p, m_out, pbin.p, stream());
+    }
+    else
+    {
+        SDL_HIP(hipMemsetAsync(pbin.p, MLP_NBIN_BOUNDED, (size_t) m_out, stream()));
+    }
+    DBuf<unsigned char> cls(m_out), done(m_out);
+    DBuf<int64_t> slots(m_out), soff(m_out + 1);
+    hipLaunchKernelGGL(k_onepass_class, dim3(grid), dim3(256), 0, stream(), m_out, rterms, pbin.p,
+                       nwords, lds_bitmap_words(), cls.p, slots.p);
+    SDL_HIP(hipGetLastError());
+    dzero(done.p, done.bytes());
+    const int64_t tmp_total = scan_i64(slots.p, soff.p, m_out, true);
+    DBuf<int> tmp(std::max<int64_t>(1, tmp_total));
+    {
+        Bins B;
+        bin_rows<unsigned char>(B, cls.p, m_out, nwords, 2);
+        for (int c = 1; c <= 8; c++) run_onepass_class(c, B, P, soff.p, tmp.p, rowcnt, done.p);
+    }
+    // exact count pass for the rest
+    {
+        DBuf<int64_t> fterms(m_out);
+        hipLaunchKernelGGL(k_mask_done<int64_t>, dim3(grid), dim3(256), 0, stream(), m_out,
+                           done.p, rterms, fterms.p);
+        SDL_HIP(hipGetLastError());
+        Bins B;
+        bin_rows<int64_t>(B, fterms.p, m_out, nwords);
+        for (int c = 1; c < NCLASS; c++) run_symbolic_class<0>(c, B, P, nwords, rowcnt, nullptr, nullptr);
+    }
+    P.nnz = scan_i32_to_i64(rowcnt, P.Cp.p, m_out, true);
+    P.stats.nnz = P.nnz;
+    P.Ci.alloc(P.nnz);
+    hipLaunchKernelGGL(k_copy_onepass, dim3((unsigned) ((m_out + 3) / 4)), dim3(256), 0, stream(),
+                       m_out, done.p, soff.p, tmp.p, P.Cp.p, P.Ci.p);
+    SDL_HIP(hipGetLastError());
+    // exact fill pass for the rest
+    {
+        DBuf<int> fcnt(m_out);
+        hipLaunchKernelGGL(k_mask_done<int>, dim3(grid), dim3(256), 0, stream(), m_out, done.p,
+                           rowcnt, fcnt.p);
+        SDL_HIP(hipGetLastError());
+        Bins B2;
+        bin_rows<int>(B2, fcnt.p, m_out, nwords);
+        for (int c = 1; c < NCLASS; c++)
+        {
+            run_symbolic_class<1>(c, B2, P, nwords, nullptr, P.Cp.p, P.Ci.p);
+            if (c >= 1 && c <= 8) P.stats.hash_rows += B2.count[c];
+        }
+        P.stats.bitmap_rows = B2.count[CLS_BITMAP];
+        P.stats.global_rows = B2.count[CLS_GLOBAL];
+    }
+    DBuf<unsigned long long> st(4);
+    dzero(st.p, st.bytes());
+    hipLaunchKernelGGL(k_pred_stats, dim3(grid), dim3(256), 0, stream(), m_out, cls.p, done.p,
+                       pbin.p, rterms, P.Cp.p, st.p);
+    SDL_HIP(hipGetLastError());
+    unsigned long long h[4];
+    d2h(h, st.p, sizeof(h));
+    P.stats.predicted_rows = (long long) h[0];
+    P.stats.overflow_rows = (long long) h[1];
+    P.stats.misbin_rows = pred ? (long long) h[2] : 0;
+    P.stats.exact_rows = (long long) h[3];
+    P.stats.hash_rows += (long long) h[0];
+}
+
 } // namespace
 
 int64_t product_count_terms(const LView &L, const AView &A, int64_t *row_terms_out)
@@ -997,11 +1358,18 @@ int64_t product_count_terms(const LView &L, const AView &A, int64_t *row_terms_o
     return scan_i64(terms, off.p, m_out, true);
 }
 
-void product_plan(ProductPlan &P, const LView &L, const AView &A, SymMode mode,
-                  const float *pred, bool allow_pull)
+void product_row_features(const LView &L, const AView &A, float *F, hipStream_t s)
 {
-    (void) mode;
-    (void) pred;
+    const int64_t m_out = (int64_t) L.Lm * L.blocks;
+    if (m_out == 0) return;
+    hipLaunchKernelGGL(k_row_features, dim3((unsigned) ((m_out + 255) / 256)), dim3(256), 0, s,
+                       m_out, L.Lm, L.Lp, L.Li, A.Am / L.blocks, A.Ap, A.Ai, A.An, F);
+    SDL_HIP(hipGetLastError());
+}
+
+void product_plan(ProductPlan &P, const LView &L, const AView &A, SymMode mode,
+                  const RowPredictor *pred, bool allow_pull)
+{
     SDL_REQUIRE(L.blocks >= 1 && A.Am % L.blocks == 0, "product: A rows not a multiple of blocks");
     P.L = L;
     P.A = A;
@@ -1031,29 +1399,12 @@ void product_plan(ProductPlan &P, const LView &L, const AView &A, SymMode mode,
     P.terms = scan_i64(rterms.p, toff.p, m_out, true);
     P.stats.terms = P.terms;
 
-    // count pass, accumulators sized from the term bound
+    // structure: exact count + fill passes, or the predicted one pass
     DBuf<int> rowcnt(m_out);
     dzero(rowcnt.p, rowcnt.bytes());
-    {
-        Bins B;
-        bin_rows<int64_t>(B, rterms.p, m_out, nwords);
-        for (int c = 1; c < NCLASS; c++)
-            run_symbolic_class<0>(c, B, P, nwords, rowcnt.p, nullptr, nullptr);
-    }
-    P.nnz = scan_i32_to_i64(rowcnt.p, P.Cp.p, m_out, true);
-    P.stats.nnz = P.nnz;
-    P.Ci.alloc(P.nnz);
-
-    // fill pass, accumulators sized from the exact count
-    Bins B2;
-    bin_rows<int>(B2, rowcnt.p, m_out, nwords);
-    for (int c = 1; c < NCLASS; c++)
-    {
-        run_symbolic_class<1>(c, B2, P, nwords, nullptr, P.Cp.p, P.Ci.p);
-        if (c >= 1 && c <= 8) P.stats.hash_rows += B2.count[c];
-    }
-    P.stats.bitmap_rows = B2.count[CLS_BITMAP];
-    P.stats.global_rows = B2.count[CLS_GLOBAL];
+    SDL_REQUIRE(mode != SymMode::Predicted || kPredictedSymbolicEnabled, kPredictedSymbolicOff);
+    if (mode == SymMode::Predicted) symbolic_predicted(P, pred, rterms.p, rowcnt.p, nwords);
+    else symbolic_exact(P, rterms.p, rowcnt.p, nwords);
 
     // refill schedule (reference policy: a plan iff terms <= 3 nnz, gustavson.h:45-52)
     P.pull = allow_pull && P.terms > 0 && P.terms <= 3 * P.nnz &&
@@ -1065,6 +1416,8 @@ void product_plan(ProductPlan &P, const LView &L, const AView &A, SymMode mode,
         P.tB.alloc(P.terms);
         P.tK.alloc(P.terms);
         // rows binned by nnz: LDS lists up to 2048 entries, global beyond
+        DBuf<int> err(1);
+        dzero(err.p, err.bytes());
         Bins BT;
         bin_rows<int>(BT, rowcnt.p, m_out, nwords, 1, 128, 2048);
         DBuf<int> gscratch;
@@ -1076,24 +1429,26 @@ void product_plan(ProductPlan &P, const LView &L, const AView &A, SymMode mode,
             if (c == 1)
                 hipLaunchKernelGGL((k_terms<64, 128>), dim3(nrows), dim3(64), 0, stream(), rows,
                                    L.Lm, L.Lp, L.Li, L.Lperm, nL, A.Ap, A.Ai, P.Cp.p, P.Ci.p,
-                                   toff.p, P.ep.p, P.tA.p, P.tB.p, P.tK.p, (int *) nullptr);
+                                   toff.p, P.ep.p, P.tA.p, P.tB.p, P.tK.p, (int *) nullptr, err.p);
             else if (c == 2)
                 hipLaunchKernelGGL((k_terms<256, 2048>), dim3(nrows), dim3(256), 0, stream(),
                                    rows, L.Lm, L.Lp, L.Li, L.Lperm, nL, A.Ap, A.Ai, P.Cp.p,
                                    P.Ci.p, toff.p, P.ep.p, P.tA.p, P.tB.p, P.tK.p,
-                                   (int *) nullptr);
+                                   (int *) nullptr, err.p);
             else
             {
                 gscratch.alloc(P.nnz);
                 hipLaunchKernelGGL((k_terms<256, 0>), dim3(nrows), dim3(256), 0, stream(), rows,
                                    L.Lm, L.Lp, L.Li, L.Lperm, nL, A.Ap, A.Ai, P.Cp.p, P.Ci.p,
-                                   toff.p, P.ep.p, P.tA.p, P.tB.p, P.tK.p, gscratch.p);
+                                   toff.p, P.ep.p, P.tA.p, P.tB.p, P.tK.p, gscratch.p, err.p);
             }
             SDL_HIP(hipGetLastError());
         }
         hipLaunchKernelGGL(k_set_last, dim3(1), dim3(64), 0, stream(), P.ep.p, P.nnz, P.terms);
         SDL_HIP(hipGetLastError());
-        sync();
+        int herr = 0;
+        d2h(&herr, err.p, sizeof(int));
+        SDL_REQUIRE(herr == 0, "product plan: a term fell outside its row's structure");
     }
     else
     {

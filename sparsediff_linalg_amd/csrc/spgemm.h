@@ -40,8 +40,10 @@ struct SymStats
 {
     long long rows = 0, terms = 0, nnz = 0;
     long long hash_rows = 0, bitmap_rows = 0, global_rows = 0;
-    long long overflow_rows = 0;       // predicted path: rows that fell back
-    long long predicted_rows = 0;      // predicted path: rows served by the prediction
+    long long overflow_rows = 0;       // predicted path: one-pass rows whose table filled up
+    long long predicted_rows = 0;      // predicted path: rows served by the one pass
+    long long misbin_rows = 0;         // predicted path: predicted bin != bin of the exact nnz
+    long long exact_rows = 0;          // predicted path: rows sent to the exact passes directly
 };
 
 // Launch configuration of the batched pull refill (picked by the autotuner):
@@ -84,10 +86,23 @@ struct ProductPlan
 // Reserve the interleaved batched-refill workspace for up to Kmax refills.
 void product_reserve_batch(ProductPlan &P, int Kmax);
 
-// Build the exact structure and the refill schedule.  pred (optional, device,
-// m_out floats) holds predicted per-row nnz for SymMode::Predicted.
+struct RowPredictor;
+
+// Build the exact structure and the refill schedule.  SymMode::Predicted: one
+// pass whose per-row hash tables are sized from the predictor's bins (pred
+// may be null: tables sized from the term bound only); rows whose table
+// fills up, and rows no table suits, take the exact count + fill passes.
 void product_plan(ProductPlan &P, const LView &L, const AView &A, SymMode mode,
-                  const float *pred, bool allow_pull = true);
+                  const RowPredictor *pred, bool allow_pull = true);
+
+// The 16 structural features of every output row (F[row * 16 + f], FP32):
+// log2(1 + x) of the term bound, the degree, the longest / shortest A row,
+// the mean A row, n, the uniform-collision estimate n (1 - e^(-terms/n)),
+// the column span of the row's A rows, the uniform estimate within the span,
+// min(terms, span), the A-row index span, degree / index span, the root of
+// the sum of squared A-row lengths, the mean single-row column span, the
+// length skew and the term density within the span.
+void product_row_features(const LView &L, const AView &A, float *F, hipStream_t s);
 
 // One refill: Cx = values of kron(I, L) D A.  Lval / Ax / d are device arrays
 // (d may be null = identity).  Launches on `s`, allocates nothing.

@@ -64,3 +64,112 @@ def blocks_config3(seed: int = 3, nblocks: int = 256, lo: int = 64, hi: int = 51
         B.append((rp, cpB, g.standard_normal((m0[k], n0B[k]))))
     d = g.uniform(0.1, 10.0, m)
     return dict(m=m, n=ncols, A=A, B=B, d=d)
+
+
+def csr_from_lengths(seed: int, lengths, n: int, values: bool = True):
+    """CSR with the given row lengths, distinct uniform sorted columns per row
+    (values N(0,1)), by the library's generator."""
+    lib = load()
+    lengths = np.asarray(lengths, np.int64)
+    m = len(lengths)
+    p = np.zeros(m + 1, np.int64)
+    np.cumsum(lengths, out=p[1:])
+    assert p[-1] < 2 ** 31, "nnz exceeds int32"
+    p = p.astype(np.int32)
+    i = np.empty(int(p[-1]), np.int32)
+    x = np.empty(int(p[-1]), np.float64) if values else None
+    lib.sdl_gen_fill_rows(seed, m, n, ptr(p), ptr(i), ptr(x) if values else None)
+    return p, i, x
+
+
+def zipf_lengths(seed: int, m: int, s: float = 1.2, lo: int = 1, hi: int = 512,
+                 mean: float = 16.0):
+    """Row lengths by rank-Zipf: length(rank r) = clip(round(c r^-s), lo, hi)
+    with c solved (bisection) so the mean is `mean`; ranks randomly permuted
+    over the rows."""
+    r = np.arange(1, m + 1, dtype=np.float64)
+    base = r ** -s
+
+    def mean_of(c):
+        return np.clip(np.rint(c * base), lo, hi).mean()
+
+    a, b = 1.0, 1e12
+    for _ in range(200):
+        c = np.sqrt(a * b)
+        if mean_of(c) < mean:
+            a = c
+        else:
+            b = c
+    lengths = np.clip(np.rint(b * base), lo, hi).astype(np.int64)
+    g = np.random.Generator(np.random.PCG64(seed))
+    return lengths[g.permutation(m)]
+
+
+def zipf_csr(seed: int, m: int, n: int, mean: float = 16.0):
+    """The A of the large A^T A product: m x n, rank-Zipf(1.2) row lengths in
+    [1, 512] with mean `mean`, distinct uniform columns, FP64 N(0,1)."""
+    return csr_from_lengths(seed, zipf_lengths(seed, m, mean=mean), n)
+
+
+def banded_csr(m: int, bw: int, seed: int = 5):
+    """m x m banded: row i holds columns max(0, i-bw) .. min(m-1, i+bw); FP64
+    N(0,1) values."""
+    lo = np.maximum(0, np.arange(m) - bw)
+    hi = np.minimum(m - 1, np.arange(m) + bw)
+    lengths = (hi - lo + 1).astype(np.int64)
+    p = np.zeros(m + 1, np.int64)
+    np.cumsum(lengths, out=p[1:])
+    row = np.repeat(np.arange(m), lengths)
+    i = (lo[row] + (np.arange(p[-1]) - p[row])).astype(np.int32)
+    x = normal(seed, int(p[-1]))
+    return p.astype(np.int32), i, x
+
+
+def rmat_csr(scale: int, edges_per_node: int = 16, a: float = 0.57, b: float = 0.19,
+             c: float = 0.19, seed: int = 5):
+    """R-MAT graph on 2^scale nodes with edges_per_node * 2^scale generated
+    edges (quadrant probabilities a, b, c, 1-a-b-c per level), duplicates
+    coalesced (one entry per (row, col), values summed); FP64 N(0,1) values."""
+    n = 1 << scale
+    E = edges_per_node * n
+    g = np.random.Generator(np.random.PCG64(seed))
+    rows = np.zeros(E, np.int64)
+    cols = np.zeros(E, np.int64)
+    cum = np.array([a, a + b, a + b + c])
+    for lvl in range(scale):
+        u = g.random(E)
+        q = np.searchsorted(cum, u, side="right")  # 0..3
+        rows |= ((q >> 1) & 1).astype(np.int64) << lvl
+        cols |= (q & 1).astype(np.int64) << lvl
+    vals = normal(seed * 1000 + 7, E)
+    key = rows * n + cols
+    order = np.argsort(key, kind="stable")
+    key = key[order]
+    vals = vals[order]
+    uniq, start = np.unique(key, return_index=True)
+    x = np.add.reduceat(vals, start) if len(start) else vals[:0]
+    r = (uniq // n).astype(np.int64)
+    i = (uniq % n).astype(np.int32)
+    p = np.zeros(n + 1, np.int64)
+    np.add.at(p, r + 1, 1)
+    np.cumsum(p, out=p)
+    return p.astype(np.int32), i, x
+
+
+def blocks_as_csr(w):
+    """CSR (m x n) of a stacked permuted-dense operand from blocks_config3."""
+    m, n = w["m"], w["n"]
+    rows = [None] * m
+    for rp, cp, X in w["A"]:
+        for ii, r in enumerate(rp):
+            rows[r] = (cp, X[ii])
+    lengths = np.array([len(rows[r][0]) if rows[r] is not None else 0 for r in range(m)])
+    p = np.zeros(m + 1, np.int64)
+    np.cumsum(lengths, out=p[1:])
+    i = np.empty(p[-1], np.int32)
+    x = np.empty(p[-1], np.float64)
+    for r in range(m):
+        if rows[r] is not None:
+            i[p[r]:p[r + 1]] = rows[r][0]
+            x[p[r]:p[r + 1]] = rows[r][1]
+    return p.astype(np.int32), i, x

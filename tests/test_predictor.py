@@ -1,0 +1,242 @@
+"""GPU: the learned row binning.  Features, logits and gradients of the HIP
+kernels against the CPU FP64 oracle (oracle/mlp_oracle.py, itself pinned by
+finite differences); training on the GPU lowers the loss; predicted plans give
+the exact plan's structure bit for bit (trained, absent and adversarial
+predictors, overflow fallback); parameter files round-trip and reject
+corruption; the drop-in allocs use a bound predictor; and the trained
+network's gain over the one pass without it on a banded A^T A."""
+import ctypes as C
+import os
+import time
+
+import numpy as np
+import pytest
+
+from oracle import mlp_oracle as MO
+from sparsediff_linalg_amd import DeviceArray, check, load, workloads as W
+from sparsediff_linalg_amd.predictor import Plan, Predictor, training_products, training_set
+from tests.helpers import random_csr
+
+pytestmark = pytest.mark.gpu
+SKIP_PREDICTED = pytest.mark.skip(reason="the predicted one-pass symbolic phase is switched "
+                                  "off: its first GPU test ended in a device memory fault "
+                                  "whose cause is not yet found (CPU emulation of the same "
+                                  "kernels under AddressSanitizer is clean)")
+
+
+def csc_of(p, i, m, n):
+    """CSC (column pointers, row indices ascending) of a CSR structure."""
+    p = np.asarray(p)
+    rows = np.repeat(np.arange(m), np.diff(p))
+    order = np.lexsort((rows, np.asarray(i)))
+    cp = np.zeros(n + 1, np.int64)
+    np.add.at(cp, np.asarray(i) + 1, 1)
+    return np.cumsum(cp), rows[order]
+
+
+@pytest.fixture(scope="module")
+def trained():
+    X, Y, n = training_set(training_products(small=True))
+    mlp = Predictor(seed=1)
+    mlp.train(X, Y, n, epochs=40, batch=2048, lr=3e-3, seed=7)
+    yield mlp
+    mlp.close()
+
+
+def test_features_match_oracle():
+    rng = np.random.default_rng(1)
+    m, nA, nB = 400, 90, 70
+    Ap, Ai, _ = random_csr(rng, m, nA, 0.05)
+    Bp, Bi, _ = random_csr(rng, m, nB, 0.08)
+    pl = Plan((Ap, Ai), (Bp, Bi), nA, nB)
+    F = pl.features().numpy()
+    Lp, Li = csc_of(Bp, Bi, m, nB)
+    ref = MO.row_features(Lp, Li, Ap, Ai, nA)
+    assert np.allclose(F, ref, rtol=1e-6, atol=1e-5), np.abs(F - ref).max()
+
+
+def test_forward_and_gradients_match_oracle():
+    rng = np.random.default_rng(2)
+    P = (rng.standard_normal(MO.NPARAM) * 0.2).astype(np.float32)
+    n = 3000
+    X = (rng.random((n, MO.NIN)) * 8).astype(np.float32)
+    Y = rng.integers(0, MO.NOUT, n).astype(np.int32)
+    mlp = Predictor()
+    mlp.set_params(P)
+    dX, dY = DeviceArray.from_numpy(X), DeviceArray.from_numpy(Y)
+    Z = DeviceArray((n, MO.NOUT), np.float32)
+    check(load().sdl_mlp_forward(mlp.handle, dX.ptr, n, Z.ptr))
+    zref = MO.forward(P.astype(np.float64), X.astype(np.float64))
+    assert np.allclose(Z.numpy(), zref, rtol=1e-5, atol=1e-4)
+    G = DeviceArray(MO.NPARAM, np.float32)
+    loss = C.c_float()
+    check(load().sdl_mlp_gradients(mlp.handle, dX.ptr, dY.ptr, n, G.ptr, C.byref(loss)))
+    lref, gref = MO.loss_and_grad(P.astype(np.float64), X.astype(np.float64), Y)
+    g = G.numpy()
+    assert abs(loss.value - lref) <= 1e-5 * abs(lref) + 1e-6
+    assert np.allclose(g, gref, rtol=1e-3, atol=1e-6), np.abs(g - gref).max()
+    assert np.linalg.norm(g - gref) <= 1e-4 * np.linalg.norm(gref)
+    # the tau rule on the device matches the oracle's
+    bins = mlp.predict(dX, n)
+    assert np.mean(bins == MO.predict_bins(P.astype(np.float64), X.astype(np.float64), 0.95)) > 0.999
+    mlp.close()
+
+
+def test_training_lowers_loss_and_predicts_bins(trained):
+    X, Y, n = training_set(training_products(small=True))
+    fresh = Predictor(seed=1)
+    G = DeviceArray(MO.NPARAM, np.float32)
+    l0, l1 = C.c_float(), C.c_float()
+    check(load().sdl_mlp_gradients(fresh.handle, X.ptr, Y.ptr, n, G.ptr, C.byref(l0)))
+    check(load().sdl_mlp_gradients(trained.handle, X.ptr, Y.ptr, n, G.ptr, C.byref(l1)))
+    assert l1.value < 0.5 * l0.value
+    pred = trained.predict(X, n)
+    y = Y.numpy()
+    assert np.mean(pred == y) > 0.8
+    fresh.close()
+
+
+def structures_equal(a, b):
+    pa, ia = a.structure()
+    pb, ib = b.structure()
+    return np.array_equal(pa, pb) and np.array_equal(ia, ib)
+
+
+def cases():
+    rng = np.random.default_rng(4)
+    Ap, Ai, _ = random_csr(rng, 3000, 800, 0.01)
+    Bp, Bi, _ = random_csr(rng, 3000, 600, 0.012)
+    yield "uniform", (Ap, Ai), (Bp, Bi), 800, 600
+    p, i, _ = W.banded_csr(4096, 16)
+    yield "banded16", (p, i), None, 4096, 4096
+    p, i, _ = W.zipf_csr(11, 20000, 4000)
+    yield "zipf", (p, i), None, 4000, 4000
+    p, i, _ = W.rmat_csr(12)
+    yield "rmat12", (p, i), None, 1 << 12, 1 << 12
+    wb = W.blocks_config3(5, nblocks=12, lo=16, hi=64, ncols=512)
+    p, i, _ = W.blocks_as_csr(wb)
+    yield "blocks", (p, i), None, 512, 512
+
+
+@SKIP_PREDICTED
+@pytest.mark.parametrize("which", ["trained", "none", "adversarial"])
+def test_predicted_structure_is_exact(which, trained):
+    adv = None
+    if which == "adversarial":
+        adv = Predictor()
+        P = np.zeros(MO.NPARAM, np.float32)
+        P[-6] = 20.0  # every row predicted in the smallest bin: tables of 64 slots
+        adv.set_params(P)
+    mlp = {"trained": trained, "none": None, "adversarial": adv}[which]
+    overflow = 0
+    for name, A, B, nA, nB in cases():
+        ex = Plan(A, B, nA, nB)
+        pr = Plan(A, B, nA, nB, mode=1, mlp=mlp)
+        assert pr.nnz() == ex.nnz(), name
+        assert structures_equal(ex, pr), name
+        st = pr.pred_stats()
+        assert st["predicted"] + st["exact"] + st["overflow"] <= nB
+        overflow += st["overflow"]
+        ex.close()
+        pr.close()
+    if which == "adversarial":
+        assert overflow > 0  # the fallback was exercised
+        adv.close()
+
+
+def test_parameter_file_roundtrip_and_checksum(tmp_path, trained):
+    path = str(tmp_path / "p.bin")
+    trained.save(path)
+    other = Predictor(seed=9)
+    check(load().sdl_mlp_load(other.handle, path.encode()))
+    assert np.array_equal(other.params(), trained.params())
+    raw = bytearray(open(path, "rb").read())
+    raw[40] ^= 0x01
+    bad = str(tmp_path / "bad.bin")
+    open(bad, "wb").write(bytes(raw))
+    assert load().sdl_mlp_load(other.handle, bad.encode()) != 0
+    assert np.array_equal(other.params(), trained.params())  # unchanged on rejection
+    other.close()
+
+
+def test_shipped_parameters_load():
+    from sparsediff_linalg_amd.predictor import DEFAULT_PATH
+    if not os.path.exists(DEFAULT_PATH):
+        pytest.skip("no shipped predictor file")
+    p = Predictor.from_file(DEFAULT_PATH)
+    assert np.all(np.isfinite(p.params()))
+    p.close()
+
+
+@SKIP_PREDICTED
+def test_dropin_alloc_uses_bound_predictor(trained):
+    from tests import dropin as D
+    L = D.lib()
+    p, i, x = W.banded_csr(2048, 8)
+    A = D.sparse(p, i, x, 2048, 2048)
+    exact = L.ATA_matrices_alloc(A)
+    lib = load()
+    lib.sdl_set_row_predictor(trained.handle)
+    try:
+        pred = L.ATA_matrices_alloc(A)
+    finally:
+        lib.sdl_set_row_predictor(None)
+    pe, ie, _ = D.mat_csr(exact)
+    pp, ip, _ = D.mat_csr(pred)
+    assert np.array_equal(pe, pp) and np.array_equal(ie, ip)
+    d = np.linspace(0.5, 2.0, 2048)
+    L.ATDA_matrices_fill_values(A, D.dptr(d), exact)
+    L.ATDA_matrices_fill_values(A, D.dptr(d), pred)
+    assert np.array_equal(D.mat_csr(exact)[2].view(np.int64), D.mat_csr(pred)[2].view(np.int64))
+    for M in (exact, pred, A):
+        L.free_matrix(M)
+
+
+@SKIP_PREDICTED
+def test_trained_predictor_gain_on_banded(trained):
+    """A^T A of a banded matrix (bandwidth 64): every output row has ~16.6k
+    terms but only ~257 distinct columns, so the term bound puts every row in
+    a column bitmap while the predicted bin fits a 1024-slot LDS table.
+    Reports the three symbolic times; requires the exact structure, few
+    overflows, and the trained network faster than the one pass without it."""
+    n = 1 << 17
+    p, i, _ = W.banded_csr(n, 64)
+    lib = load()
+
+    def timed(**kw):
+        Plan((p, i), None, n, n, **kw).close()  # warm-up (kernels, allocator)
+        lib.sdl_synchronize()
+        t0 = time.perf_counter()
+        pl = Plan((p, i), None, n, n, **kw)
+        lib.sdl_synchronize()
+        return pl, time.perf_counter() - t0
+
+    ex, t_exact = timed()
+    none, t_none = timed(mode=1)
+    pr, t_pred = timed(mode=1, mlp=trained)
+    assert structures_equal(ex, pr) and structures_equal(ex, none)
+    st = pr.pred_stats()
+    print(f"\nbanded A^T A symbolic: exact {t_exact * 1e3:.1f} ms, one pass without the "
+          f"network {t_none * 1e3:.1f} ms, with it {t_pred * 1e3:.1f} ms; {st}")
+    assert st["overflow"] <= 0.05 * n
+    assert st["predicted"] >= 0.9 * n
+    assert t_pred < t_none
+    for pl in (ex, none, pr):
+        pl.close()
+
+
+def test_predicted_path_reports_switched_off(trained):
+    """While the predicted phase is off, every way in returns an error instead
+    of running it."""
+    lib = load()
+    p, i, _ = W.banded_csr(256, 4)
+    arrays = [DeviceArray.from_numpy(np.ascontiguousarray(a, np.int32)) for a in (p, i)]
+    h = C.c_void_p()
+    assert lib.sdl_btda_plan_create_predicted(256, 256, 256, arrays[0].ptr, arrays[1].ptr,
+                                              arrays[0].ptr, arrays[1].ptr, trained.handle,
+                                              C.byref(h)) != 0
+    assert b"switched off" in lib.sdl_last_error()
+    assert lib.sdl_btda_plan_create(256, 256, 256, arrays[0].ptr, arrays[1].ptr, arrays[0].ptr,
+                                    arrays[1].ptr, 1, C.byref(h)) != 0
+    assert lib.sdl_set_row_predictor(trained.handle) != 0
+    assert lib.sdl_set_row_predictor(None) == 0
