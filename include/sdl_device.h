@@ -46,7 +46,11 @@ typedef struct sdl_btda_plan sdl_btda_plan;
 enum
 {
     SDL_SYM_EXACT = 0,     /* count pass + fill pass */
-    SDL_SYM_PREDICTED = 1  /* one pass sized by the row-nnz predictor, exact fallback */
+    SDL_SYM_PREDICTED = 1, /* one pass sized by the row-nnz predictor, exact fallback */
+    /* or-ed into mode: build the pull schedule (per-entry term lists) whatever
+       the terms / nnz ratio, as the batched, captured and range refills need
+       it (the reference builds it iff terms <= 3 nnz; values are the same) */
+    SDL_PLAN_PULL = 256
 };
 
 /* A: m x nA CSR (dAp[m+1], dAi[nnzA]); B: m x nB CSR.  The plan copies the
@@ -76,6 +80,13 @@ int sdl_btda_refill_batched(const sdl_btda_plan *plan, const double *dAx,
                             const double *dBx, const double *dD, long long ldd, int K,
                             double *dC, long long ldc, void *stream);
 
+/* Refill of the entries [e0, e1) of C only (plans with a pull schedule):
+   dCx is the whole value array, entries outside the range are untouched.
+   One launch: the per-call path for one instance of a block-diagonal plan. */
+int sdl_btda_refill_range(const sdl_btda_plan *plan, const double *dAx, const double *dBx,
+                          const double *dd, double *dCx, long long e0, long long e1,
+                          void *stream);
+
 /* Alloc-phase reservation of the batched-refill workspace for up to Kmax
    refills per call (d transposed to k-major + term operand pairs: 8*m*Kmax +
    16*terms bytes).  Batched refills with K <= Kmax then use the interleaved
@@ -89,7 +100,8 @@ int sdl_btda_bind_values(const sdl_btda_plan *plan, const double *dAx, const dou
 enum
 {
     SDL_BATCH_VALUES_BOUND = 1, /* operand pairs already bound: skip the gather */
-    SDL_BATCH_D_KMAJOR = 2      /* dD is k-major: refill r of row k at dD[k*ldd + r] */
+    SDL_BATCH_D_KMAJOR = 2      /* dD is k-major: refill r of row k at dD[k*ldd + r], with
+                                   ldd >= K rounded up to 16 (rows are read in runs of 16) */
 };
 /* Batched refill with options; ev_start / ev_stop (hipEvent_t, may be NULL)
    are recorded on the stream right before / after the main refill kernel. */
@@ -120,6 +132,17 @@ int sdl_btda_autotune(sdl_btda_plan *plan, const double *dAx, const double *dBx,
                       int reps, sdl_refill_cfg *best, float *best_us);
 /* Pin a configuration (NULL: back to the default heuristic). */
 int sdl_btda_set_refill_cfg(sdl_btda_plan *plan, const sdl_refill_cfg *cfg);
+
+/* ---- HIP graphs of library calls ----
+   Work issued on `stream` (NULL = library stream) between sdl_graph_begin
+   and sdl_graph_end is captured, not run; sdl_graph_launch replays it.  Only
+   calls that allocate nothing and do not synchronize may be captured (the
+   refills with their workspace reserved and operand pairs bound). */
+typedef struct sdl_graph sdl_graph;
+int sdl_graph_begin(void *stream);
+int sdl_graph_end(void *stream, sdl_graph **out);
+int sdl_graph_launch(const sdl_graph *graph, void *stream);
+void sdl_graph_destroy(sdl_graph *graph);
 
 /* ---- learned row binning (no reference counterpart) ----
    A 16 -> 32 -> 6 FP32 MLP predicts, from 16 structural features of an

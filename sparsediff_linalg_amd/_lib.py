@@ -113,6 +113,11 @@ SIGNATURES = {
     "sdl_btda_bind_values": (C.c_int, [vp, vp, vp, vp]),
     "sdl_btda_refill_batched_ex": (C.c_int, [vp, vp, vp, vp, C.c_longlong, C.c_int, vp,
                                              C.c_longlong, C.c_uint, vp, vp, vp]),
+    "sdl_btda_refill_range": (C.c_int, [vp, vp, vp, vp, vp, C.c_longlong, C.c_longlong, vp]),
+    "sdl_graph_begin": (C.c_int, [vp]),
+    "sdl_graph_end": (C.c_int, [vp, C.POINTER(vp)]),
+    "sdl_graph_launch": (C.c_int, [vp, vp]),
+    "sdl_graph_destroy": (None, [vp]),
     # learned row binning
     "sdl_mlp_create": (C.c_int, [C.c_uint64, C.POINTER(vp)]),
     "sdl_mlp_destroy": (None, [vp]),

@@ -28,22 +28,24 @@ extern "C" {
 int sdl_btda_plan_create(int m, int nA, int nB, const int *dAp, const int *dAi,
                          const int *dBp, const int *dBi, int mode, sdl_btda_plan **out)
 {
-    if (mode == SDL_SYM_PREDICTED && !sdl::kPredictedSymbolicEnabled)
+    const int sym = mode & 0xff;
+    if (sym == SDL_SYM_PREDICTED && !sdl::kPredictedSymbolicEnabled)
     {
         sdl::set_error("sdl_btda_plan_create: %s", sdl::kPredictedSymbolicOff);
         return 1;
     }
     return sdl_btda_plan_create_mode(m, nA, nB, dAp, dAi, dBp, dBi,
-                                     mode == SDL_SYM_PREDICTED ? sdl::SymMode::Predicted
-                                                               : sdl::SymMode::Exact,
-                                     sdl::g_row_predictor, out);
+                                     sym == SDL_SYM_PREDICTED ? sdl::SymMode::Predicted
+                                                              : sdl::SymMode::Exact,
+                                     sdl::g_row_predictor, out, (mode & SDL_PLAN_PULL) != 0);
 }
 #pragma GCC visibility pop
 } // extern "C"
 
 int sdl_btda_plan_create_mode(int m, int nA, int nB, const int *dAp, const int *dAi,
                               const int *dBp, const int *dBi, sdl::SymMode mode,
-                              const sdl::RowPredictor *pred, sdl_btda_plan **out)
+                              const sdl::RowPredictor *pred, sdl_btda_plan **out,
+                              bool force_pull)
 {
     if (!out || m < 0 || nA < 0 || nB < 0 || !dAp || !dBp)
     {
@@ -79,7 +81,7 @@ int sdl_btda_plan_create_mode(int m, int nA, int nB, const int *dAp, const int *
     A.An = nA;
     A.Ap = pl->Ap.p;
     A.Ai = pl->Ai.p;
-    sdl::product_plan(pl->P, L, A, mode, pred);
+    sdl::product_plan(pl->P, L, A, mode, pred, true, force_pull);
     *out = pl;
     return 0;
 }
@@ -140,6 +142,18 @@ int sdl_btda_refill(const sdl_btda_plan *plan, const double *dAx, const double *
 {
     if (!plan) return 1;
     sdl::product_numeric(plan->P, dBx, dd, dAx, dCx, as_stream(stream));
+    return 0;
+}
+
+int sdl_btda_refill_range(const sdl_btda_plan *plan, const double *dAx, const double *dBx,
+                          const double *dd, double *dCx, long long e0, long long e1, void *stream)
+{
+    if (!plan || !plan->P.pull || e0 < 0 || e1 < e0 || e1 > plan->P.nnz)
+    {
+        sdl::set_error("sdl_btda_refill_range: invalid range or the plan has no pull schedule");
+        return 1;
+    }
+    sdl::product_numeric_range(plan->P, dBx, dd, dAx, dCx, e0, e1, as_stream(stream));
     return 0;
 }
 
@@ -256,7 +270,8 @@ int sdl_btda_refill_batched_ex(const sdl_btda_plan *plan, const double *dAx, con
         sdl::set_error("sdl_btda_refill_batched_ex: invalid arguments");
         return 1;
     }
-    if ((flags & SDL_BATCH_D_KMAJOR) ? ldd < K : ldd < plan->m)
+    // a k-major D is read in whole runs of 16 refills per row
+    if ((flags & SDL_BATCH_D_KMAJOR) ? ldd < ((long long) K + 15) / 16 * 16 : ldd < plan->m)
     {
         sdl::set_error("sdl_btda_refill_batched_ex: ldd too small for the layout");
         return 1;
@@ -295,6 +310,63 @@ int sdl_btda_set_refill_cfg(sdl_btda_plan *plan, const sdl_refill_cfg *cfg)
     plan->P.tuned.variant = cfg->variant;
     plan->P.tuned.rchunk = cfg->rchunk;
     return 0;
+}
+
+// ---- stream capture into HIP graphs ----
+struct sdl_graph
+{
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+int sdl_graph_begin(void *stream)
+{
+    hipError_t e = hipStreamBeginCapture(as_stream(stream), hipStreamCaptureModeRelaxed);
+    if (e != hipSuccess)
+    {
+        sdl::set_error("sdl_graph_begin: %s", hipGetErrorString(e));
+        return 1;
+    }
+    return 0;
+}
+
+int sdl_graph_end(void *stream, sdl_graph **out)
+{
+    if (!out) return 1;
+    sdl_graph *g = new (std::nothrow) sdl_graph();
+    if (!g) return 1;
+    hipError_t e = hipStreamEndCapture(as_stream(stream), &g->graph);
+    if (e == hipSuccess) e = hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0);
+    if (e != hipSuccess)
+    {
+        sdl::set_error("sdl_graph_end: %s", hipGetErrorString(e));
+        if (g->graph) (void) hipGraphDestroy(g->graph);
+        delete g;
+        return 1;
+    }
+    *out = g;
+    return 0;
+}
+
+int sdl_graph_launch(const sdl_graph *g, void *stream)
+{
+    if (!g) return 1;
+    hipError_t e = hipGraphLaunch(g->exec, as_stream(stream));
+    if (e != hipSuccess)
+    {
+        sdl::set_error("sdl_graph_launch: %s", hipGetErrorString(e));
+        return 1;
+    }
+    return 0;
+}
+
+void sdl_graph_destroy(sdl_graph *g)
+{
+    if (!g) return;
+    sdl::sync();
+    if (g->exec) (void) hipGraphExecDestroy(g->exec);
+    if (g->graph) (void) hipGraphDestroy(g->graph);
+    delete g;
 }
 
 #pragma GCC visibility pop

@@ -14,4 +14,5 @@ struct sdl_btda_plan
 // Plan creation with an explicit symbolic mode and predictor (may be null).
 int sdl_btda_plan_create_mode(int m, int nA, int nB, const int *dAp, const int *dAi,
                               const int *dBp, const int *dBi, sdl::SymMode mode,
-                              const sdl::RowPredictor *pred, sdl_btda_plan **out);
+                              const sdl::RowPredictor *pred, sdl_btda_plan **out,
+                              bool force_pull = false);

@@ -994,6 +994,27 @@ __global__ void __launch_bounds__(256)
     }
 }
 
+// One refill over the pre-gathered operand pairs: C[e] = sum over the entry's
+// terms (ascending k) of (L value * d[k]) * A value -- the fold of every
+// other refill kernel.  One 256-entry tile per workgroup (with a single
+// refill no tile's data is reused by another workgroup).
+__global__ void __launch_bounds__(256)
+    k_num_pull_tv1(int64_t nnz, const int64_t *__restrict__ ep, const int *__restrict__ tK,
+                   const double *__restrict__ tv, const double *__restrict__ d,
+                   double *__restrict__ C)
+{
+    const int64_t e = (int64_t) blockIdx.x * 256 + threadIdx.x;
+    if (e >= nnz) return;
+    const int64_t s0 = ep[e], s1 = ep[e + 1];
+    double out = 0.0;
+    for (int64_t s = s0; s < s1; s++)
+    {
+        const dvec2 ba = *(const dvec2 *) (tv + 2 * s);
+        out = out + (ba.x * d[tK[s]]) * ba.y;
+    }
+    __builtin_nontemporal_store(out, C + e);
+}
+
 typedef double dvec4 __attribute__((ext_vector_type(4)));
 
 // Interleaved batched refill with EPT adjacent entries per thread: one
@@ -1368,7 +1389,7 @@ void product_row_features(const LView &L, const AView &A, float *F, hipStream_t 
 }
 
 void product_plan(ProductPlan &P, const LView &L, const AView &A, SymMode mode,
-                  const RowPredictor *pred, bool allow_pull)
+                  const RowPredictor *pred, bool allow_pull, bool force_pull)
 {
     SDL_REQUIRE(L.blocks >= 1 && A.Am % L.blocks == 0, "product: A rows not a multiple of blocks");
     P.L = L;
@@ -1407,7 +1428,7 @@ void product_plan(ProductPlan &P, const LView &L, const AView &A, SymMode mode,
     else symbolic_exact(P, rterms.p, rowcnt.p, nwords);
 
     // refill schedule (reference policy: a plan iff terms <= 3 nnz, gustavson.h:45-52)
-    P.pull = allow_pull && P.terms > 0 && P.terms <= 3 * P.nnz &&
+    P.pull = allow_pull && P.terms > 0 && (force_pull || P.terms <= 3 * P.nnz) &&
              P.terms < (int64_t) 0x7fffffff && P.nnz < (int64_t) 0x7fffffff;
     if (P.pull)
     {
@@ -1515,7 +1536,22 @@ void product_numeric_batched(const ProductPlan &P, const double *Lval, const dou
         return;
     }
     RefillCfg c = cfg ? *cfg : (P.tuned.variant >= 0 ? P.tuned : default_refill_cfg(P, K));
+    const bool kmajor = (opts & BATCH_D_KMAJOR) != 0;
+    // only the interleaved kernels read a k-major D
+    if (kmajor && c.variant < 3) c = RefillCfg{3, c.rchunk == 16 ? 16 : 8};
+    SDL_REQUIRE(!kmajor || K <= P.batch_cap, "k-major refill beyond the reservation");
     if (c.variant >= 3 && K > P.batch_cap) c = RefillCfg{2, 8};
+    if (c.variant >= 3 && K == 1 && !kmajor)
+    {
+        // a single refill: d read in place, no transposition
+        if (!(opts & BATCH_VALUES_BOUND)) product_bind_values(P, Lval, Ax, s);
+        if (ev_start) SDL_HIP(hipEventRecord((hipEvent_t) ev_start, s));
+        hipLaunchKernelGGL(k_num_pull_tv1, dim3((unsigned) ((P.nnz + 255) / 256)), dim3(256), 0,
+                           s, P.nnz, P.ep.p, P.tK.p, P.tv.p, D, Cx);
+        SDL_HIP(hipGetLastError());
+        if (ev_stop) SDL_HIP(hipEventRecord((hipEvent_t) ev_stop, s));
+        return;
+    }
     if (c.variant >= 3)
     {
         const int rc = c.rchunk == 8 ? 8 : 16;
@@ -1626,10 +1662,26 @@ void product_reserve_batch(ProductPlan &P, int Kmax)
 {
     if (!P.pull || Kmax <= P.batch_cap) return;
     if (!P.tv.p) P.tv.alloc((size_t) 2 * std::max<int64_t>(P.terms, 1));
+    if (Kmax == 1)  // single refills read d in place: no transposed copy
+    {
+        P.batch_cap = 1;
+        return;
+    }
     int64_t ld = ((int64_t) Kmax + 15) / 16 * 16;
     P.dT.alloc((size_t) std::max<int64_t>(P.A.Am, 1) * ld);
     P.batch_ld = ld;
     P.batch_cap = (int) ld;
+}
+
+void product_numeric_range(const ProductPlan &P, const double *Lval, const double *d,
+                           const double *Ax, double *Cx, int64_t e0, int64_t e1, hipStream_t s)
+{
+    SDL_REQUIRE(P.pull, "range refill: the plan has no pull schedule");
+    SDL_REQUIRE(0 <= e0 && e0 <= e1 && e1 <= P.nnz, "range refill: entries out of range");
+    if (e1 == e0) return;
+    hipLaunchKernelGGL(k_num_pull, dim3((unsigned) ((e1 - e0 + 255) / 256)), dim3(256), 0, s,
+                       e1 - e0, P.ep.p + e0, P.tA.p, P.tB.p, P.tK.p, Lval, d, Ax, Cx + e0);
+    SDL_HIP(hipGetLastError());
 }
 
 std::vector<RefillCfg> refill_candidates()

@@ -86,14 +86,21 @@ struct ProductPlan
 // Reserve the interleaved batched-refill workspace for up to Kmax refills.
 void product_reserve_batch(ProductPlan &P, int Kmax);
 
+// One refill of the entries [e0, e1) only (pull plans): Cx[e0..e1) written,
+// the rest of Cx untouched.  One launch.
+void product_numeric_range(const ProductPlan &P, const double *Lval, const double *d,
+                           const double *Ax, double *Cx, int64_t e0, int64_t e1, hipStream_t s);
+
 struct RowPredictor;
 
-// Build the exact structure and the refill schedule.  SymMode::Predicted: one
+// Build the exact structure and the refill schedule (pull iff terms <= 3 nnz,
+// the reference's plan policy; force_pull: whenever the counts fit int32 --
+// both schedules give the same values bit for bit).  SymMode::Predicted: one
 // pass whose per-row hash tables are sized from the predictor's bins (pred
 // may be null: tables sized from the term bound only); rows whose table
 // fills up, and rows no table suits, take the exact count + fill passes.
 void product_plan(ProductPlan &P, const LView &L, const AView &A, SymMode mode,
-                  const RowPredictor *pred, bool allow_pull = true);
+                  const RowPredictor *pred, bool allow_pull = true, bool force_pull = false);
 
 // The 16 structural features of every output row (F[row * 16 + f], FP32):
 // log2(1 + x) of the term bound, the degree, the longest / shortest A row,

@@ -173,3 +173,45 @@ def blocks_as_csr(w):
             i[p[r]:p[r + 1]] = rows[r][0]
             x[p[r]:p[r + 1]] = rows[r][1]
     return p.astype(np.int32), i, x
+
+
+def multi_instances(seed: int = 4, ninst: int = 4096, lo: int = 100, hi: int = 50000,
+                    nnz_lo: int = 4, nnz_hi: int = 12):
+    """Independent B^T D A instances as block-diagonal operands: instance i has
+    m_i log-uniform on [lo, hi] rows and n_i = m_i // 4 columns; A_i and B_i
+    have row lengths uniform in [nnz_lo, min(nnz_hi, n_i)] with distinct
+    uniform columns, FP64 N(0,1).  Returns the block-diagonal CSR of A and B
+    (global column indices) with the instances' row and column offsets."""
+    g = np.random.Generator(np.random.PCG64(seed))
+    m = np.rint(np.exp(g.uniform(np.log(lo), np.log(hi), ninst))).astype(np.int64)
+    n = m // 4
+    roff = np.zeros(ninst + 1, np.int64)
+    np.cumsum(m, out=roff[1:])
+    coff = np.zeros(ninst + 1, np.int64)
+    np.cumsum(n, out=coff[1:])
+    M = int(roff[-1])
+    assert M < 2 ** 31 and coff[-1] < 2 ** 31
+    lib = load()
+    out = {}
+    for tag, salt in (("A", 1), ("B", 2)):
+        lengths = np.empty(M, np.int64)
+        tmp = np.empty(int(m.max()) + 1, np.int32)
+        for i in range(ninst):
+            s = seed * 10_000_000 + i * 10 + salt
+            hi_i = int(min(nnz_hi, n[i]))
+            lib.sdl_gen_row_lengths_uniform(s, int(m[i]), nnz_lo, hi_i, ptr(tmp[:m[i] + 1]))
+            lengths[roff[i]:roff[i + 1]] = np.diff(tmp[:m[i] + 1])
+        p = np.zeros(M + 1, np.int64)
+        np.cumsum(lengths, out=p[1:])
+        assert p[-1] < 2 ** 31
+        i_all = np.empty(int(p[-1]), np.int32)
+        x_all = np.empty(int(p[-1]), np.float64)
+        for i in range(ninst):
+            s = seed * 10_000_000 + i * 10 + salt
+            a, b = int(p[roff[i]]), int(p[roff[i + 1]])
+            lp = (p[roff[i]:roff[i + 1] + 1] - a).astype(np.int32)
+            lib.sdl_gen_fill_rows(s, int(m[i]), int(n[i]), ptr(lp), ptr(i_all[a:b]),
+                                  ptr(x_all[a:b]))
+            i_all[a:b] += np.int32(coff[i])
+        out[tag] = (p.astype(np.int32), i_all, x_all)
+    return dict(m=m, n=n, roff=roff, coff=coff, A=out["A"], B=out["B"])
