@@ -55,7 +55,7 @@ def parse():
     ap.add_argument("--multi-k", type=int, default=8,
                     help="multi: refill rounds per launch (even; output ring of this many slots)")
     ap.add_argument("--workload", default="btda",
-                    choices=["btda", "blocks-atda", "blocks-btda", "multi"],
+                    choices=["btda", "blocks-atda", "blocks-btda", "multi", "ata-large", "coo"],
                     help="btda: the sparse refill (default line); blocks-*: the stacked "
                          "permuted-dense products (A^T D A, or B^T D A of two stacks)")
     ap.add_argument("--gemm-slice", type=int, default=0,
@@ -76,6 +76,10 @@ def main():
         sys.exit(2)
     if args.workload == "multi":
         return main_multi(args)
+    if args.workload == "ata-large":
+        return main_ata_large(args)
+    if args.workload == "coo":
+        return main_coo(args)
     if args.workload != "btda":
         return main_blocks(args)
     import sparsediff_linalg_amd as sdl
@@ -239,6 +243,156 @@ def main():
     }
     lib.sdl_btda_plan_destroy(h)
     print(json.dumps(out), flush=True)
+
+
+def main_ata_large(args):
+    """C = A^T diag(d) A for A 2,000,000 x 500,000 with rank-Zipf(1.2) row
+    lengths in [1, 512] (mean 16), uniform columns, FP64 N(0,1): one symbolic
+    pass (int64 plan; nnz(C) beyond INT_MAX) and refills for fresh d."""
+    import scipy.sparse as sp
+
+    import sparsediff_linalg_amd as sdl
+    from sparsediff_linalg_amd import DeviceArray, check, workloads
+    from sparsediff_linalg_amd.predictor import Plan
+
+    lib = sdl.load()
+    m, n = 2_000_000, 500_000
+    Ap, Ai, Ax = workloads.zipf_csr(2, m, n)
+    nnzA = int(Ap[-1])
+    t0 = time.perf_counter()
+    pl = Plan((Ap, Ai), None, n, n)
+    lib.sdl_synchronize()
+    t_sym = time.perf_counter() - t0
+    nnz, terms = pl.nnz(), lib.sdl_btda_plan_terms(pl.h)
+    dAx = DeviceArray.from_numpy(Ax)
+    ds = [DeviceArray.from_numpy(workloads.uniform(90 + i, m, 0.1, 10.0)) for i in range(2)]
+    Cx = DeviceArray(nnz, np.float64)
+    stream = lib.sdl_library_stream()
+    check(lib.sdl_btda_refill(pl.h, dAx.ptr, dAx.ptr, ds[0].ptr, Cx.ptr, stream), "refill")
+    lib.sdl_synchronize()
+    e0, e1 = lib.sdl_event_create(), lib.sdl_event_create()
+    steps = max(1, min(args.steps, 5))
+    lib.sdl_event_record(e0, stream)
+    for i in range(steps):
+        check(lib.sdl_btda_refill(pl.h, dAx.ptr, dAx.ptr, ds[i % 2].ptr, Cx.ptr, stream), "refill")
+    lib.sdl_event_record(e1, stream)
+    lib.sdl_synchronize()
+    ms = lib.sdl_event_elapsed_ms(e0, e1) / steps
+    # sanity: 32 rows against scipy (different summation order: 1e-13 relative)
+    d_last = ds[(steps - 1) % 2].numpy()
+    A = sp.csr_matrix((Ax, Ai, Ap), shape=(m, n))
+    cp = np.empty(n + 1, np.int64)
+    lib.sdl_memcpy_dtoh(cp.ctypes.data, lib.sdl_btda_plan_rowptr(pl.h), cp.nbytes)
+    AT = A.T.tocsr()
+    worst = 0.0
+    for r in np.random.default_rng(0).choice(n, 32, replace=False):
+        row = (AT[r].multiply(d_last[None, :])) @ A
+        row = row.tocsr()
+        a, b = int(cp[r]), int(cp[r + 1])
+        gx = np.empty(b - a)
+        lib.sdl_memcpy_dtoh(gx.ctypes.data, Cx.offset(a), gx.nbytes)
+        dense = np.zeros(n)
+        dense[row.indices] = row.data
+        gi = np.empty(b - a, np.int32)
+        lib.sdl_memcpy_dtoh(gi.ctypes.data, lib.sdl_btda_plan_colind(pl.h) + 4 * a, gi.nbytes)
+        mag = np.abs(AT[r].multiply(d_last[None, :])) @ np.abs(A)
+        magd = np.zeros(n)
+        magd[mag.indices] = mag.data
+        err = np.abs(gx - dense[gi]) / (magd[gi] + 1e-300)
+        worst = max(worst, float(err.max()) if len(err) else 0.0)
+    assert worst <= 1e-13, worst
+    bytes_ = (4 * (m + 1) + 12 * nnzA) + 8 * m + (4 * (n + 1) + 12 * nnz)
+    t = ms * 1e-3
+    out = {
+        "metric": METRIC + " (large A^T A)",
+        "value": nnz / t, "unit": "nnz-out/s", "n_gpus": 1, "steps": steps, "warmup": 1,
+        "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {
+            "workload": ("C = A^T diag(d) A, A 2,000,000 x 500,000 CSR, rank-Zipf(1.2) row "
+                         "lengths clipped [1,512] mean 16, uniform columns, FP64 N(0,1), seed 2; "
+                         "one step = one refill for a fresh d"),
+            "nnz_A": nnzA, "terms": terms, "nnz_out": nnz, "symbolic_s": t_sym,
+            "schedule": "pull" if lib.sdl_btda_plan_is_pull(pl.h) else "row accumulators",
+            "sampled_max_rel_err_vs_scipy": worst,
+        },
+        "roofline": {"bound": "hbm", "kernel": "row-accumulator refill (dense accumulators)",
+                     "achieved": bytes_ / t / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": bytes_ / t / 1e9 / HBM_PEAK_GBS, "traffic": None,
+                     "alg_bytes_per_launch": bytes_},
+        "gflops": (nnzA + 2 * terms) / t / 1e9,
+        "symbolic_terms_per_s": terms / t_sym,
+        "cpu_baseline": None,
+    }
+    pl.close()
+    print(json.dumps(out), flush=True)
+
+
+def main_coo(args):
+    """Coalesce 50M COO triplets (35M distinct keys uniform in 1M x 1M, 15M
+    repeats of earlier keys, shuffled; FP64 N(0,1)) into CSR with duplicates
+    summed in input order."""
+    import sparsediff_linalg_amd as sdl
+    from sparsediff_linalg_amd import DeviceArray, check, workloads
+
+    lib = sdl.load()
+    m = n = 1_000_000
+    rows, cols, x = workloads.coo_triplets(2)
+    nin = len(rows)
+    dr, dc, dx = (DeviceArray.from_numpy(a) for a in (rows, cols, x))
+    lib.sdl_synchronize()
+    h = C.c_void_p()
+    t0 = time.perf_counter()
+    check(lib.sdl_coo_plan_create(nin, dr.ptr, dc.ptr, m, n, C.byref(h)), "coo plan")
+    lib.sdl_synchronize()
+    t_plan = time.perf_counter() - t0
+    nnz = lib.sdl_coo_plan_nnz(h)
+    out = DeviceArray(nnz, np.float64)
+    stream = lib.sdl_library_stream()
+    check(lib.sdl_coo_fill(h, dx.ptr, out.ptr, stream), "fill")
+    lib.sdl_synchronize()
+    e0, e1 = lib.sdl_event_create(), lib.sdl_event_create()
+    steps = max(1, min(args.steps, 50))
+    lib.sdl_event_record(e0, stream)
+    for _ in range(steps):
+        check(lib.sdl_coo_fill(h, dx.ptr, out.ptr, stream), "fill")
+    lib.sdl_event_record(e1, stream)
+    lib.sdl_synchronize()
+    ms = lib.sdl_event_elapsed_ms(e0, e1) / steps
+    # check: every output value is the sum of its key's duplicates (numpy,
+    # any order: 1e-13 of the summed magnitudes)
+    cp = np.empty(m + 1, np.int64)
+    lib.sdl_memcpy_dtoh(cp.ctypes.data, lib.sdl_coo_plan_rowptr(h), cp.nbytes)
+    got = out.numpy()
+    key = rows.astype(np.int64) * n + cols
+    uk, inv = np.unique(key, return_inverse=True)
+    ref = np.bincount(inv, weights=x)
+    mag = np.bincount(inv, weights=np.abs(x))
+    assert nnz == len(uk) and cp[-1] == nnz
+    worst = float(np.max(np.abs(got - ref) / (mag + 1e-300)))
+    assert worst <= 1e-13, worst
+    bytes_ = 8 * nin + 8 * nnz                       # the value fill: values in, values out
+    bytes_all = 16 * nin + 4 * (m + 1) + 12 * nnz    # the whole coalesce (structure + values)
+    out_json = {
+        "metric": "COO coalesce triplets/s (duplicates summed in input order)",
+        "value": nin / (ms * 1e-3), "unit": "triplets/s", "n_gpus": 1, "steps": steps,
+        "warmup": 1, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": ("coalesce 50M COO triplets in 1M x 1M (35M distinct keys, "
+                                "30% duplicates, shuffled), FP64 N(0,1), seed 2; one step = "
+                                "one value fill of the coalesced CSR"),
+                   "n_in": nin, "nnz_out": nnz, "plan_s": t_plan,
+                   "max_rel_err_vs_numpy": worst},
+        "roofline": {"bound": "hbm", "kernel": "k_coalesce_fill",
+                     "achieved": bytes_ / (ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": bytes_ / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                     "traffic": None, "alg_bytes_per_launch": bytes_},
+        "plan_triplets_per_s": nin / t_plan,
+        "alg_bytes_whole_coalesce": bytes_all,
+        "cpu_baseline": None,
+    }
+    lib.sdl_coo_plan_destroy(h)
+    print(json.dumps(out_json), flush=True)
 
 
 def main_multi(args):

@@ -133,6 +133,23 @@ int sdl_btda_autotune(sdl_btda_plan *plan, const double *dAx, const double *dBx,
 /* Pin a configuration (NULL: back to the default heuristic). */
 int sdl_btda_set_refill_cfg(sdl_btda_plan *plan, const sdl_refill_cfg *cfg);
 
+/* ---- COO coalescing on device-resident triplets ----
+   The reference has no direct COO -> CSR entry; its coalescing semantics are
+   row_reduce's (src/sparse/sparse_matrix.c:136-240): output rows hold sorted,
+   unique columns and every entry is ((+0.0 + x_t1) + x_t2) + ... over its
+   duplicates in input order.  drows / dcols: n_items int32 coordinates in an
+   m x n space (n_items < 2^32). */
+typedef struct sdl_coo_plan sdl_coo_plan;
+int sdl_coo_plan_create(long long n_items, const int *drows, const int *dcols, int m, int n,
+                        sdl_coo_plan **out);
+long long sdl_coo_plan_nnz(const sdl_coo_plan *plan);
+/* Device row pointer (int64, m+1) and column indices (nnz) of the output. */
+const long long *sdl_coo_plan_rowptr(const sdl_coo_plan *plan);
+const int *sdl_coo_plan_colind(const sdl_coo_plan *plan);
+/* dCx[nnz] from the n_items values dx; one pass, allocates nothing. */
+int sdl_coo_fill(const sdl_coo_plan *plan, const double *dx, double *dCx, void *stream);
+void sdl_coo_plan_destroy(sdl_coo_plan *plan);
+
 /* ---- HIP graphs of library calls ----
    Work issued on `stream` (NULL = library stream) between sdl_graph_begin
    and sdl_graph_end is captured, not run; sdl_graph_launch replays it.  Only

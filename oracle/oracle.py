@@ -123,6 +123,29 @@ def btda(m, nA, nB, Ap, Ai, Ax, Bp, Bi, Bx, d):
     return spgemm(nB, Lp, Li, Lx, 1, d, m, nA, Ap, Ai, Ax)
 
 
+def spgemm_rows(rows, Lm, Lp, Li, Lx, blocks, d, Am, An, Ap, Ai, Ax):
+    """Selected output rows of C = kron(I_blocks, L) diag(d) A, one at a time
+    (gustavson.c:43-68, :78-101, :117-141 for one row): a list of
+    (cols int32, vals float64).  For products too large to form whole."""
+    o = lib()
+    Lp, Li, Ap, Ai = (np.ascontiguousarray(v, np.int32) for v in (Lp, Li, Ap, Ai))
+    Lx, Ax = np.ascontiguousarray(Lx, np.float64), np.ascontiguousarray(Ax, np.float64)
+    mark = np.full(An, -1, np.int32)
+    acc = np.zeros(An, np.float64)
+    out = []
+    for r in rows:
+        b, rr = divmod(int(r), Lm)
+        bound = int(sum(Ap[k + b * (Am // blocks) + 1] - Ap[k + b * (Am // blocks)]
+                        for k in Li[Lp[rr]:Lp[rr + 1]]))
+        cols = np.zeros(max(min(bound, An), 1), np.int32)
+        vals = np.zeros(max(min(bound, An), 1), np.float64)
+        cnt = o.orc_spgemm_row(int(r), Lm, I(Lp), I(Li), D(Lx), blocks,
+                               D(np.ascontiguousarray(d, np.float64)) if d is not None else None,
+                               Am, An, I(Ap), I(Ai), D(Ax), I(mark), D(acc), I(cols), D(vals))
+        out.append((cols[:cnt].copy(), vals[:cnt].copy()))
+    return out
+
+
 def coo_coalesce(rows, cols, x, m, n):
     o = lib()
     rows, cols = np.ascontiguousarray(rows, np.int32), np.ascontiguousarray(cols, np.int32)

@@ -114,6 +114,12 @@ SIGNATURES = {
     "sdl_btda_refill_batched_ex": (C.c_int, [vp, vp, vp, vp, C.c_longlong, C.c_int, vp,
                                              C.c_longlong, C.c_uint, vp, vp, vp]),
     "sdl_btda_refill_range": (C.c_int, [vp, vp, vp, vp, vp, C.c_longlong, C.c_longlong, vp]),
+    "sdl_coo_plan_create": (C.c_int, [C.c_longlong, vp, vp, C.c_int, C.c_int, C.POINTER(vp)]),
+    "sdl_coo_plan_nnz": (C.c_longlong, [vp]),
+    "sdl_coo_plan_rowptr": (vp, [vp]),
+    "sdl_coo_plan_colind": (vp, [vp]),
+    "sdl_coo_fill": (C.c_int, [vp, vp, vp, vp]),
+    "sdl_coo_plan_destroy": (None, [vp]),
     "sdl_graph_begin": (C.c_int, [vp]),
     "sdl_graph_end": (C.c_int, [vp, C.POINTER(vp)]),
     "sdl_graph_launch": (C.c_int, [vp, vp]),

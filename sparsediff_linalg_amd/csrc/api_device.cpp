@@ -312,6 +312,49 @@ int sdl_btda_set_refill_cfg(sdl_btda_plan *plan, const sdl_refill_cfg *cfg)
     return 0;
 }
 
+// ---- COO coalescing on device-resident triplets ----
+struct sdl_coo_plan
+{
+    sdl::CoalescePlan P;
+};
+
+int sdl_coo_plan_create(long long n_items, const int *drows, const int *dcols, int m, int n,
+                        sdl_coo_plan **out)
+{
+    if (!out || n_items < 0 || m < 0 || n < 0 || (n_items > 0 && (!drows || !dcols)) ||
+        n_items >= (1ll << 32))
+    {
+        sdl::set_error("sdl_coo_plan_create: invalid arguments");
+        return 1;
+    }
+    sdl_coo_plan *p = new (std::nothrow) sdl_coo_plan();
+    if (!p) return 1;
+    sdl::coalesce_plan(p->P, n_items, drows, dcols, m, n);
+    *out = p;
+    return 0;
+}
+
+long long sdl_coo_plan_nnz(const sdl_coo_plan *p) { return p ? p->P.nnz : -1; }
+const long long *sdl_coo_plan_rowptr(const sdl_coo_plan *p)
+{
+    return p ? (const long long *) p->P.Cp.p : nullptr;
+}
+const int *sdl_coo_plan_colind(const sdl_coo_plan *p) { return p ? p->P.Ci.p : nullptr; }
+
+int sdl_coo_fill(const sdl_coo_plan *p, const double *dx, double *dCx, void *stream)
+{
+    if (!p) return 1;
+    sdl::coalesce_fill(p->P, dx, dCx, as_stream(stream));
+    return 0;
+}
+
+void sdl_coo_plan_destroy(sdl_coo_plan *p)
+{
+    if (!p) return;
+    sdl::sync();
+    delete p;
+}
+
 // ---- stream capture into HIP graphs ----
 struct sdl_graph
 {

@@ -1142,6 +1142,46 @@ __global__ void __launch_bounds__(G)
         for (int i = threadIdx.x; i < cnt; i += G) Cx[cbase + i] = s_val[i];
 }
 
+// Long rows with a dense accumulator of n doubles per workgroup (the
+// reference's acc[n], gustavson.c:107-143): a persistent grid strides over the
+// rows; per row the entries' slots are zeroed, L's row is walked in ascending
+// k with a barrier per k (the columns of one A row are distinct, so its terms
+// update distinct slots in parallel), then the row is gathered into C.
+template <int G>
+__global__ void __launch_bounds__(G)
+    k_num_rowacc_dense(const int *__restrict__ rows, int nrows, int Lm, const int *__restrict__ Lp,
+                       const int *__restrict__ Li, const int *__restrict__ Lperm, int nL,
+                       const int *__restrict__ Ap, const int *__restrict__ Ai,
+                       const int64_t *__restrict__ Cp, const int *__restrict__ Ci,
+                       const double *__restrict__ Lval, const double *__restrict__ d,
+                       const double *__restrict__ Ax, double *__restrict__ Cx,
+                       double *__restrict__ scratch, int64_t n)
+{
+    double *acc = scratch + (int64_t) blockIdx.x * n;
+    for (int ri = blockIdx.x; ri < nrows; ri += gridDim.x)
+    {
+        const int row = rows[ri];
+        const int64_t cbase = Cp[row], cnt = Cp[row + 1] - cbase;
+        for (int64_t i = threadIdx.x; i < cnt; i += G) acc[Ci[cbase + i]] = 0.0;
+        __syncthreads();
+        RowCtx c = row_ctx(row, Lm, Lp, nL);
+        for (int q = c.q0; q < c.q1; q++)
+        {
+            const int k = Li[q] + c.koff;
+            double v = Lval[Lperm ? Lperm[q] : q];
+            if (d) v = v * d[k];
+            for (int t = Ap[k] + (int) threadIdx.x; t < Ap[k + 1]; t += G)
+            {
+                const int j = Ai[t];
+                acc[j] = acc[j] + v * Ax[t];
+            }
+            __syncthreads();
+        }
+        for (int64_t i = threadIdx.x; i < cnt; i += G) Cx[cbase + i] = acc[Ci[cbase + i]];
+        __syncthreads();
+    }
+}
+
 // ------------------------------------------------------------- host ----
 struct Bins
 {
@@ -1484,6 +1524,15 @@ void product_plan(ProductPlan &P, const LView &L, const AView &A, SymMode mode,
         P.rbin_off[2] = small + large;
         d2d(P.rbins.p, BR.rows.p + BR.off[1], (size_t) small * sizeof(int));
         d2d(P.rbins.p + small, BR.rows.p + BR.off[3], (size_t) large * sizeof(int));
+        // dense accumulators for the long rows: at most 2 GB, at least 64 rows
+        // in flight, else C itself is the accumulator
+        const int64_t budget = (int64_t) 2 << 30;
+        const int64_t fit = budget / (8 * std::max<int64_t>(P.n, 1));
+        if (large > 0 && fit >= 64)
+        {
+            P.dense_grid = (int) std::min<int64_t>({(int64_t) large, fit, 1024});
+            P.acc.alloc((size_t) P.dense_grid * std::max<int64_t>(P.n, 1));
+        }
         sync();
     }
 }
@@ -1506,7 +1555,12 @@ void product_numeric(const ProductPlan &P, const double *Lval, const double *d,
             hipLaunchKernelGGL((k_num_rowacc<256, 1024>), dim3(n0), dim3(256), 0, s, P.rbins.p,
                                P.L.Lm, P.L.Lp, P.L.Li, P.L.Lperm, nL, P.A.Ap, P.A.Ai, P.Cp.p,
                                P.Ci.p, Lval, d, Ax, Cx);
-        if (n1)
+        if (n1 && P.dense_grid > 0)
+            hipLaunchKernelGGL((k_num_rowacc_dense<256>), dim3(P.dense_grid), dim3(256), 0, s,
+                               P.rbins.p + P.rbin_off[1], n1, P.L.Lm, P.L.Lp, P.L.Li, P.L.Lperm,
+                               nL, P.A.Ap, P.A.Ai, P.Cp.p, P.Ci.p, Lval, d, Ax, Cx, P.acc.p,
+                               (int64_t) P.n);
+        else if (n1)
             hipLaunchKernelGGL((k_num_rowacc<256, 0>), dim3(n1), dim3(256), 0, s,
                                P.rbins.p + P.rbin_off[1], P.L.Lm, P.L.Lp, P.L.Li, P.L.Lperm, nL,
                                P.A.Ap, P.A.Ai, P.Cp.p, P.Ci.p, Lval, d, Ax, Cx);

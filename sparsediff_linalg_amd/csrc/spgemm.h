@@ -74,6 +74,11 @@ struct ProductPlan
     // Row-accumulator schedule (otherwise): rows binned by nnz.
     DBuf<int> rbins;                 // row ids grouped by class
     int rbin_off[8] = {0};           // class c is rbins[rbin_off[c] .. rbin_off[c+1])
+    // Rows too long for LDS accumulate in a dense per-workgroup accumulator of
+    // n doubles (reserved here: dense_grid * n), or in C itself when n is too
+    // large for that (dense_grid == 0).
+    int dense_grid = 0;
+    DBuf<double> acc;
     SymStats stats;
     RefillCfg tuned;   // variant < 0: not tuned, use the default heuristic
     // Batched-refill workspace (reserved in the alloc phase): term operand pairs

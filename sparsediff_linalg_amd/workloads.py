@@ -215,3 +215,23 @@ def multi_instances(seed: int = 4, ninst: int = 4096, lo: int = 100, hi: int = 5
             i_all[a:b] += np.int32(coff[i])
         out[tag] = (p.astype(np.int32), i_all, x_all)
     return dict(m=m, n=n, roff=roff, coff=coff, A=out["A"], B=out["B"])
+
+
+def coo_triplets(seed: int = 2, n_items: int = 50_000_000, unique: int = 35_000_000,
+                 m: int = 1_000_000, n: int = 1_000_000):
+    """COO triplets for coalescing: `unique` distinct (row, col) keys drawn
+    uniformly from the m x n space, then n_items - unique triplets re-using
+    uniformly chosen earlier keys, the whole list shuffled; values N(0,1).
+    Returns (rows int32, cols int32, x float64)."""
+    g = np.random.Generator(np.random.PCG64(seed))
+    space = np.int64(m) * np.int64(n)
+    keys = np.unique(g.integers(0, space, int(unique * 1.001) + 64, dtype=np.int64))
+    while len(keys) < unique:
+        keys = np.unique(np.concatenate([keys, g.integers(0, space, unique, dtype=np.int64)]))
+    keys = g.permutation(keys)[:unique]
+    rep = keys[g.integers(0, unique, n_items - unique)]
+    allk = g.permutation(np.concatenate([keys, rep]))
+    rows = (allk // n).astype(np.int32)
+    cols = (allk % n).astype(np.int32)
+    x = normal(seed * 1000 + 11, n_items)
+    return rows, cols, x

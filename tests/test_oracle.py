@@ -268,3 +268,21 @@ def test_library_exports_every_declared_symbol():
     lib = C.CDLL(LIB_PATH, mode=os.RTLD_LOCAL)
     missing = [n for n in sorted(_declared_functions()) if not hasattr(lib, n)]
     assert not missing, missing
+
+
+def test_spgemm_rows_equal_full_product():
+    """The sampled-row oracle gives exactly the rows of the full product."""
+    import numpy as _np
+    from oracle import oracle as _O
+    from tests.helpers import random_csr as _rc
+    rng = _np.random.default_rng(21)
+    m, nA, nB = 300, 70, 50
+    Ap, Ai, Ax = _rc(rng, m, nA, 0.08)
+    Bp, Bi, Bx = _rc(rng, m, nB, 0.1)
+    d = rng.uniform(0.1, 10, m)
+    Cp, Ci, Cx, _ = _O.btda(m, nA, nB, Ap, Ai, Ax, Bp, Bi, Bx, d)
+    Lp, Li, Lx = _O.transpose(m, nB, Bp, Bi, Bx)
+    rows = [0, 7, 13, nB - 1, 25]
+    for r, (cols, vals) in zip(rows, _O.spgemm_rows(rows, nB, Lp, Li, Lx, 1, d, m, nA, Ap, Ai, Ax)):
+        assert _np.array_equal(cols, Ci[Cp[r]:Cp[r + 1]])
+        assert _np.array_equal(vals.view(_np.int64), Cx[Cp[r]:Cp[r + 1]].view(_np.int64))
