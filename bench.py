@@ -55,9 +55,13 @@ def parse():
     ap.add_argument("--multi-k", type=int, default=8,
                     help="multi: refill rounds per launch (even; output ring of this many slots)")
     ap.add_argument("--workload", default="btda",
-                    choices=["btda", "blocks-atda", "blocks-btda", "multi", "ata-large", "coo"],
+                    choices=["btda", "blocks-atda", "blocks-btda", "multi", "ata-large", "coo",
+                             "learned"],
                     help="btda: the sparse refill (default line); blocks-*: the stacked "
                          "permuted-dense products (A^T D A, or B^T D A of two stacks)")
+    ap.add_argument("--rmat-scale", type=int, default=18,
+                    help="learned: R-MAT scale of the A^T A evaluated beside the banded one")
+    ap.add_argument("--epochs", type=int, default=100, help="learned: training epochs")
     ap.add_argument("--gemm-slice", type=int, default=0,
                     help="blocks-*: reduction slice depth of the block GEMM (16 or 32; 0 = "
                          "library default)")
@@ -80,6 +84,8 @@ def main():
         return main_ata_large(args)
     if args.workload == "coo":
         return main_coo(args)
+    if args.workload == "learned":
+        return main_learned(args)
     if args.workload != "btda":
         return main_blocks(args)
     import sparsediff_linalg_amd as sdl
@@ -393,6 +399,108 @@ def main_coo(args):
     }
     lib.sdl_coo_plan_destroy(h)
     print(json.dumps(out_json), flush=True)
+
+
+def main_learned(args):
+    """Learned row binning: the symbolic phase of A^T A for a banded matrix
+    (2^20 rows, bandwidth 64), a strided band (2^19 rows of 129 consecutive
+    columns starting 4 apart, 2^21 columns) and an R-MAT graph (a, b, c =
+    0.57, 0.19, 0.19, 16 edges per node, duplicates coalesced), with the count
+    pass sized from the term bound (exact path) and from a trained network's
+    predicted bins; structures compared bit for bit.  Two networks, each
+    trained on the GPU from exact counts: on the sparse B^T D A, rank-Zipf
+    A^T A and stacked block products only, and with banded / strided-band /
+    R-MAT products of other sizes added."""
+    import sparsediff_linalg_amd as sdl
+    from sparsediff_linalg_amd import workloads
+    from sparsediff_linalg_amd.predictor import Plan, Predictor, training_products, training_set
+
+    lib = sdl.load()
+    nets = {}
+    for name, structured in (("sparse_zipf_blocks", False), ("with_structured", True)):
+        t0 = time.perf_counter()
+        X, Y, n = training_set(training_products(small=False, structured=structured))
+        t_label = time.perf_counter() - t0
+        mlp = Predictor(seed=1)
+        t0 = time.perf_counter()
+        loss = mlp.train(X, Y, n, epochs=args.epochs)
+        lib.sdl_synchronize()
+        t_train = time.perf_counter() - t0
+        acc = float(np.mean(mlp.predict(X, n) == Y.numpy()))
+        nets[name] = (mlp, {"rows": n, "final_loss": loss, "exact_bin_train": acc,
+                            "label_s": t_label, "train_s": t_train})
+        print(f"[learned] trained {name}: {nets[name][1]}", file=sys.stderr, flush=True)
+
+    def structure(pl, nrows):
+        nnz = pl.nnz()
+        cp = np.empty(nrows + 1, np.int64)
+        ci = np.empty(max(nnz, 1), np.int32)
+        lib.sdl_memcpy_dtoh(cp.ctypes.data, lib.sdl_btda_plan_rowptr(pl.h), cp.nbytes)
+        if nnz:
+            lib.sdl_memcpy_dtoh(ci.ctypes.data, lib.sdl_btda_plan_colind(pl.h), 4 * nnz)
+        return cp, ci[:nnz]
+
+    def timed(p, i, n, **kw):
+        Plan((p, i), None, n, n, **kw).close()  # warm-up (kernels, allocator)
+        lib.sdl_synchronize()
+        t0 = time.perf_counter()
+        pl = Plan((p, i), None, n, n, **kw)
+        lib.sdl_synchronize()
+        return pl, time.perf_counter() - t0
+
+    sc = args.rmat_scale
+    products = [("banded 2^20 x 2^20, bandwidth 64", lambda: workloads.banded_csr(1 << 20, 64),
+                 1 << 20),
+                ("strided band 2^19 x 2^21, 129 columns per row starting 4 apart",
+                 lambda: workloads.strided_band_csr(1 << 19, 1 << 21, 129, 4), 1 << 21),
+                (f"R-MAT 2^{sc} nodes, 16 edges/node", lambda: workloads.rmat_csr(sc), 1 << sc)]
+    results = []
+    tot_terms = tot_exact = tot_pred = 0.0
+    for label, make, n in products:
+        p, i, _ = make()
+        ex, t_ex = timed(p, i, n)
+        terms = lib.sdl_btda_plan_terms(ex.h)
+        ref = structure(ex, n)
+        rec = {"product": f"A^T A, {label}", "nnz_A": int(p[-1]), "terms": terms,
+               "nnz_out": ex.nnz(), "exact_symbolic_s": t_ex, "networks": {}}
+        ex.close()
+        runs = [(k, v[0]) for k, v in nets.items()]
+        for name, mlp in runs:
+            pr, t_pr = timed(p, i, n, mode=1, mlp=mlp)
+            got = structure(pr, n)
+            same = bool(np.array_equal(got[0], ref[0]) and np.array_equal(got[1], ref[1]))
+            assert same, (label, name)
+            st = pr.pred_stats()
+            rec["networks"][name] = {"predicted_symbolic_s": t_pr, "gain": t_ex / t_pr,
+                                     "structure_bit_exact": same,
+                                     "misbin_rate": st["misbin"] / n,
+                                     "recount_rate": st["overflow"] / n, **st}
+            pr.close()
+            if name == "with_structured":
+                tot_pred += t_pr
+        tot_terms += terms
+        tot_exact += t_ex
+        results.append(rec)
+        print(f"[learned] {json.dumps(rec)}", file=sys.stderr, flush=True)
+    out = {
+        "metric": "symbolic A^T A terms/s with predicted row binning (learned binning)",
+        "value": tot_terms / tot_pred, "unit": "terms/s", "n_gpus": 1, "steps": 1, "warmup": 1,
+        "ms_per_step": tot_pred * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "int32", "data": "synthetic",
+        "config": {"workload": ("symbolic A^T A of a banded matrix (2^20 rows, bandwidth 64), "
+                                "a strided band (2^19 x 2^21) and an R-MAT graph "
+                                f"(2^{sc} nodes); one step = the three symbolic phases with "
+                                "the network trained with the structured families"),
+                   "products": results,
+                   "networks": {k: v[1] for k, v in nets.items()}},
+        "exact_terms_per_s": tot_terms / tot_exact,
+        "gain_vs_term_bound": tot_exact / tot_pred,
+        "roofline": None,
+        "cpu_baseline": None,
+    }
+    for mlp, _ in nets.values():
+        mlp.close()
+    print(json.dumps(out), flush=True)
 
 
 def main_multi(args):

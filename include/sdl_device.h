@@ -46,7 +46,7 @@ typedef struct sdl_btda_plan sdl_btda_plan;
 enum
 {
     SDL_SYM_EXACT = 0,     /* count pass + fill pass */
-    SDL_SYM_PREDICTED = 1, /* one pass sized by the row-nnz predictor, exact fallback */
+    SDL_SYM_PREDICTED = 1, /* count pass sized by the row-nnz predictor, exact fallback */
     /* or-ed into mode: build the pull schedule (per-entry term lists) whatever
        the terms / nnz ratio, as the batched, captured and range refills need
        it (the reference builds it iff terms <= 3 nnz; values are the same) */
@@ -195,12 +195,15 @@ int sdl_mlp_save(const sdl_mlp *mlp, const char *path);
 int sdl_mlp_load(sdl_mlp *mlp, const char *path);
 /* Predictor for the sparse product allocs of the reference API and for
    sdl_btda_plan_create(..., SDL_SYM_PREDICTED, ...) (NULL: exact passes for
-   the allocs; term-bound tables for SDL_SYM_PREDICTED).
-   The predicted one-pass symbolic phase is currently switched off: binding
-   a non-NULL predictor, SDL_SYM_PREDICTED plans and
-   sdl_btda_plan_create_predicted return an error (sdl_last_error says so). */
+   the allocs; term-bound sizes for SDL_SYM_PREDICTED, i.e. the exact
+   path).  Predicted symbolic phase: the count pass sizes each row's
+   accumulator from the row's predicted nnz bin (never above its term
+   bound), rows whose count exceeds the bin are counted again from the term
+   bound, and the fill pass sizes from the exact counts, so the structure is
+   always exact. */
 int sdl_set_row_predictor(const sdl_mlp *mlp);
-/* Plan with the one-pass predicted symbolic phase (mlp may be NULL). */
+/* Plan with the predicted symbolic phase (mlp NULL: term-bound sizes, the
+   exact path). */
 int sdl_btda_plan_create_predicted(int m, int nA, int nB, const int *dAp, const int *dAi,
                                    const int *dBp, const int *dBi, const sdl_mlp *mlp,
                                    sdl_btda_plan **out);
@@ -208,9 +211,10 @@ int sdl_btda_plan_create_predicted(int m, int nA, int nB, const int *dAp, const 
    row counts (training labels). */
 int sdl_btda_plan_features(const sdl_btda_plan *plan, float *dF);
 int sdl_btda_plan_row_bins(const sdl_btda_plan *plan, int *dY);
-/* Predicted plans: rows served by the one pass, one-pass rows whose table
-   filled up (exact fallback), rows whose predicted bin differs from the bin
-   of their exact nnz, rows sent straight to the exact passes. */
+/* Predicted plans, over non-empty rows: counted within the predicted bound,
+   counted again from the term bound (the bound was exceeded), predicted bin
+   different from the bin of the exact nnz, predicted bound not below the
+   term bound. */
 int sdl_btda_plan_pred_stats(const sdl_btda_plan *plan, long long *out4);
 
 /* ---- device refills of permuted_dense / stacked_pd product outputs ---- */

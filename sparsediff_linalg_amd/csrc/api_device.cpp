@@ -29,11 +29,6 @@ int sdl_btda_plan_create(int m, int nA, int nB, const int *dAp, const int *dAi,
                          const int *dBp, const int *dBi, int mode, sdl_btda_plan **out)
 {
     const int sym = mode & 0xff;
-    if (sym == SDL_SYM_PREDICTED && !sdl::kPredictedSymbolicEnabled)
-    {
-        sdl::set_error("sdl_btda_plan_create: %s", sdl::kPredictedSymbolicOff);
-        return 1;
-    }
     return sdl_btda_plan_create_mode(m, nA, nB, dAp, dAi, dBp, dBi,
                                      sym == SDL_SYM_PREDICTED ? sdl::SymMode::Predicted
                                                               : sdl::SymMode::Exact,

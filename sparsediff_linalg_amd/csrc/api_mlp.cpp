@@ -184,11 +184,6 @@ int sdl_mlp_load(sdl_mlp *p, const char *path)
 
 int sdl_set_row_predictor(const sdl_mlp *p)
 {
-    if (p && !sdl::kPredictedSymbolicEnabled)
-    {
-        sdl::set_error("sdl_set_row_predictor: %s", sdl::kPredictedSymbolicOff);
-        return 1;
-    }
     sdl::g_row_predictor = p ? &p->R : nullptr;
     return 0;
 }
@@ -197,11 +192,6 @@ int sdl_btda_plan_create_predicted(int m, int nA, int nB, const int *dAp, const 
                                    const int *dBp, const int *dBi, const sdl_mlp *mlp,
                                    sdl_btda_plan **out)
 {
-    if (!sdl::kPredictedSymbolicEnabled)
-    {
-        sdl::set_error("sdl_btda_plan_create_predicted: %s", sdl::kPredictedSymbolicOff);
-        return 1;
-    }
     return sdl_btda_plan_create_mode(m, nA, nB, dAp, dAi, dBp, dBi, sdl::SymMode::Predicted,
                                      mlp ? &mlp->R : nullptr, out);
 }

@@ -449,124 +449,59 @@ __global__ void k_row_features(int64_t m_out, int Lm, const int *__restrict__ Lp
     f[15] = lg(span > 0 ? terms / span : 0.0);
 }
 
-// Hash class of a one-pass table of S slots (S = 64 << (class - 1)).
-__device__ __forceinline__ int hash_class_of_slots(int S)
-{
-    int c = 1;
-    for (int s = 64; s < S; s <<= 1) c++;
-    return c;
-}
-
-// One-pass class of every row: the smaller of the term-bound hash class
-// (never overflows) and the predicted bin's table (2 x bin bound); 0 = the
-// row goes to the exact passes (empty rows have no work, and rows whose
-// column bitmap is no bigger than the predicted table keep the bitmap).
-__global__ void k_onepass_class(int64_t m, const int64_t *__restrict__ terms,
-                                const unsigned char *__restrict__ pbin, int nwords,
-                                int lds_words, unsigned char *__restrict__ cls,
-                                int64_t *__restrict__ slots)
+// Count-pass size bound of every row: the term bound, lowered to the largest
+// nnz of the row's predicted bin when that bin is bounded.  The count pass
+// sizes each row's accumulator from this bound, so a row predicted small gets
+// a small LDS hash table in place of the term bound's table or column bitmap.
+__global__ void k_pred_cap(int64_t m, const int64_t *__restrict__ terms,
+                           const unsigned char *__restrict__ pbin, int64_t *__restrict__ cap)
 {
     int64_t row = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
     if (row >= m) return;
     const int64_t t = terms[row];
-    int c = 0;
-    if (t > 0)
-    {
-        const int ct = classify(t, nwords, lds_words);
-        const int b = pbin[row];
-        const int cp = b < MLP_NBIN_BOUNDED ? hash_class_of_slots(2 * mlp_bin_max(b)) : 0;
-        if (ct >= 1 && ct < CLS_BITMAP) c = (cp > 0 && cp < ct) ? cp : ct;
-        else if (cp > 0 && !(ct == CLS_BITMAP && nwords <= (64 << (cp - 1)))) c = cp;
-    }
-    cls[row] = (unsigned char) c;
-    slots[row] = c ? (int64_t) (64 << (c - 1)) : 0;
+    const int b = pbin[row];
+    cap[row] = b < MLP_NBIN_BOUNDED ? min(t, (int64_t) mlp_bin_max(b)) : t;
 }
 
-// One pass: insert every term's column; if the table fills up the row is
-// marked for the exact passes, else its sorted columns go to tmp at toff and
-// its count to rowcnt.
-template <int G, int S>
-__global__ void __launch_bounds__(G)
-    k_sym_hash_once(const int *__restrict__ rows, int Lm, const int *__restrict__ Lp,
-                    const int *__restrict__ Li, int nL, const int *__restrict__ Ap,
-                    const int *__restrict__ Ai, const int64_t *__restrict__ toff,
-                    int *__restrict__ tmp, int *__restrict__ rowcnt,
-                    unsigned char *__restrict__ done)
-{
-    __shared__ int tab[S];
-    __shared__ Expand<G> ex;
-    __shared__ long long red[G / 64];
-    __shared__ int full;
-    const int row = rows[blockIdx.x];
-    for (int i = threadIdx.x; i < S; i += G) tab[i] = -1;
-    if (threadIdx.x == 0) full = 0;
-    __syncthreads();
-    RowCtx c = row_ctx(row, Lm, Lp, nL);
-    for_each_term<G>(ex, c.q0, c.q1, c.koff, Li, Ap, [&](int t, int, int) {
-        if (full) return;
-        if (!hash_insert<S>(tab, Ai[t])) full = 1;
-    });
-    __syncthreads();
-    if (full)
-    {
-        if (threadIdx.x == 0)
-        {
-            rowcnt[row] = 0;
-            done[row] = 0;
-        }
-        return;
-    }
-    long long cnt = 0;
-    for (int i = threadIdx.x; i < S; i += G) cnt += tab[i] != -1;
-    cnt = block_sum_ll<G>(cnt, red);
-    bitonic_sort<G, S>((unsigned *) tab);
-    const int64_t base = toff[row];
-    for (int i = threadIdx.x; i < (int) cnt; i += G) tmp[base + i] = tab[i];
-    if (threadIdx.x == 0)
-    {
-        rowcnt[row] = (int) cnt;
-        done[row] = 1;
-    }
-}
-
-// Rows served by the one pass: copy their columns from tmp to Ci (one
-// wavefront per row).
-__global__ void __launch_bounds__(256)
-    k_copy_onepass(int64_t m, const unsigned char *__restrict__ done,
-                   const int64_t *__restrict__ toff, const int *__restrict__ tmp,
-                   const int64_t *__restrict__ Cp, int *__restrict__ Ci)
-{
-    const int64_t row = (int64_t) blockIdx.x * 4 + (threadIdx.x >> 6);
-    const int lane = threadIdx.x & 63;
-    if (row >= m || !done[row]) return;
-    const int64_t b = Cp[row], cnt = Cp[row + 1] - b, t = toff[row];
-    for (int64_t i = lane; i < cnt; i += 64) Ci[b + i] = tmp[t + i];
-}
-
-// v_out[row] = done[row] ? 0 : v[row]
-template <typename T>
-__global__ void k_mask_done(int64_t m, const unsigned char *__restrict__ done,
-                            const T *__restrict__ v, T *__restrict__ out)
-{
-    int64_t row = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
-    if (row < m) out[row] = done[row] ? (T) 0 : v[row];
-}
-
-// [0] rows served by the one pass, [1] one-pass rows that filled up,
-// [2] rows whose predicted bin differs from the bin of the exact nnz,
-// [3] non-empty rows sent straight to the exact passes
-__global__ void k_pred_stats(int64_t m, const unsigned char *__restrict__ cls,
-                             const unsigned char *__restrict__ done,
-                             const unsigned char *__restrict__ pbin,
-                             const int64_t *__restrict__ terms, const int64_t *__restrict__ Cp,
-                             unsigned long long *__restrict__ out)
+// Rows to count again from the term bound: those whose count exceeded their
+// predicted bound.  A hash table has at least twice its row's bound in slots
+// and a failed insert leaves it full, so a count within the bound is exact.
+__global__ void k_pred_recount(int64_t m, const int *__restrict__ rowcnt,
+                               const int64_t *__restrict__ cap,
+                               const int64_t *__restrict__ terms, int64_t *__restrict__ recount)
 {
     int64_t row = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
     if (row >= m) return;
-    if (done[row]) atomicAdd(&out[0], 1ull);
-    if (cls[row] && !done[row]) atomicAdd(&out[1], 1ull);
-    if (terms[row] > 0 && pbin[row] != mlp_bin_of(Cp[row + 1] - Cp[row])) atomicAdd(&out[2], 1ull);
-    if (!cls[row] && terms[row] > 0) atomicAdd(&out[3], 1ull);
+    const int64_t t = terms[row];
+    recount[row] = (cap[row] < t && (int64_t) rowcnt[row] > cap[row]) ? t : 0;
+}
+
+// Over non-empty rows: [0] counted within the predicted bound, [1] counted
+// again (the predicted bound was exceeded), [2] predicted bin differs from
+// the bin of the exact nnz, [3] predicted bound not below the term bound.
+__global__ void __launch_bounds__(256)
+    k_pred_stats(int64_t m, const int64_t *__restrict__ cap,
+                 const int64_t *__restrict__ recount, const unsigned char *__restrict__ pbin,
+                 const int64_t *__restrict__ terms, const int64_t *__restrict__ Cp,
+                 unsigned long long *__restrict__ out)
+{
+    int64_t row = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    bool f[4] = {false, false, false, false};
+    if (row < m && terms[row] > 0)
+    {
+        const bool lowered = cap[row] < terms[row];
+        f[0] = lowered && recount[row] == 0;
+        f[1] = lowered && recount[row] != 0;
+        f[2] = pbin[row] != mlp_bin_of(Cp[row + 1] - Cp[row]);
+        f[3] = !lowered;
+    }
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+    {
+        const unsigned long long v = __ballot(f[i]);
+        if (lane == 0 && v) atomicAdd(&out[i], (unsigned long long) __popcll(v));
+    }
 }
 
 // Term lists of one output row: entries' term counts, offsets, scatter, then
@@ -1149,7 +1084,8 @@ __global__ void __launch_bounds__(G)
 // update distinct slots in parallel), then the row is gathered into C.
 template <int G>
 __global__ void __launch_bounds__(G)
-    k_num_rowacc_dense(const int *__restrict__ rows, int nrows, int Lm, const int *__restrict__ Lp,
+    k_num_rowacc_dense(const int *__restrict__ rows, int nrows, const int *__restrict__ nrows_dev,
+                       int Lm, const int *__restrict__ Lp,
                        const int *__restrict__ Li, const int *__restrict__ Lperm, int nL,
                        const int *__restrict__ Ap, const int *__restrict__ Ai,
                        const int64_t *__restrict__ Cp, const int *__restrict__ Ci,
@@ -1158,6 +1094,7 @@ __global__ void __launch_bounds__(G)
                        double *__restrict__ scratch, int64_t n)
 {
     double *acc = scratch + (int64_t) blockIdx.x * n;
+    if (nrows_dev) nrows = *nrows_dev;
     for (int ri = blockIdx.x; ri < nrows; ri += gridDim.x)
     {
         const int row = rows[ri];
@@ -1180,6 +1117,181 @@ __global__ void __launch_bounds__(G)
         for (int64_t i = threadIdx.x; i < cnt; i += G) Cx[cbase + i] = acc[Ci[cbase + i]];
         __syncthreads();
     }
+}
+
+// Long rows by rank, for n up to RANK_MAX_WORDS * 32 columns and rows of up
+// to RANK_MAX_ENTRIES entries: the row's output columns become an LDS column
+// bitmap with per-word ranks, so a term's entry position is two LDS reads and
+// a popcount.  Pass 1 marks the entries hit once and those hit more than once;
+// pass 2 writes every once-hit entry directly (0 + its term, the fold of a
+// single term) and lists the terms of the other entries as (entry, L index q,
+// term).  The list is sorted by (entry, q) -- q ascends with k -- and each
+// entry folds its terms in that order, so every entry is the same left fold
+// in ascending k as the serial accumulators'.  Rows whose list outgrows its
+// LDS capacity are appended to `spill` for the dense-accumulator kernel.  A
+// persistent grid strides over the rows (one workgroup per CU: the bitmap and
+// ranks take most of the LDS).
+constexpr int RANK_MAX_WORDS = 16384;      // 524,288 columns
+constexpr int RANK_MAX_ENTRIES = 65536;
+constexpr int RANK_LIST = 1024;
+
+__device__ __forceinline__ int rank_of(const unsigned *bm, const unsigned short *sub,
+                                       const unsigned *sup, int col)
+{
+    const int w = col >> 5;
+    const unsigned b = sup[w >> 10];
+    const unsigned below = bm[w] & ((1u << (col & 31)) - 1u);
+    return (int) (b + (unsigned short) (sub[w] - (unsigned short) b) + __popc(below));
+}
+
+template <int G>
+__global__ void __launch_bounds__(G)
+    k_num_rowacc_rank(const int *__restrict__ rows, int nrows, int Lm,
+                      const int *__restrict__ Lp, const int *__restrict__ Li,
+                      const int *__restrict__ Lperm, int nL, const int *__restrict__ Ap,
+                      const int *__restrict__ Ai, const int64_t *__restrict__ Cp,
+                      const int *__restrict__ Ci, const double *__restrict__ Lval,
+                      const double *__restrict__ d, const double *__restrict__ Ax,
+                      double *__restrict__ Cx, int nwords, int *__restrict__ spill,
+                      int *__restrict__ nspill)
+{
+    extern __shared__ unsigned long long rank_lds[];
+    unsigned long long *lkey = rank_lds;                       // RANK_LIST
+    double *lval = (double *) (lkey + RANK_LIST);              // RANK_LIST
+    unsigned *seen1 = (unsigned *) (lval + RANK_LIST);         // RANK_MAX_ENTRIES / 32
+    unsigned *seen2 = seen1 + RANK_MAX_ENTRIES / 32;           // RANK_MAX_ENTRIES / 32
+    unsigned *bm = seen2 + RANK_MAX_ENTRIES / 32;              // nwords
+    const int nsup = (nwords + 1023) >> 10;
+    unsigned *sup = bm + nwords;                               // nsup
+    unsigned short *sub = (unsigned short *) (sup + nsup);     // nwords
+    __shared__ Expand<G> ex;
+    __shared__ int wsum[G / 64];
+    __shared__ int nlist;
+    const int per = (nwords + G - 1) / G;
+    for (int ri = blockIdx.x; ri < nrows; ri += gridDim.x)
+    {
+        const int row = rows[ri];
+        const int64_t cbase = Cp[row];
+        const int cnt = (int) (Cp[row + 1] - cbase);
+        const int swords = (cnt + 31) >> 5;
+        for (int i = threadIdx.x; i < nwords; i += G) bm[i] = 0u;
+        for (int i = threadIdx.x; i < swords; i += G)
+        {
+            seen1[i] = 0u;
+            seen2[i] = 0u;
+        }
+        if (threadIdx.x == 0) nlist = 0;
+        __syncthreads();
+        for (int i = threadIdx.x; i < cnt; i += G)
+        {
+            const int col = Ci[cbase + i];
+            atomicOr(&bm[col >> 5], 1u << (col & 31));
+        }
+        __syncthreads();
+        // ranks: a contiguous word range per thread, exclusive scan over threads
+        {
+            const int w0 = min(nwords, (int) threadIdx.x * per), w1 = min(nwords, w0 + per);
+            int local = 0;
+            for (int w = w0; w < w1; w++) local += __popc(bm[w]);
+            int tot;
+            int run = block_excl_scan_int<G>(local, &tot, wsum);
+            for (int w = w0; w < w1; w++)
+            {
+                if ((w & 1023) == 0) sup[w >> 10] = (unsigned) run;
+                sub[w] = (unsigned short) run;
+                run += __popc(bm[w]);
+            }
+        }
+        __syncthreads();
+        RowCtx c = row_ctx(row, Lm, Lp, nL);
+        // pass 1: entries hit once / more than once
+        for_each_term<G>(ex, c.q0, c.q1, c.koff, Li, Ap, [&](int t, int, int) {
+            const int pos = rank_of(bm, sub, sup, Ai[t]);
+            const unsigned bit = 1u << (pos & 31);
+            if (atomicOr(&seen1[pos >> 5], bit) & bit) atomicOr(&seen2[pos >> 5], bit);
+        });
+        // pass 2: single-term entries written, the others' terms listed
+        for_each_term<G>(ex, c.q0, c.q1, c.koff, Li, Ap, [&](int t, int q, int k) {
+            const int pos = rank_of(bm, sub, sup, Ai[t]);
+            double v = Lval[Lperm ? Lperm[q] : q];
+            if (d) v = v * d[k];
+            const double x = v * Ax[t];
+            if (!((seen2[pos >> 5] >> (pos & 31)) & 1u))
+            {
+                Cx[cbase + pos] = 0.0 + x;
+            }
+            else
+            {
+                const int i = atomicAdd(&nlist, 1);
+                if (i < RANK_LIST)
+                {
+                    lkey[i] = ((unsigned long long) (unsigned) pos << 32) | (unsigned) q;
+                    lval[i] = x;
+                }
+            }
+        });
+        const int nl = nlist;
+        if (nl > RANK_LIST)
+        {
+            if (threadIdx.x == 0) spill[atomicAdd(nspill, 1)] = row;
+            __syncthreads();
+            continue;
+        }
+        // sort the listed terms by (entry, q): bitonic over the next power of two
+        int np = 1;
+        while (np < nl) np <<= 1;
+        for (int i = nl + (int) threadIdx.x; i < np; i += G)
+        {
+            lkey[i] = ~0ull;
+            lval[i] = 0.0;
+        }
+        __syncthreads();
+        for (int kk = 2; kk <= np; kk <<= 1)
+        {
+            for (int j = kk >> 1; j > 0; j >>= 1)
+            {
+                for (int p = (int) threadIdx.x; p < np / 2; p += G)
+                {
+                    const int i = (p / j) * 2 * j + (p % j);
+                    const int ixj = i + j;
+                    const unsigned long long a = lkey[i], b = lkey[ixj];
+                    const bool up = (i & kk) == 0;
+                    if ((a > b) == up)
+                    {
+                        lkey[i] = b;
+                        lkey[ixj] = a;
+                        const double va = lval[i];
+                        lval[i] = lval[ixj];
+                        lval[ixj] = va;
+                    }
+                }
+                __syncthreads();
+            }
+        }
+        // fold each listed entry's terms in ascending q
+        for (int i = threadIdx.x; i < nl; i += G)
+        {
+            const unsigned pos = (unsigned) (lkey[i] >> 32);
+            if (i > 0 && (unsigned) (lkey[i - 1] >> 32) == pos) continue;
+            double acc = 0.0;
+            for (int j = i; j < nl && (unsigned) (lkey[j] >> 32) == pos; j++) acc = acc + lval[j];
+            Cx[cbase + pos] = acc;
+        }
+        __syncthreads();
+    }
+}
+
+__global__ void k_zero_int(int *p, int n)
+{
+    const int i = (int) (blockIdx.x * blockDim.x + threadIdx.x);
+    if (i < n) p[i] = 0;
+}
+
+size_t rank_lds_bytes(int nwords)
+{
+    const int nsup = (nwords + 1023) >> 10;
+    return (size_t) RANK_LIST * 16 + (size_t) (RANK_MAX_ENTRIES / 32) * 8 + (size_t) nwords * 4 +
+           (size_t) nsup * 4 + (size_t) nwords * 2;
 }
 
 // ------------------------------------------------------------- host ----
@@ -1268,46 +1380,21 @@ void run_symbolic_class(int c, const Bins &B, const ProductPlan &P, int nwords, 
     SDL_HIP(hipGetLastError());
 }
 
-void run_onepass_class(int c, const Bins &B, const ProductPlan &P, const int64_t *toff, int *tmp,
-                       int *rowcnt, unsigned char *done)
+// Count pass: each row's accumulator sized from cap[row] (rows with cap 0 are
+// skipped); rowcnt[row] = the row's distinct columns.
+void symbolic_count(const ProductPlan &P, const int64_t *cap, int *rowcnt, int nwords)
 {
-    int nrows = B.count[c];
-    if (nrows == 0) return;
-    const int *rows = B.rows.p + B.off[c];
-    const int nL = P.A.Am / P.L.blocks;
-    const LView &L = P.L;
-    const AView &A = P.A;
-#define SDL_ONCE_CASE(CLS, G, S)                                                              \
-    case CLS:                                                                                 \
-        hipLaunchKernelGGL((k_sym_hash_once<G, S>), dim3(nrows), dim3(G), 0, stream(), rows,  \
-                           L.Lm, L.Lp, L.Li, nL, A.Ap, A.Ai, toff, tmp, rowcnt, done);        \
-        break;
-    switch (c)
-    {
-        SDL_ONCE_CASE(1, 64, 64)
-        SDL_ONCE_CASE(2, 64, 128)
-        SDL_ONCE_CASE(3, 64, 256)
-        SDL_ONCE_CASE(4, 128, 512)
-        SDL_ONCE_CASE(5, 128, 1024)
-        SDL_ONCE_CASE(6, 256, 2048)
-        SDL_ONCE_CASE(7, 256, 4096)
-        SDL_ONCE_CASE(8, 256, 8192)
-    default: break;
-    }
-#undef SDL_ONCE_CASE
-    SDL_HIP(hipGetLastError());
+    Bins B;
+    bin_rows<int64_t>(B, cap, P.m_out, nwords);
+    for (int c = 1; c < NCLASS; c++) run_symbolic_class<0>(c, B, P, nwords, rowcnt, nullptr, nullptr);
+    sync();
 }
 
-// Count pass (tables sized from the term bound), scan, fill pass (tables
-// sized from the exact count).
-void symbolic_exact(ProductPlan &P, const int64_t *rterms, int *rowcnt, int nwords)
+// Scan of the exact counts, then the fill pass (tables sized from the exact
+// count).
+void symbolic_fill(ProductPlan &P, const int *rowcnt, int nwords)
 {
     const int64_t m_out = P.m_out;
-    {
-        Bins B;
-        bin_rows<int64_t>(B, rterms, m_out, nwords);
-        for (int c = 1; c < NCLASS; c++) run_symbolic_class<0>(c, B, P, nwords, rowcnt, nullptr, nullptr);
-    }
     P.nnz = scan_i32_to_i64(rowcnt, P.Cp.p, m_out, true);
     P.stats.nnz = P.nnz;
     P.Ci.alloc(P.nnz);
@@ -1320,19 +1407,30 @@ void symbolic_exact(ProductPlan &P, const int64_t *rterms, int *rowcnt, int nwor
     }
     P.stats.bitmap_rows = B2.count[CLS_BITMAP];
     P.stats.global_rows = B2.count[CLS_GLOBAL];
+    sync();
 }
 
-// One pass with predicted table sizes, then the exact passes for the rows it
-// could not serve, then the one-pass rows' columns copied into place.
+// Count pass with accumulators sized from the term bound, scan, fill pass.
+void symbolic_exact(ProductPlan &P, const int64_t *rterms, int *rowcnt, int nwords)
+{
+    symbolic_count(P, rterms, rowcnt, nwords);
+    symbolic_fill(P, rowcnt, nwords);
+}
+
+// Count pass with accumulators sized from the predicted bins (never above the
+// term bound), a second count from the term bound for the rows whose count
+// exceeded the predicted bound, then the scan and fill pass of the exact path.
+// With no predictor every bin is unbounded and this is the exact path.
 void symbolic_predicted(ProductPlan &P, const RowPredictor *pred, const int64_t *rterms,
                         int *rowcnt, int nwords)
 {
     const int64_t m_out = P.m_out;
     const unsigned grid = (unsigned) ((m_out + 255) / 256);
     DBuf<unsigned char> pbin(m_out);
+    DBuf<float> F;
     if (pred)
     {
-        DBuf<float> F((size_t) m_out * MLP_IN);
+        F.alloc((size_t) m_out * MLP_IN);
         product_row_features(P.L, P.A, F.p, stream());
         mlp_predict(*pred, F.p, m_out, pbin.p, stream());
     }
@@ -1340,54 +1438,19 @@ void symbolic_predicted(ProductPlan &P, const RowPredictor *pred, const int64_t 
     {
         SDL_HIP(hipMemsetAsync(pbin.p, MLP_NBIN_BOUNDED, (size_t) m_out, stream()));
     }
-    DBuf<unsigned char> cls(m_out), done(m_out);
-    DBuf<int64_t> slots(m_out), soff(m_out + 1);
-    hipLaunchKernelGGL(k_onepass_class, dim3(grid), dim3(256), 0, stream(), m_out, rterms, pbin.p,
-                       nwords, lds_bitmap_words(), cls.p, slots.p);
+    DBuf<int64_t> cap(m_out), recount(m_out);
+    hipLaunchKernelGGL(k_pred_cap, dim3(grid), dim3(256), 0, stream(), m_out, rterms, pbin.p,
+                       cap.p);
     SDL_HIP(hipGetLastError());
-    dzero(done.p, done.bytes());
-    const int64_t tmp_total = scan_i64(slots.p, soff.p, m_out, true);
-    DBuf<int> tmp(std::max<int64_t>(1, tmp_total));
-    {
-        Bins B;
-        bin_rows<unsigned char>(B, cls.p, m_out, nwords, 2);
-        for (int c = 1; c <= 8; c++) run_onepass_class(c, B, P, soff.p, tmp.p, rowcnt, done.p);
-    }
-    // exact count pass for the rest
-    {
-        DBuf<int64_t> fterms(m_out);
-        hipLaunchKernelGGL(k_mask_done<int64_t>, dim3(grid), dim3(256), 0, stream(), m_out,
-                           done.p, rterms, fterms.p);
-        SDL_HIP(hipGetLastError());
-        Bins B;
-        bin_rows<int64_t>(B, fterms.p, m_out, nwords);
-        for (int c = 1; c < NCLASS; c++) run_symbolic_class<0>(c, B, P, nwords, rowcnt, nullptr, nullptr);
-    }
-    P.nnz = scan_i32_to_i64(rowcnt, P.Cp.p, m_out, true);
-    P.stats.nnz = P.nnz;
-    P.Ci.alloc(P.nnz);
-    hipLaunchKernelGGL(k_copy_onepass, dim3((unsigned) ((m_out + 3) / 4)), dim3(256), 0, stream(),
-                       m_out, done.p, soff.p, tmp.p, P.Cp.p, P.Ci.p);
+    symbolic_count(P, cap.p, rowcnt, nwords);
+    hipLaunchKernelGGL(k_pred_recount, dim3(grid), dim3(256), 0, stream(), m_out, rowcnt, cap.p,
+                       rterms, recount.p);
     SDL_HIP(hipGetLastError());
-    // exact fill pass for the rest
-    {
-        DBuf<int> fcnt(m_out);
-        hipLaunchKernelGGL(k_mask_done<int>, dim3(grid), dim3(256), 0, stream(), m_out, done.p,
-                           rowcnt, fcnt.p);
-        SDL_HIP(hipGetLastError());
-        Bins B2;
-        bin_rows<int>(B2, fcnt.p, m_out, nwords);
-        for (int c = 1; c < NCLASS; c++)
-        {
-            run_symbolic_class<1>(c, B2, P, nwords, nullptr, P.Cp.p, P.Ci.p);
-            if (c >= 1 && c <= 8) P.stats.hash_rows += B2.count[c];
-        }
-        P.stats.bitmap_rows = B2.count[CLS_BITMAP];
-        P.stats.global_rows = B2.count[CLS_GLOBAL];
-    }
+    symbolic_count(P, recount.p, rowcnt, nwords);
+    symbolic_fill(P, rowcnt, nwords);
     DBuf<unsigned long long> st(4);
     dzero(st.p, st.bytes());
-    hipLaunchKernelGGL(k_pred_stats, dim3(grid), dim3(256), 0, stream(), m_out, cls.p, done.p,
+    hipLaunchKernelGGL(k_pred_stats, dim3(grid), dim3(256), 0, stream(), m_out, cap.p, recount.p,
                        pbin.p, rterms, P.Cp.p, st.p);
     SDL_HIP(hipGetLastError());
     unsigned long long h[4];
@@ -1396,7 +1459,6 @@ void symbolic_predicted(ProductPlan &P, const RowPredictor *pred, const int64_t 
     P.stats.overflow_rows = (long long) h[1];
     P.stats.misbin_rows = pred ? (long long) h[2] : 0;
     P.stats.exact_rows = (long long) h[3];
-    P.stats.hash_rows += (long long) h[0];
 }
 
 } // namespace
@@ -1460,10 +1522,9 @@ void product_plan(ProductPlan &P, const LView &L, const AView &A, SymMode mode,
     P.terms = scan_i64(rterms.p, toff.p, m_out, true);
     P.stats.terms = P.terms;
 
-    // structure: exact count + fill passes, or the predicted one pass
+    // structure: count pass sized from the term bound or the predicted bins, fill pass
     DBuf<int> rowcnt(m_out);
     dzero(rowcnt.p, rowcnt.bytes());
-    SDL_REQUIRE(mode != SymMode::Predicted || kPredictedSymbolicEnabled, kPredictedSymbolicOff);
     if (mode == SymMode::Predicted) symbolic_predicted(P, pred, rterms.p, rowcnt.p, nwords);
     else symbolic_exact(P, rterms.p, rowcnt.p, nwords);
 
@@ -1513,24 +1574,39 @@ void product_plan(ProductPlan &P, const LView &L, const AView &A, SymMode mode,
     }
     else
     {
-        // row-accumulator bins: rows with <= 1024 entries accumulate in LDS, larger
-        // rows in place in C.x
+        // row-accumulator bins: rows with <= 1024 entries accumulate in LDS; rows
+        // up to RANK_MAX_ENTRIES entries by rank when the column bitmap fits
+        // LDS; longer rows in dense accumulators
         Bins BR;
-        bin_rows<int>(BR, rowcnt.p, m_out, nwords, 1, 1024, 1024);
-        int small = BR.count[1], large = BR.count[3];
-        P.rbins.alloc(std::max(1, small + large));
+        bin_rows<int>(BR, rowcnt.p, m_out, nwords, 1, 1024, RANK_MAX_ENTRIES);
+        const int small = BR.count[1], mid = BR.count[2], large = BR.count[3];
+        P.rbins.alloc(std::max(1, small + mid + large));
         P.rbin_off[0] = 0;
         P.rbin_off[1] = small;
-        P.rbin_off[2] = small + large;
+        P.rbin_off[2] = small + mid;
+        P.rbin_off[3] = small + mid + large;
         d2d(P.rbins.p, BR.rows.p + BR.off[1], (size_t) small * sizeof(int));
-        d2d(P.rbins.p + small, BR.rows.p + BR.off[3], (size_t) large * sizeof(int));
-        // dense accumulators for the long rows: at most 2 GB, at least 64 rows
-        // in flight, else C itself is the accumulator
+        d2d(P.rbins.p + small, BR.rows.p + BR.off[2], (size_t) mid * sizeof(int));
+        d2d(P.rbins.p + small + mid, BR.rows.p + BR.off[3], (size_t) large * sizeof(int));
+        P.rank = mid > 0 && nwords <= RANK_MAX_WORDS;
+        if (P.rank)
+        {
+            P.spill.alloc(mid);
+            P.nspill.alloc(1);
+            int dev = 0, cus = 0;
+            SDL_HIP(hipGetDevice(&dev));
+            SDL_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+            P.rank_grid = std::max(1, std::min(mid, cus));
+        }
+        // dense accumulators for the long rows (and rows spilled by the rank
+        // kernel): at most 2 GB and 1024 workgroups, at least 64, else C itself
+        // is the accumulator
         const int64_t budget = (int64_t) 2 << 30;
         const int64_t fit = budget / (8 * std::max<int64_t>(P.n, 1));
-        if (large > 0 && fit >= 64)
+        const int dense_rows = large + mid;
+        if (dense_rows > 0 && fit >= 64)
         {
-            P.dense_grid = (int) std::min<int64_t>({(int64_t) large, fit, 1024});
+            P.dense_grid = (int) std::min<int64_t>({(int64_t) dense_rows, fit, 1024});
             P.acc.alloc((size_t) P.dense_grid * std::max<int64_t>(P.n, 1));
         }
         sync();
@@ -1549,21 +1625,48 @@ void product_numeric(const ProductPlan &P, const double *Lval, const double *d,
     else
     {
         const int nL = P.A.Am / P.L.blocks;
-        int n0 = P.rbin_off[1] - P.rbin_off[0];
-        int n1 = P.rbin_off[2] - P.rbin_off[1];
+        const int n0 = P.rbin_off[1] - P.rbin_off[0];
+        const int *long_rows = P.rbins.p + P.rbin_off[1];
+        int n1 = P.rbin_off[3] - P.rbin_off[1];
         if (n0)
             hipLaunchKernelGGL((k_num_rowacc<256, 1024>), dim3(n0), dim3(256), 0, s, P.rbins.p,
                                P.L.Lm, P.L.Lp, P.L.Li, P.L.Lperm, nL, P.A.Ap, P.A.Ai, P.Cp.p,
                                P.Ci.p, Lval, d, Ax, Cx);
+        if (P.rank)
+        {
+            // rows of the rank bin; its spills and the longer rows follow
+            const int nmid = P.rbin_off[2] - P.rbin_off[1];
+            const int nwords = (int) ((P.n + 31) / 32);
+            const size_t lds = rank_lds_bytes(nwords);
+            static bool attr_set = false;
+            if (!attr_set)
+            {
+                SDL_HIP(hipFuncSetAttribute((const void *) k_num_rowacc_rank<1024>,
+                                            hipFuncAttributeMaxDynamicSharedMemorySize,
+                                            (int) rank_lds_bytes(RANK_MAX_WORDS)));
+                attr_set = true;
+            }
+            hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, s, P.nspill.p, 1);
+            hipLaunchKernelGGL((k_num_rowacc_rank<1024>), dim3(P.rank_grid), dim3(1024), lds, s,
+                               long_rows, nmid, P.L.Lm, P.L.Lp, P.L.Li, P.L.Lperm, nL, P.A.Ap,
+                               P.A.Ai, P.Cp.p, P.Ci.p, Lval, d, Ax, Cx, nwords, P.spill.p,
+                               P.nspill.p);
+            hipLaunchKernelGGL((k_num_rowacc_dense<256>), dim3(P.dense_grid), dim3(256), 0, s,
+                               P.spill.p, 0, P.nspill.p, P.L.Lm, P.L.Lp, P.L.Li, P.L.Lperm, nL,
+                               P.A.Ap, P.A.Ai, P.Cp.p, P.Ci.p, Lval, d, Ax, Cx, P.acc.p,
+                               (int64_t) P.n);
+            long_rows += nmid;
+            n1 -= nmid;
+        }
         if (n1 && P.dense_grid > 0)
             hipLaunchKernelGGL((k_num_rowacc_dense<256>), dim3(P.dense_grid), dim3(256), 0, s,
-                               P.rbins.p + P.rbin_off[1], n1, P.L.Lm, P.L.Lp, P.L.Li, P.L.Lperm,
-                               nL, P.A.Ap, P.A.Ai, P.Cp.p, P.Ci.p, Lval, d, Ax, Cx, P.acc.p,
-                               (int64_t) P.n);
+                               long_rows, n1, (const int *) nullptr, P.L.Lm, P.L.Lp, P.L.Li,
+                               P.L.Lperm, nL, P.A.Ap, P.A.Ai, P.Cp.p, P.Ci.p, Lval, d, Ax, Cx,
+                               P.acc.p, (int64_t) P.n);
         else if (n1)
-            hipLaunchKernelGGL((k_num_rowacc<256, 0>), dim3(n1), dim3(256), 0, s,
-                               P.rbins.p + P.rbin_off[1], P.L.Lm, P.L.Lp, P.L.Li, P.L.Lperm, nL,
-                               P.A.Ap, P.A.Ai, P.Cp.p, P.Ci.p, Lval, d, Ax, Cx);
+            hipLaunchKernelGGL((k_num_rowacc<256, 0>), dim3(n1), dim3(256), 0, s, long_rows,
+                               P.L.Lm, P.L.Lp, P.L.Li, P.L.Lperm, nL, P.A.Ap, P.A.Ai, P.Cp.p,
+                               P.Ci.p, Lval, d, Ax, Cx);
     }
     SDL_HIP(hipGetLastError());
 }

@@ -77,13 +77,4 @@ float mlp_train(RowPredictor &R, MlpScratch &S, const float *X, const int *Y, in
 // exact symbolic passes).
 extern const RowPredictor *g_row_predictor;
 
-// The one-pass predicted symbolic phase is switched off: its first GPU test
-// ended in a device memory fault whose cause is not yet found (the CPU
-// emulation of the same kernels under AddressSanitizer is clean).  Plans
-// requested in predicted mode fail with this message.
-constexpr bool kPredictedSymbolicEnabled = false;
-constexpr const char *kPredictedSymbolicOff =
-    "the predicted one-pass symbolic phase is switched off (an unexplained GPU memory fault in "
-    "its first test); use the exact symbolic phase";
-
 } // namespace sdl

@@ -32,18 +32,18 @@ struct AView
 enum class SymMode
 {
     Exact = 0,      // count pass + fill pass, tables sized from the term bound / exact count
-    Predicted = 1,  // one pass with tables sized from a per-row prediction; overflowing
-                    // rows fall back to the exact path
+    Predicted = 1,  // count pass with tables sized from a per-row prediction; rows
+                    // over their predicted bound are counted again from the term bound
 };
 
 struct SymStats
 {
     long long rows = 0, terms = 0, nnz = 0;
     long long hash_rows = 0, bitmap_rows = 0, global_rows = 0;
-    long long overflow_rows = 0;       // predicted path: one-pass rows whose table filled up
-    long long predicted_rows = 0;      // predicted path: rows served by the one pass
+    long long overflow_rows = 0;       // predicted path: count exceeded the predicted bound
+    long long predicted_rows = 0;      // predicted path: counted within the predicted bound
     long long misbin_rows = 0;         // predicted path: predicted bin != bin of the exact nnz
-    long long exact_rows = 0;          // predicted path: rows sent to the exact passes directly
+    long long exact_rows = 0;          // predicted path: predicted bound not below the term bound
 };
 
 // Launch configuration of the batched pull refill (picked by the autotuner):
@@ -71,9 +71,14 @@ struct ProductPlan
     bool pull = false;
     DBuf<int64_t> ep;
     DBuf<int> tA, tB, tK;
-    // Row-accumulator schedule (otherwise): rows binned by nnz.
+    // Row-accumulator schedule (otherwise): rows binned by nnz -- [0, 1) up to
+    // 1024 entries (LDS accumulators), [1, 2) up to 65536 (by rank when the
+    // column bitmap fits LDS), [2, 3) longer.
     DBuf<int> rbins;                 // row ids grouped by class
     int rbin_off[8] = {0};           // class c is rbins[rbin_off[c] .. rbin_off[c+1])
+    bool rank = false;               // bin 1 by rank; its spills in spill[0 .. *nspill)
+    int rank_grid = 0;
+    DBuf<int> spill, nspill;
     // Rows too long for LDS accumulate in a dense per-workgroup accumulator of
     // n doubles (reserved here: dense_grid * n), or in C itself when n is too
     // large for that (dense_grid == 0).
@@ -100,10 +105,11 @@ struct RowPredictor;
 
 // Build the exact structure and the refill schedule (pull iff terms <= 3 nnz,
 // the reference's plan policy; force_pull: whenever the counts fit int32 --
-// both schedules give the same values bit for bit).  SymMode::Predicted: one
-// pass whose per-row hash tables are sized from the predictor's bins (pred
-// may be null: tables sized from the term bound only); rows whose table
-// fills up, and rows no table suits, take the exact count + fill passes.
+// both schedules give the same values bit for bit).  SymMode::Predicted: the
+// count pass sizes each row's accumulator from its predicted bin (never above
+// the term bound; pred may be null: the term bound, i.e. the exact path), rows
+// whose count exceeds the predicted bound are counted again from the term
+// bound, and the fill pass is the exact path's.
 void product_plan(ProductPlan &P, const LView &L, const AView &A, SymMode mode,
                   const RowPredictor *pred, bool allow_pull = true, bool force_pull = false);
 

@@ -7,7 +7,12 @@ symbolic plans (features and exact-count bins computed on the GPU), train the
 trains on the products of the sparse B^T D A workload (20000 x 5000, 6
 nnz/row), the A^T A of rank-Zipf(1.2) matrices (mean row length 16, several
 sizes) and the A^T A of the stacked permuted-dense operand read as CSR, and
-writes the parameters (the file shipped beside this module is made this way).
+writes the parameters.  With --structured the set also holds the A^T A of
+banded matrices (bandwidths 4, 16, 48, 96 on 2^15 rows), of strided bands
+(2^16 rows of 65 or 129 consecutive columns starting 2 or 8 columns apart, in
+2^17 or 2^21 columns) and of an R-MAT graph (2^13 nodes): structured column patterns whose nnz follows the
+column span rather than the uniform collision estimate (the file shipped beside this
+module is made this way).
 """
 from __future__ import annotations
 
@@ -144,10 +149,11 @@ class Predictor:
             pass
 
 
-def training_products(small=False):
+def training_products(small=False, structured=False):
     """(name, A, B, nA, nB) products whose exact plans label the training set:
     the sparse B^T D A workload, A^T A of rank-Zipf matrices and of the
-    stacked permuted-dense operand as CSR."""
+    stacked permuted-dense operand as CSR; with `structured`, also A^T A of
+    banded matrices and of an R-MAT graph."""
     from . import workloads as W
     out = []
     w1 = W.btda_config1(1, m=4000 if small else 20000, n=1000 if small else 5000)
@@ -159,16 +165,31 @@ def training_products(small=False):
     wb = W.blocks_config3(3, nblocks=32 if small else 256)
     p, i, _ = W.blocks_as_csr(wb)
     out.append(("blocks", (p, i), None, wb["n"], wb["n"]))
+    if structured:
+        nb = 1 << (13 if small else 15)
+        for bw in (4, 16, 48, 96):
+            p, i, _ = W.banded_csr(nb, bw)
+            out.append((f"banded{bw}", (p, i), None, nb, nb))
+        for width, stride in ((65, 2), (129, 8)):
+            for ncols in (4 * nb, 64 * nb):
+                p, i, _ = W.strided_band_csr(2 * nb, ncols, width, stride)
+                out.append((f"strided{width}x{stride}n{ncols}", (p, i), None, ncols, ncols))
+        sc = 11 if small else 13
+        p, i, _ = W.rmat_csr(sc, seed=3)
+        out.append((f"rmat{sc}", (p, i), None, 1 << sc, 1 << sc))
     return out
 
 
 def training_set(products):
-    """Features and labels of the products' output rows (device arrays)."""
+    """Features and labels of the products' non-empty output rows (device
+    arrays; rows without terms take no accumulator)."""
     Fs, Ys = [], []
     for _, A, B, nA, nB in products:
         pl = Plan(A, B, nA, nB)
-        Fs.append(pl.features().numpy())
-        Ys.append(pl.bins().numpy())
+        F = pl.features().numpy()
+        keep = F[:, 0] > 0  # log2(1 + term bound)
+        Fs.append(F[keep])
+        Ys.append(pl.bins().numpy()[keep])
         pl.close()
     X = np.ascontiguousarray(np.concatenate(Fs), np.float32)
     Y = np.ascontiguousarray(np.concatenate(Ys), np.int32)
@@ -178,10 +199,11 @@ def training_set(products):
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--out", default=DEFAULT_PATH)
-    ap.add_argument("--epochs", type=int, default=60)
+    ap.add_argument("--epochs", type=int, default=100)
     ap.add_argument("--small", action="store_true")
+    ap.add_argument("--structured", action="store_true")
     args = ap.parse_args()
-    X, Y, n = training_set(training_products(args.small))
+    X, Y, n = training_set(training_products(args.small, args.structured))
     mlp = Predictor(seed=1)
     loss = mlp.train(X, Y, n, epochs=args.epochs)
     pred = mlp.predict(X, n)

@@ -111,6 +111,19 @@ def zipf_csr(seed: int, m: int, n: int, mean: float = 16.0):
     return csr_from_lengths(seed, zipf_lengths(seed, m, mean=mean), n)
 
 
+def strided_band_csr(m: int, n: int, width: int, stride: int, seed: int = 5):
+    """m x n: row i holds the `width` consecutive columns starting at
+    (stride * i) mod (n - width + 1); FP64 N(0,1) values.  Each column of A^T A
+    collects about width / stride rows whose columns span about
+    width + width - 1 around it, so the product rows have width * width /
+    stride terms but only ~2 * width - 1 distinct columns."""
+    start = (stride * np.arange(m, dtype=np.int64)) % (n - width + 1)
+    p = np.arange(m + 1, dtype=np.int64) * width
+    i = (start[:, None] + np.arange(width, dtype=np.int64)[None, :]).reshape(-1).astype(np.int32)
+    x = normal(seed, m * width)
+    return p.astype(np.int32), i, x
+
+
 def banded_csr(m: int, bw: int, seed: int = 5):
     """m x m banded: row i holds columns max(0, i-bw) .. min(m-1, i+bw); FP64
     N(0,1) values."""
