@@ -71,6 +71,18 @@ def parse():
     return ap.parse_args()
 
 
+def measured_traffic(name):
+    """HBM bytes per launch measured by rocprofv3 counters (FETCH_SIZE and
+    WRITE_SIZE runs, scaled by a streaming-copy calibration), from the summary
+    kept under profiles/; None when absent."""
+    path = os.path.join(ROOT, "profiles", name)
+    try:
+        with open(path) as f:
+            return json.load(f).get("hbm_bytes_per_launch")
+    except (OSError, ValueError):
+        return None
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -190,14 +202,7 @@ def main():
     value = K * nnz / (ms_step * 1e-3)
     achieved = bytes_K / (ms_step * 1e-3) / 1e9
 
-    traffic = None
-    tpath = os.path.join(ROOT, "profiles", "btda_refill_config1_traffic.json")
-    if os.path.exists(tpath):
-        try:
-            with open(tpath) as f:
-                traffic = json.load(f).get("hbm_bytes_per_launch")
-        except Exception:
-            traffic = None
+    traffic = measured_traffic("btda_refill_config1_traffic.json")
 
     cpu = None
     if not args.no_cpu:
@@ -322,9 +327,11 @@ def main_ata_large(args):
             "schedule": "pull" if lib.sdl_btda_plan_is_pull(pl.h) else "row accumulators",
             "sampled_max_rel_err_vs_scipy": worst,
         },
-        "roofline": {"bound": "hbm", "kernel": "row-accumulator refill (dense accumulators)",
+        "roofline": {"bound": "hbm",
+                     "kernel": "k_num_rowacc_rank<1024> (rows by rank) + k_num_rowacc_dense spills",
                      "achieved": bytes_ / t / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": bytes_ / t / 1e9 / HBM_PEAK_GBS, "traffic": None,
+                     "frac": bytes_ / t / 1e9 / HBM_PEAK_GBS,
+                     "traffic": measured_traffic("k_num_rowacc_rank_ata_large_traffic.json"),
                      "alg_bytes_per_launch": bytes_},
         "gflops": (nnzA + 2 * terms) / t / 1e9,
         "symbolic_terms_per_s": terms / t_sym,

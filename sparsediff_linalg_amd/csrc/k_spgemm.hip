@@ -147,6 +147,52 @@ __device__ __forceinline__ void for_each_term(Expand<G> &ex, int q0, int q1, int
     }
 }
 
+// The chunk setup of for_each_term as a separate step (for kernels that
+// process several terms per thread at once): L entries qb .. qb+G-1 of the
+// row with their A-row starts and exclusive term offsets; returns the chunk's
+// term count.  All G threads must call it; it ends with a barrier.
+template <int G>
+__device__ __forceinline__ int expand_chunk(Expand<G> &ex, int qb, int q1, int koff,
+                                            const int *__restrict__ Li,
+                                            const int *__restrict__ Ap)
+{
+    int q = qb + (int) threadIdx.x;
+    int len = 0, st = 0, k = 0;
+    if (q < q1)
+    {
+        k = Li[q] + koff;
+        st = Ap[k];
+        len = Ap[k + 1] - st;
+    }
+    int tot;
+    int ex_off = block_excl_scan_int<G>(len, &tot, ex.wsum);
+    ex.start[threadIdx.x] = st;
+    ex.k[threadIdx.x] = k;
+    ex.q[threadIdx.x] = q;
+    ex.off[threadIdx.x] = ex_off;
+    if (threadIdx.x == 0) ex.off[G] = tot;
+    __syncthreads();
+    return tot;
+}
+
+// Term fi of the current chunk (nq = its number of L entries): A entry t,
+// L entry q, A row k.
+template <int G>
+__device__ __forceinline__ void expand_term(const Expand<G> &ex, int nq, int fi, int &t, int &q,
+                                            int &k)
+{
+    int lo = 0, hi = nq - 1;
+    while (lo < hi)
+    {
+        int mid = (lo + hi + 1) >> 1;
+        if (ex.off[mid] <= fi) lo = mid;
+        else hi = mid - 1;
+    }
+    t = ex.start[lo] + (fi - ex.off[lo]);
+    q = ex.q[lo];
+    k = ex.k[lo];
+}
+
 template <int S> __device__ __forceinline__ unsigned hash_slot(int col)
 {
     constexpr int LOG2 = __builtin_ctz(S);
@@ -277,15 +323,27 @@ __global__ void k_classify(int64_t m, const T *__restrict__ cap, int mode, int n
     if (threadIdx.x < NCLASS && hist[threadIdx.x]) atomicAdd(&counts[threadIdx.x], hist[threadIdx.x]);
 }
 
+// Row ids into their class's range: one atomic per (wavefront, class) --
+// most rows of a product share a class, and per-row atomics on one counter
+// serialise.
 __global__ void k_scatter_rows(int64_t m, const unsigned char *__restrict__ cls,
                                int *__restrict__ cursor, int *__restrict__ rows)
 {
-    int64_t row = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
-    if (row >= m) return;
-    int c = cls[row];
-    if (c == 0) return;
-    int pos = atomicAdd(&cursor[c], 1);
-    rows[pos] = (int) row;
+    const int64_t row = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    const int c = row < m ? (int) cls[row] : 0;
+    unsigned long long todo = __ballot(c != 0);
+    while (todo)
+    {
+        const int leader = __ffsll((long long) todo) - 1;
+        const int cl = __shfl(c, leader, 64);
+        const unsigned long long same = __ballot(c == cl);
+        int base = 0;
+        if (lane == leader) base = atomicAdd(&cursor[cl], __popcll(same));
+        base = __shfl(base, leader, 64);
+        if (c == cl) rows[base + __popcll(same & ((1ull << lane) - 1ull))] = (int) row;
+        todo &= ~same;
+    }
 }
 
 // PASS 0: count distinct columns -> rowcnt.  PASS 1: write sorted columns.
@@ -1134,6 +1192,7 @@ __global__ void __launch_bounds__(G)
 constexpr int RANK_MAX_WORDS = 16384;      // 524,288 columns
 constexpr int RANK_MAX_ENTRIES = 65536;
 constexpr int RANK_LIST = 1024;
+constexpr int RU = 4;                      // terms per thread per step of the rank passes
 
 __device__ __forceinline__ int rank_of(const unsigned *bm, const unsigned short *sub,
                                        const unsigned *sup, int col)
@@ -1182,10 +1241,18 @@ __global__ void __launch_bounds__(G)
         }
         if (threadIdx.x == 0) nlist = 0;
         __syncthreads();
-        for (int i = threadIdx.x; i < cnt; i += G)
+        for (int i0 = threadIdx.x; i0 < cnt; i0 += RU * G)
         {
-            const int col = Ci[cbase + i];
-            atomicOr(&bm[col >> 5], 1u << (col & 31));
+            int col[RU];
+#pragma unroll
+            for (int u = 0; u < RU; u++)
+            {
+                const int i = i0 + u * G;
+                col[u] = i < cnt ? Ci[cbase + i] : -1;
+            }
+#pragma unroll
+            for (int u = 0; u < RU; u++)
+                if (col[u] >= 0) atomicOr(&bm[col[u] >> 5], 1u << (col[u] & 31));
         }
         __syncthreads();
         // ranks: a contiguous word range per thread, exclusive scan over threads
@@ -1204,32 +1271,82 @@ __global__ void __launch_bounds__(G)
         }
         __syncthreads();
         RowCtx c = row_ctx(row, Lm, Lp, nL);
-        // pass 1: entries hit once / more than once
-        for_each_term<G>(ex, c.q0, c.q1, c.koff, Li, Ap, [&](int t, int, int) {
-            const int pos = rank_of(bm, sub, sup, Ai[t]);
-            const unsigned bit = 1u << (pos & 31);
-            if (atomicOr(&seen1[pos >> 5], bit) & bit) atomicOr(&seen2[pos >> 5], bit);
-        });
-        // pass 2: single-term entries written, the others' terms listed
-        for_each_term<G>(ex, c.q0, c.q1, c.koff, Li, Ap, [&](int t, int q, int k) {
-            const int pos = rank_of(bm, sub, sup, Ai[t]);
-            double v = Lval[Lperm ? Lperm[q] : q];
-            if (d) v = v * d[k];
-            const double x = v * Ax[t];
-            if (!((seen2[pos >> 5] >> (pos & 31)) & 1u))
+        // pass 1: entries hit once / more than once (RU terms per thread at a
+        // time, their loads issued together)
+        for (int qb = c.q0; qb < c.q1; qb += G)
+        {
+            const int tot = expand_chunk<G>(ex, qb, c.q1, c.koff, Li, Ap);
+            const int nq = min(G, c.q1 - qb);
+            for (int f0 = threadIdx.x; f0 < tot; f0 += RU * G)
             {
-                Cx[cbase + pos] = 0.0 + x;
-            }
-            else
-            {
-                const int i = atomicAdd(&nlist, 1);
-                if (i < RANK_LIST)
+                int col[RU];
+#pragma unroll
+                for (int u = 0; u < RU; u++)
                 {
-                    lkey[i] = ((unsigned long long) (unsigned) pos << 32) | (unsigned) q;
-                    lval[i] = x;
+                    const int fi = f0 + u * G;
+                    int t = 0, q, k;
+                    if (fi < tot) expand_term<G>(ex, nq, fi, t, q, k);
+                    col[u] = fi < tot ? Ai[t] : -1;
+                }
+#pragma unroll
+                for (int u = 0; u < RU; u++)
+                {
+                    if (col[u] < 0) continue;
+                    const int pos = rank_of(bm, sub, sup, col[u]);
+                    const unsigned bit = 1u << (pos & 31);
+                    if (atomicOr(&seen1[pos >> 5], bit) & bit) atomicOr(&seen2[pos >> 5], bit);
                 }
             }
-        });
+            __syncthreads();
+        }
+        // pass 2: single-term entries written, the others' terms listed
+        for (int qb = c.q0; qb < c.q1; qb += G)
+        {
+            const int tot = expand_chunk<G>(ex, qb, c.q1, c.koff, Li, Ap);
+            const int nq = min(G, c.q1 - qb);
+            for (int f0 = threadIdx.x; f0 < tot; f0 += RU * G)
+            {
+                int col[RU], qq[RU];
+                double x[RU];
+#pragma unroll
+                for (int u = 0; u < RU; u++)
+                {
+                    const int fi = f0 + u * G;
+                    int t = 0, q = 0, k = 0;
+                    if (fi < tot) expand_term<G>(ex, nq, fi, t, q, k);
+                    col[u] = -1;
+                    qq[u] = q;
+                    x[u] = 0.0;
+                    if (fi < tot)
+                    {
+                        col[u] = Ai[t];
+                        double v = Lval[Lperm ? Lperm[q] : q];
+                        if (d) v = v * d[k];
+                        x[u] = v * Ax[t];
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < RU; u++)
+                {
+                    if (col[u] < 0) continue;
+                    const int pos = rank_of(bm, sub, sup, col[u]);
+                    if (!((seen2[pos >> 5] >> (pos & 31)) & 1u))
+                    {
+                        Cx[cbase + pos] = 0.0 + x[u];
+                    }
+                    else
+                    {
+                        const int i = atomicAdd(&nlist, 1);
+                        if (i < RANK_LIST)
+                        {
+                            lkey[i] = ((unsigned long long) (unsigned) pos << 32) | (unsigned) qq[u];
+                            lval[i] = x[u];
+                        }
+                    }
+                }
+            }
+            __syncthreads();
+        }
         const int nl = nlist;
         if (nl > RANK_LIST)
         {
