@@ -142,35 +142,65 @@ __global__ void __launch_bounds__(256)
     loss[r] = lse - zy;
 }
 
-// One thread per parameter; sums over rows in ascending order.
+// Gradient reduction in two deterministic stages.  Stage 1: one workgroup per
+// chunk of MLP_RCH rows (staged in LDS); each parameter's chunk partial sums
+// the chunk's rows in ascending order.  Stage 2: one thread per parameter sums
+// the chunk partials in ascending chunk order.
+constexpr int MLP_RCH = 64;
+
 __global__ void __launch_bounds__(256)
-    k_mlp_reduce(int64_t n, const float *__restrict__ X, const float *__restrict__ H,
-                 const float *__restrict__ DZ, const float *__restrict__ DP,
-                 float *__restrict__ G)
+    k_mlp_reduce_chunks(int64_t n, const float *__restrict__ X, const float *__restrict__ H,
+                        const float *__restrict__ DZ, const float *__restrict__ DP,
+                        float *__restrict__ part)
+{
+    __shared__ float sx[MLP_RCH][MLP_IN], sh[MLP_RCH][MLP_HID], sz[MLP_RCH][MLP_OUT],
+        sp[MLP_RCH][MLP_HID];
+    const int64_t r0 = (int64_t) blockIdx.x * MLP_RCH;
+    const int nr = (int) min((int64_t) MLP_RCH, n - r0);
+    for (int i = threadIdx.x; i < nr * MLP_IN; i += blockDim.x)
+        sx[i / MLP_IN][i % MLP_IN] = X[r0 * MLP_IN + i];
+    for (int i = threadIdx.x; i < nr * MLP_HID; i += blockDim.x)
+    {
+        sh[i / MLP_HID][i % MLP_HID] = H[r0 * MLP_HID + i];
+        sp[i / MLP_HID][i % MLP_HID] = DP[r0 * MLP_HID + i];
+    }
+    for (int i = threadIdx.x; i < nr * MLP_OUT; i += blockDim.x)
+        sz[i / MLP_OUT][i % MLP_OUT] = DZ[r0 * MLP_OUT + i];
+    __syncthreads();
+    for (int p = threadIdx.x; p < MLP_NPARAM; p += blockDim.x)
+    {
+        float acc = 0.0f;
+        if (p < MLP_OFF_B1)
+        {
+            const int j = p / MLP_IN, i = p % MLP_IN;
+            for (int r = 0; r < nr; r++) acc = acc + sp[r][j] * sx[r][i];
+        }
+        else if (p < MLP_OFF_W2)
+        {
+            const int j = p - MLP_OFF_B1;
+            for (int r = 0; r < nr; r++) acc = acc + sp[r][j];
+        }
+        else if (p < MLP_OFF_B2)
+        {
+            const int c = (p - MLP_OFF_W2) / MLP_HID, j = (p - MLP_OFF_W2) % MLP_HID;
+            for (int r = 0; r < nr; r++) acc = acc + sz[r][c] * sh[r][j];
+        }
+        else
+        {
+            const int c = p - MLP_OFF_B2;
+            for (int r = 0; r < nr; r++) acc = acc + sz[r][c];
+        }
+        part[(int64_t) blockIdx.x * MLP_NPARAM + p] = acc;
+    }
+}
+
+__global__ void __launch_bounds__(256)
+    k_mlp_reduce(int nchunks, const float *__restrict__ part, float *__restrict__ G)
 {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= MLP_NPARAM) return;
     float acc = 0.0f;
-    if (p < MLP_OFF_B1)
-    {
-        const int j = p / MLP_IN, i = p % MLP_IN;
-        for (int64_t r = 0; r < n; r++) acc = acc + DP[r * MLP_HID + j] * X[r * MLP_IN + i];
-    }
-    else if (p < MLP_OFF_W2)
-    {
-        const int j = p - MLP_OFF_B1;
-        for (int64_t r = 0; r < n; r++) acc = acc + DP[r * MLP_HID + j];
-    }
-    else if (p < MLP_OFF_B2)
-    {
-        const int c = (p - MLP_OFF_W2) / MLP_HID, j = (p - MLP_OFF_W2) % MLP_HID;
-        for (int64_t r = 0; r < n; r++) acc = acc + DZ[r * MLP_OUT + c] * H[r * MLP_HID + j];
-    }
-    else
-    {
-        const int c = p - MLP_OFF_B2;
-        for (int64_t r = 0; r < n; r++) acc = acc + DZ[r * MLP_OUT + c];
-    }
+    for (int c = 0; c < nchunks; c++) acc = acc + part[(int64_t) c * MLP_NPARAM + p];
     G[p] = acc;
 }
 
@@ -245,6 +275,7 @@ void MlpScratch::reserve(int64_t n)
     DZ.alloc((size_t) n * MLP_OUT);
     DP.alloc((size_t) n * MLP_HID);
     loss.alloc((size_t) n + 1);
+    part.alloc((size_t) ((n + MLP_RCH - 1) / MLP_RCH) * MLP_NPARAM);
     cap = n;
 }
 
@@ -282,19 +313,31 @@ void mlp_predict(const RowPredictor &R, const float *X, int64_t n, unsigned char
     SDL_HIP(hipGetLastError());
 }
 
+// The launches of mlp_gradients; the mean loss goes to loss_out (device).
+void mlp_gradients_launch(const RowPredictor &R, const float *X, const int *Y, int64_t n,
+                          float *G, MlpScratch &S, hipStream_t s, float *loss_out)
+{
+    S.reserve(n);
+    hipLaunchKernelGGL(k_mlp_rows, dim3(grid_of(n)), dim3(256), 0, s, n, X, Y, R.params.p, S.H.p,
+                       S.DZ.p, S.DP.p, S.loss.p);
+    SDL_HIP(hipGetLastError());
+    const int nchunks = (int) ((n + MLP_RCH - 1) / MLP_RCH);
+    hipLaunchKernelGGL(k_mlp_reduce_chunks, dim3(nchunks), dim3(256), 0, s, n, X, S.H.p, S.DZ.p,
+                       S.DP.p, S.part.p);
+    SDL_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_mlp_reduce, dim3(grid_of(MLP_NPARAM)), dim3(256), 0, s, nchunks,
+                       S.part.p, G);
+    SDL_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_mean, dim3(1), dim3(256), 0, s, n, S.loss.p, loss_out);
+    SDL_HIP(hipGetLastError());
+}
+
 float mlp_gradients(const RowPredictor &R, const float *X, const int *Y, int64_t n, float *G,
                     MlpScratch &S, hipStream_t s)
 {
     SDL_REQUIRE(n > 0, "mlp_gradients: empty batch");
     S.reserve(n);
-    hipLaunchKernelGGL(k_mlp_rows, dim3(grid_of(n)), dim3(256), 0, s, n, X, Y, R.params.p, S.H.p,
-                       S.DZ.p, S.DP.p, S.loss.p);
-    SDL_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_mlp_reduce, dim3(grid_of(MLP_NPARAM)), dim3(256), 0, s, n, X, S.H.p,
-                       S.DZ.p, S.DP.p, G);
-    SDL_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_mean, dim3(1), dim3(256), 0, s, n, S.loss.p, S.loss.p + n);
-    SDL_HIP(hipGetLastError());
+    mlp_gradients_launch(R, X, Y, n, G, S, s, S.loss.p + n);
     float loss = 0.0f;
     SDL_HIP(hipMemcpyAsync(&loss, S.loss.p + n, sizeof(float), hipMemcpyDeviceToHost, s));
     SDL_HIP(hipStreamSynchronize(s));
@@ -328,6 +371,9 @@ float mlp_train(RowPredictor &R, MlpScratch &S, const float *X, const int *Y, in
     DBuf<float> Xb((size_t) B * MLP_IN);
     DBuf<int> Yb(B);
     std::vector<int> perm(n);
+    const int64_t nb = n / B;  // full batches per epoch
+    DBuf<float> bloss(nb);
+    std::vector<float> hloss(nb);
     uint64_t st = seed;
     float last = 0.0f;
     for (int e = 0; e < epochs; e++)
@@ -338,19 +384,24 @@ float mlp_train(RowPredictor &R, MlpScratch &S, const float *X, const int *Y, in
             const int64_t j = (int64_t) (splitmix(st) % (uint64_t) (i + 1));
             std::swap(perm[i], perm[j]);
         }
-        h2d(dperm.p, perm.data(), (size_t) n * sizeof(int));
-        double sum = 0.0;
-        int nbatch = 0;
-        for (int64_t b0 = 0; b0 + B <= n; b0 += B)  // full batches only
+        SDL_HIP(hipMemcpyAsync(dperm.p, perm.data(), (size_t) n * sizeof(int),
+                               hipMemcpyHostToDevice, s));
+        for (int64_t b = 0; b < nb; b++)  // full batches only; no host sync inside the epoch
         {
-            hipLaunchKernelGGL(k_gather_batch, dim3(grid_of(B)), dim3(256), 0, s, B, dperm.p + b0,
-                               X, Y, Xb.p, Yb.p);
+            hipLaunchKernelGGL(k_gather_batch, dim3(grid_of(B)), dim3(256), 0, s, B,
+                               dperm.p + b * B, X, Y, Xb.p, Yb.p);
             SDL_HIP(hipGetLastError());
-            sum += mlp_gradients(R, Xb.p, Yb.p, B, R.grad.p, S, s);
+            mlp_gradients_launch(R, Xb.p, Yb.p, B, R.grad.p, S, s, bloss.p + b);
             mlp_adam(R, R.grad.p, lr, s);
-            nbatch++;
         }
-        last = (float) (sum / std::max(nbatch, 1));
+        // the batch losses in order (the copy also keeps perm alive until the
+        // epoch's work is done)
+        SDL_HIP(hipMemcpyAsync(hloss.data(), bloss.p, (size_t) nb * sizeof(float),
+                               hipMemcpyDeviceToHost, s));
+        SDL_HIP(hipStreamSynchronize(s));
+        double sum = 0.0;
+        for (int64_t b = 0; b < nb; b++) sum += hloss[b];
+        last = (float) (sum / std::max<int64_t>(nb, 1));
     }
     SDL_HIP(hipStreamSynchronize(s));
     return last;

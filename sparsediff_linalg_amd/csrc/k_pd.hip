@@ -18,14 +18,27 @@ typedef double d4 __attribute__((ext_vector_type(4)));
 
 unsigned grid_for(int64_t n, int b = 256) { return (unsigned) std::max<int64_t>(1, (n + b - 1) / b); }
 
-__global__ void k_pull_sum(int n, const int *__restrict__ seg, const int *__restrict__ idx,
-                           const double *__restrict__ src, double *__restrict__ out)
+// pos null: the listed outputs are all outputs, in order
+__global__ void k_pull_sum(int n, const int *__restrict__ pos, const int *__restrict__ seg,
+                           const int *__restrict__ idx, const double *__restrict__ src,
+                           double *__restrict__ out)
 {
     int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= n) return;
     double acc = 0.0;
     for (int s = seg[e]; s < seg[e + 1]; s++) acc = acc + src[idx[s]];
-    out[e] = acc;
+    out[pos ? pos[e] : e] = acc;
+}
+
+// One workgroup per chunk of single-source outputs: out = 0 + src (the sum of
+// one term from +0, so a -0 source gives +0 as in the ordered sum).
+__global__ void __launch_bounds__(256)
+    k_pull_copy(const int *__restrict__ co, const int *__restrict__ cs, const int *__restrict__ cl,
+                const double *__restrict__ src, double *__restrict__ out)
+{
+    const int c = blockIdx.x;
+    const int o = co[c], sidx = cs[c], len = cl[c];
+    for (int i = threadIdx.x; i < len; i += 256) out[o + i] = 0.0 + src[sidx + i];
 }
 
 // one wavefront per output list, lanes over the dense dimension
@@ -236,16 +249,64 @@ int g_gemm_slice = 16;  // reduction slice depth of k_ggemm (16 or 32)
 
 void pull_build(PullList &P, const std::vector<int> &seg, const std::vector<int> &idx)
 {
-    P.n = (int) seg.size() - 1;
-    P.seg = upload(seg.data(), (int64_t) seg.size());
-    P.idx = upload(idx.data(), (int64_t) idx.size());
+    constexpr int MIN_RUN = 16, CHUNK = 4096;
+    const int n = (int) seg.size() - 1;
+    P.n = n;
+    std::vector<int> co, cs, cl, pos, rseg{0}, ridx;
+    auto single = [&](int e) { return seg[e + 1] - seg[e] == 1; };
+    auto rest = [&](int e) {
+        pos.push_back(e);
+        for (int s = seg[e]; s < seg[e + 1]; s++) ridx.push_back(idx[s]);
+        rseg.push_back((int) ridx.size());
+    };
+    int e = 0;
+    while (e < n)
+    {
+        int f = e;
+        if (single(e))
+            while (f + 1 < n && single(f + 1) && idx[seg[f + 1]] == idx[seg[f]] + 1) f++;
+        const int len = f - e + 1;
+        if (single(e) && len >= MIN_RUN)
+        {
+            for (int o = 0; o < len; o += CHUNK)
+            {
+                co.push_back(e + o);
+                cs.push_back(idx[seg[e]] + o);
+                cl.push_back(std::min(CHUNK, len - o));
+            }
+        }
+        else
+        {
+            for (int g = e; g <= f; g++) rest(g);
+        }
+        e = f + 1;
+    }
+    P.nchunks = (int) co.size();
+    if (P.nchunks)
+    {
+        P.co = upload(co.data(), (int64_t) co.size());
+        P.cs = upload(cs.data(), (int64_t) cs.size());
+        P.cl = upload(cl.data(), (int64_t) cl.size());
+    }
+    P.nrest = (int) pos.size();
+    if (P.nrest)
+    {
+        // every output listed in order: no position array
+        if (P.nrest < n) P.pos = upload(pos.data(), (int64_t) pos.size());
+        P.seg = upload(rseg.data(), (int64_t) rseg.size());
+        P.idx = upload(ridx.data(), (int64_t) std::max<size_t>(ridx.size(), 1));
+    }
 }
 
 void pull_sum(const PullList &P, const double *src, double *out, hipStream_t s)
 {
     if (P.n <= 0) return;
-    hipLaunchKernelGGL(k_pull_sum, dim3(grid_for(P.n)), dim3(256), 0, s, P.n, P.seg.p, P.idx.p, src,
-                       out);
+    if (P.nchunks)
+        hipLaunchKernelGGL(k_pull_copy, dim3(P.nchunks), dim3(256), 0, s, P.co.p, P.cs.p, P.cl.p,
+                           src, out);
+    if (P.nrest)
+        hipLaunchKernelGGL(k_pull_sum, dim3(grid_for(P.nrest)), dim3(256), 0, s, P.nrest, P.pos.p,
+                           P.seg.p, P.idx.p, src, out);
     SDL_HIP(hipGetLastError());
 }
 

@@ -50,6 +50,7 @@ struct RowPredictor
 struct MlpScratch
 {
     DBuf<float> H, DZ, DP, loss;   // n*32, n*6, n*32, n
+    DBuf<float> part;              // per-chunk gradient partials: ceil(n/64) * 742
     int64_t cap = 0;
     void reserve(int64_t n);
 };

@@ -13,10 +13,18 @@
 namespace sdl {
 
 // ---------------------------------------------------------------- pull ----
+// out[e] = (0 + src[idx[seg[e]]]) + src[idx[seg[e] + 1]] + ... for n outputs.
+// Outputs with exactly one source, in runs of consecutive outputs reading
+// consecutive sources, are copied by chunks (out + co[c] <- 0 + src + cs[c],
+// cl[c] cells); the rest are listed with their output positions.
 struct PullList
 {
-    int n = 0;           // outputs
-    DBuf<int> seg, idx;  // seg[n+1]; idx: source positions in order
+    int n = 0;                   // outputs
+    int nchunks = 0;
+    DBuf<int> co, cs, cl;        // chunk output start, source start, length
+    int nrest = 0;
+    DBuf<int> pos, seg, idx;     // rest: pos[nrest] (absent when the rest is every
+                                 // output in order), seg[nrest+1], idx (sources in order)
 };
 void pull_build(PullList &P, const std::vector<int> &seg, const std::vector<int> &idx);
 void pull_sum(const PullList &P, const double *src, double *out, hipStream_t s);
