@@ -335,10 +335,31 @@ def main_ata_large(args):
                      "alg_bytes_per_launch": bytes_},
         "gflops": (nnzA + 2 * terms) / t / 1e9,
         "symbolic_terms_per_s": terms / t_sym,
-        "cpu_baseline": None,
+        "cpu_baseline": None if args.no_cpu else ata_cpu_baseline(m, n, Ap, Ai, Ax, d_last,
+                                                                  args.cpu_seconds),
     }
     pl.close()
     print(json.dumps(out), flush=True)
+
+
+def ata_cpu_baseline(m, n, Ap, Ai, Ax, d, seconds):
+    """The oracle's single-thread row-at-a-time Gustavson (structure and values
+    of each row: mark-array count, sort, accumulator fill) over random output
+    rows until about `seconds` of CPU time; nnz-out/s of those rows."""
+    from oracle import oracle as O
+    Lp, Li, Lx = O.transpose(m, n, Ap, Ai, Ax)
+    rng = np.random.default_rng(11)
+    done = nnz = 0
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < seconds:
+        rows = rng.choice(n, 200, replace=False)
+        for cols, _ in O.spgemm_rows(rows, n, Lp, Li, Lx, 1, d, m, n, Ap, Ai, Ax):
+            nnz += len(cols)
+        done += len(rows)
+    t = time.perf_counter() - t0
+    return {"value": nnz / t, "unit": "nnz-out/s", "cores": 1, "kind": "port",
+            "sample": f"{done} random output rows of the same product ({nnz} entries), "
+                      f"{t:.1f} s on one core (oracle C, row at a time: symbolic + numeric)"}
 
 
 def main_coo(args):
@@ -402,7 +423,7 @@ def main_coo(args):
                      "traffic": None, "alg_bytes_per_launch": bytes_},
         "plan_triplets_per_s": nin / t_plan,
         "alg_bytes_whole_coalesce": bytes_all,
-        "cpu_baseline": None,
+        "cpu_baseline": None if args.no_cpu else coo_cpu_baseline(rows, cols, x, m, n),
     }
     lib.sdl_coo_plan_destroy(h)
     print(json.dumps(out_json), flush=True)
@@ -508,6 +529,19 @@ def main_learned(args):
     for mlp, _ in nets.values():
         mlp.close()
     print(json.dumps(out), flush=True)
+
+
+def coo_cpu_baseline(rows, cols, x, m, n, count=5_000_000):
+    """The oracle's single-thread coalesce (counting sort by row in input
+    order, per-row column sort, ordered sums: structure and values) of the
+    first `count` triplets."""
+    from oracle import oracle as O
+    t0 = time.perf_counter()
+    O.coo_coalesce(rows[:count], cols[:count], x[:count], m, n)
+    t = time.perf_counter() - t0
+    return {"value": count / t, "unit": "triplets/s", "cores": 1, "kind": "port",
+            "sample": f"first {count} triplets of the same stream, coalesced whole (structure "
+                      f"and values) in {t:.2f} s on one core (oracle C)"}
 
 
 def main_multi(args):
