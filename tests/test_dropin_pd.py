@@ -406,3 +406,43 @@ def test_config3_full_size(case):
     for M in (Cm, Ah, Bh):
         if M is not None:
             L.free_matrix(M)
+
+
+def test_spd_disjoint_columns_single_source_outputs():
+    """A^T D A of a stack whose blocks have disjoint col_perms: every output
+    cell has one source, so the ordered coalesce copies runs of consecutive
+    cells (0 + x, as the ordered sum).  Structure exactly the oracle's; every
+    value against the dense product within 1e-13 * sum |terms|."""
+    L = D.lib()
+    rng = np.random.default_rng(12)
+    nb = 24
+    n0s = rng.integers(64, 257, nb)
+    m0s = rng.integers(32, 129, nb)
+    N = int(n0s.sum()) + 500
+    m = int(m0s.sum())
+    cols = rng.permutation(N)
+    rows = rng.permutation(m)
+    blocks, c0, r0 = [], 0, 0
+    for k in range(nb):
+        cp = sorted(cols[c0:c0 + n0s[k]].tolist())
+        rp = sorted(rows[r0:r0 + m0s[k]].tolist())
+        c0 += int(n0s[k])
+        r0 += int(m0s[k])
+        blocks.append(PO.pd(m, N, rp, cp, rng.standard_normal((len(rp), len(cp)))))
+    Ao = PO.spd(m, N, blocks)
+    d = rng.uniform(0.1, 10.0, m)
+    Ah = make(Ao)
+    Cm = L.ATA_matrices_alloc(Ah)
+    L.ATDA_matrices_fill_values(Ah, D.dptr(d), Cm)
+    p, i, x = D.mat_csr(Cm)
+    ep, ei = PO.csr_pattern(PO.ata(Ao))
+    assert np.array_equal(p, np.asarray(ep)) and np.array_equal(i, np.asarray(ei))
+    Ad = np.zeros((m, N))
+    for b in blocks:
+        Ad[np.ix_(b["rp"], b["cp"])] = b["X"]
+    ref = Ad.T @ (d[:, None] * Ad)
+    mag = np.abs(Ad).T @ (d[:, None] * np.abs(Ad))
+    r = np.repeat(np.arange(N), np.diff(p))
+    assert np.all(np.abs(x - ref[r, i]) <= 1e-13 * mag[r, i] + 1e-300)
+    for M in (Cm, Ah):
+        L.free_matrix(M)
