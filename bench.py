@@ -62,6 +62,8 @@ def parse():
     ap.add_argument("--rmat-scale", type=int, default=18,
                     help="learned: R-MAT scale of the A^T A evaluated beside the banded one")
     ap.add_argument("--epochs", type=int, default=100, help="learned: training epochs")
+    ap.add_argument("--taus", type=float, nargs="*", default=[0.99, 0.999],
+                    help="learned: also evaluate the structured network at these tau values")
     ap.add_argument("--gemm-slice", type=int, default=0,
                     help="blocks-*: reduction slice depth of the block GEMM (16 or 32; 0 = "
                          "library default)")
@@ -492,9 +494,15 @@ def main_learned(args):
         rec = {"product": f"A^T A, {label}", "nnz_A": int(p[-1]), "terms": terms,
                "nnz_out": ex.nnz(), "exact_symbolic_s": t_ex, "networks": {}}
         ex.close()
-        runs = [(k, v[0]) for k, v in nets.items()]
-        for name, mlp in runs:
+        runs = [(k, v[0], None) for k, v in nets.items()]
+        runs += [(f"with_structured_tau{tau}", nets["with_structured"][0], tau)
+                 for tau in args.taus]
+        for name, mlp, tau in runs:
+            if tau is not None:
+                mlp.set_tau(tau)
             pr, t_pr = timed(p, i, n, mode=1, mlp=mlp)
+            if tau is not None:
+                mlp.set_tau(0.95)
             got = structure(pr, n)
             same = bool(np.array_equal(got[0], ref[0]) and np.array_equal(got[1], ref[1]))
             assert same, (label, name)

@@ -197,10 +197,11 @@ int sdl_mlp_load(sdl_mlp *mlp, const char *path);
    sdl_btda_plan_create(..., SDL_SYM_PREDICTED, ...) (NULL: exact passes for
    the allocs; term-bound sizes for SDL_SYM_PREDICTED, i.e. the exact
    path).  Predicted symbolic phase: the count pass sizes each row's
-   accumulator from the row's predicted nnz bin (never above its term
-   bound), rows whose count exceeds the bin are counted again from the term
-   bound, and the fill pass sizes from the exact counts, so the structure is
-   always exact. */
+   accumulator from the row's predicted nnz bin where the term bound would
+   need a large column bitmap (at least 4x the predicted hash table, or one
+   in global memory), rows whose count exceeds the bin are counted again from
+   the term bound, and the fill pass sizes from the exact counts, so the
+   structure is always exact. */
 int sdl_set_row_predictor(const sdl_mlp *mlp);
 /* Plan with the predicted symbolic phase (mlp NULL: term-bound sizes, the
    exact path). */
@@ -213,8 +214,8 @@ int sdl_btda_plan_features(const sdl_btda_plan *plan, float *dF);
 int sdl_btda_plan_row_bins(const sdl_btda_plan *plan, int *dY);
 /* Predicted plans, over non-empty rows: counted within the predicted bound,
    counted again from the term bound (the bound was exceeded), predicted bin
-   different from the bin of the exact nnz, predicted bound not below the
-   term bound. */
+   different from the bin of the exact nnz, bound not lowered (term bound
+   kept). */
 int sdl_btda_plan_pred_stats(const sdl_btda_plan *plan, long long *out4);
 
 /* ---- device refills of permuted_dense / stacked_pd product outputs ---- */

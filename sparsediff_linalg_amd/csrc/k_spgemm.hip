@@ -508,17 +508,30 @@ __global__ void k_row_features(int64_t m_out, int Lm, const int *__restrict__ Lp
 }
 
 // Count-pass size bound of every row: the term bound, lowered to the largest
-// nnz of the row's predicted bin when that bin is bounded.  The count pass
-// sizes each row's accumulator from this bound, so a row predicted small gets
-// a small LDS hash table in place of the term bound's table or column bitmap.
+// nnz of the row's predicted bin when that bin is bounded and the term bound
+// would need a column bitmap at least four times the predicted hash table (in
+// LDS) or one in global memory.  The count pass sizes each row's accumulator
+// from this bound, so such a row gets a small LDS hash table; rows whose
+// term-bound accumulator is already small keep it (a wrong prediction there
+// would cost a second count for little gain).
 __global__ void k_pred_cap(int64_t m, const int64_t *__restrict__ terms,
-                           const unsigned char *__restrict__ pbin, int64_t *__restrict__ cap)
+                           const unsigned char *__restrict__ pbin, int nwords, int lds_words,
+                           int64_t *__restrict__ cap)
 {
     int64_t row = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
     if (row >= m) return;
     const int64_t t = terms[row];
     const int b = pbin[row];
-    cap[row] = b < MLP_NBIN_BOUNDED ? min(t, (int64_t) mlp_bin_max(b)) : t;
+    int64_t c = t;
+    if (b < MLP_NBIN_BOUNDED && t > mlp_bin_max(b))
+    {
+        const int ct = classify(t, nwords, lds_words);
+        int64_t slots = 64;
+        while (slots < 2 * (int64_t) mlp_bin_max(b)) slots <<= 1;
+        if (ct == CLS_GLOBAL || (ct == CLS_BITMAP && (int64_t) nwords >= 4 * slots))
+            c = mlp_bin_max(b);
+    }
+    cap[row] = c;
 }
 
 // Rows to count again from the term bound: those whose count exceeded their
@@ -1557,7 +1570,7 @@ void symbolic_predicted(ProductPlan &P, const RowPredictor *pred, const int64_t 
     }
     DBuf<int64_t> cap(m_out), recount(m_out);
     hipLaunchKernelGGL(k_pred_cap, dim3(grid), dim3(256), 0, stream(), m_out, rterms, pbin.p,
-                       cap.p);
+                       nwords, lds_bitmap_words(), cap.p);
     SDL_HIP(hipGetLastError());
     symbolic_count(P, cap.p, rowcnt, nwords);
     hipLaunchKernelGGL(k_pred_recount, dim3(grid), dim3(256), 0, stream(), m_out, rowcnt, cap.p,

@@ -43,7 +43,7 @@ struct SymStats
     long long overflow_rows = 0;       // predicted path: count exceeded the predicted bound
     long long predicted_rows = 0;      // predicted path: counted within the predicted bound
     long long misbin_rows = 0;         // predicted path: predicted bin != bin of the exact nnz
-    long long exact_rows = 0;          // predicted path: predicted bound not below the term bound
+    long long exact_rows = 0;          // predicted path: term bound kept (not lowered)
 };
 
 // Launch configuration of the batched pull refill (picked by the autotuner):
@@ -106,8 +106,9 @@ struct RowPredictor;
 // Build the exact structure and the refill schedule (pull iff terms <= 3 nnz,
 // the reference's plan policy; force_pull: whenever the counts fit int32 --
 // both schedules give the same values bit for bit).  SymMode::Predicted: the
-// count pass sizes each row's accumulator from its predicted bin (never above
-// the term bound; pred may be null: the term bound, i.e. the exact path), rows
+// count pass sizes each row's accumulator from its predicted bin where the term
+// bound would need a large column bitmap (pred may be null: the term bound,
+// i.e. the exact path), rows
 // whose count exceeds the predicted bound are counted again from the term
 // bound, and the fill pass is the exact path's.
 void product_plan(ProductPlan &P, const LView &L, const AView &A, SymMode mode,

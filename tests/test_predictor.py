@@ -114,6 +114,9 @@ def cases():
     wb = W.blocks_config3(5, nblocks=12, lo=16, hi=64, ncols=512)
     p, i, _ = W.blocks_as_csr(wb)
     yield "blocks", (p, i), None, 512, 512
+    # wide: the term bound asks for 128 KB column bitmaps, so predictions apply
+    p, i, _ = W.strided_band_csr(1 << 12, 1 << 20, 129, 4)
+    yield "strided", (p, i), None, 1 << 20, 1 << 20
 
 
 @pytest.mark.parametrize("which", ["trained", "none", "adversarial"])
