@@ -452,19 +452,23 @@ __global__ void __launch_bounds__(G)
 // ---------------------------------------------------- learned binning ----
 __device__ __forceinline__ float lg(double x) { return (float) log2(1.0 + x); }
 
-// Structural features of every output row (one thread per row, one walk over
-// the row's L entries reading A's row bounds and first / last columns).
-__global__ void k_row_features(int64_t m_out, int Lm, const int *__restrict__ Lp,
-                               const int *__restrict__ Li, int nL, const int *__restrict__ Ap,
-                               const int *__restrict__ Ai, int An, float *__restrict__ F)
+// Structural features of every output row: one wavefront per row, the lanes
+// striding over the row's L entries (reading A's row bounds and first / last
+// columns), then wavefront reductions.  The sums are of integers below 2^53,
+// so they are exact in any order.
+__global__ void __launch_bounds__(256)
+    k_row_features(int64_t m_out, int Lm, const int *__restrict__ Lp,
+                   const int *__restrict__ Li, int nL, const int *__restrict__ Ap,
+                   const int *__restrict__ Ai, int An, float *__restrict__ F)
 {
-    int64_t row = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t row = (int64_t) blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
     if (row >= m_out) return;
     RowCtx c = row_ctx((int) row, Lm, Lp, nL);
     const int deg = c.q1 - c.q0;
     double terms = 0, maxlen = 0, minlen = 1e300, sumsq = 0, sspan = 0;
     long long cmin = LLONG_MAX, cmax = -1, kmin = LLONG_MAX, kmax = -1;
-    for (int q = c.q0; q < c.q1; q++)
+    for (int q = c.q0 + lane; q < c.q1; q += 64)
     {
         const int k = Li[q] + c.koff;
         const int st = Ap[k], en = Ap[k + 1];
@@ -483,6 +487,20 @@ __global__ void k_row_features(int64_t m_out, int Lm, const int *__restrict__ Lp
             sspan += (double) (l - f + 1);
         }
     }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+    {
+        terms += __shfl_xor(terms, off, 64);
+        sumsq += __shfl_xor(sumsq, off, 64);
+        sspan += __shfl_xor(sspan, off, 64);
+        maxlen = fmax(maxlen, __shfl_xor(maxlen, off, 64));
+        minlen = fmin(minlen, __shfl_xor(minlen, off, 64));
+        cmin = min(cmin, __shfl_xor(cmin, off, 64));
+        cmax = max(cmax, __shfl_xor(cmax, off, 64));
+        kmin = min(kmin, __shfl_xor(kmin, off, 64));
+        kmax = max(kmax, __shfl_xor(kmax, off, 64));
+    }
+    if (lane != 0) return;
     if (terms == 0) minlen = 0;
     const double n = An;
     const double span = terms > 0 ? (double) (cmax - cmin + 1) : 0.0;
@@ -1615,7 +1633,7 @@ void product_row_features(const LView &L, const AView &A, float *F, hipStream_t 
 {
     const int64_t m_out = (int64_t) L.Lm * L.blocks;
     if (m_out == 0) return;
-    hipLaunchKernelGGL(k_row_features, dim3((unsigned) ((m_out + 255) / 256)), dim3(256), 0, s,
+    hipLaunchKernelGGL(k_row_features, dim3((unsigned) ((m_out + 3) / 4)), dim3(256), 0, s,
                        m_out, L.Lm, L.Lp, L.Li, A.Am / L.blocks, A.Ap, A.Ai, A.An, F);
     SDL_HIP(hipGetLastError());
 }
