@@ -1223,7 +1223,7 @@ __global__ void __launch_bounds__(G)
 constexpr int RANK_MAX_WORDS = 16384;      // 524,288 columns
 constexpr int RANK_MAX_ENTRIES = 65536;
 constexpr int RANK_LIST = 1024;
-constexpr int RU = 4;                      // terms per thread per step of the rank passes
+constexpr int RU = 16;                     // terms per thread per step of the rank passes
 
 __device__ __forceinline__ int rank_of(const unsigned *bm, const unsigned short *sub,
                                        const unsigned *sup, int col)
