@@ -112,7 +112,10 @@ int sdl_btda_refill_batched_ex(const sdl_btda_plan *plan, const double *dAx, con
 
 /* Launch configuration of the batched refill: variant 0 = one thread per
    entry over all K; 1 / 2 = grid over (entry tile, chunk of rchunk refills)
-   with 1 / 2 entries per thread; 3 = interleaved (needs the reservation). */
+   with 1 / 2 entries per thread; 3 = interleaved (needs the reservation),
+   4 / 5 = interleaved with 2 / 4 entries per thread, 6 = interleaved reading
+   the operand values through the term lists (unbound values only; bound
+   calls use 3). */
 typedef struct sdl_refill_cfg
 {
     int variant;

@@ -1018,6 +1018,57 @@ __global__ void __launch_bounds__(256)
     }
 }
 
+// The interleaved refill reading each term's operands through tA / tB (the
+// value arrays stay in L2) instead of a pre-gathered pair array: no gather
+// pass per batch, 8 instead of 16 bytes of term data per term.
+template <int RC>
+__global__ void __launch_bounds__(256)
+    k_num_pull_interleaved_g(int64_t nnz, int ntiles, int nchunks, const int64_t *__restrict__ ep,
+                             const int *__restrict__ tA, const int *__restrict__ tB,
+                             const int *__restrict__ tK, const double *__restrict__ Lval,
+                             const double *__restrict__ Ax, const double *__restrict__ DT,
+                             int ldt, int K, double *__restrict__ C, int64_t ldc)
+{
+    const int b = blockIdx.x;
+    const int xcd = b & 7, sl = b >> 3;
+    const int tile = (sl / nchunks) * 8 + xcd;
+    const int chunk = sl % nchunks;
+    if (tile >= ntiles) return;
+    const int64_t e = (int64_t) tile * 256 + threadIdx.x;
+    if (e >= nnz) return;
+    const int r0 = chunk * RC;
+    const int64_t s0 = ep[e], s1 = ep[e + 1];
+    double out[RC];
+#pragma unroll
+    for (int r = 0; r < RC; r++) out[r] = 0.0;
+    for (int64_t s = s0; s < s1; s++)
+    {
+        const int k = tK[s];
+        const double bx = Lval[tB[s]], ay = Ax[tA[s]];
+        const dvec2 *dk = (const dvec2 *) (DT + (int64_t) k * ldt + r0);
+        dvec2 dv[RC / 2];
+#pragma unroll
+        for (int h = 0; h < RC / 2; h++) dv[h] = dk[h];
+#pragma unroll
+        for (int h = 0; h < RC / 2; h++)
+        {
+            out[2 * h] = out[2 * h] + (bx * dv[h].x) * ay;
+            out[2 * h + 1] = out[2 * h + 1] + (bx * dv[h].y) * ay;
+        }
+    }
+    const int rn = min(RC, K - r0);
+    double *cp = C + (int64_t) r0 * ldc + e;
+    if (rn == RC)
+    {
+#pragma unroll
+        for (int r = 0; r < RC; r++) __builtin_nontemporal_store(out[r], cp + (int64_t) r * ldc);
+    }
+    else
+    {
+        for (int r = 0; r < rn; r++) __builtin_nontemporal_store(out[r], cp + (int64_t) r * ldc);
+    }
+}
+
 // One refill over the pre-gathered operand pairs: C[e] = sum over the entry's
 // terms (ascending k) of (L value * d[k]) * A value -- the fold of every
 // other refill kernel.  One 256-entry tile per workgroup (with a single
@@ -1857,11 +1908,14 @@ void product_numeric_batched(const ProductPlan &P, const double *Lval, const dou
         if (ev_stop) SDL_HIP(hipEventRecord((hipEvent_t) ev_stop, s));
         return;
     }
+    // direct operand gathers need the caller's value arrays: bound values use
+    // the pair array
+    if (c.variant == 6 && (opts & BATCH_VALUES_BOUND)) c.variant = 3;
     if (c.variant >= 3)
     {
         const int rc = c.rchunk == 8 ? 8 : 16;
         const int ldt = (int) P.batch_ld;
-        if (!(opts & BATCH_VALUES_BOUND)) product_bind_values(P, Lval, Ax, s);
+        if (c.variant != 6 && !(opts & BATCH_VALUES_BOUND)) product_bind_values(P, Lval, Ax, s);
         const double *DT = D;
         int ldk = (int) ldd;
         if (!(opts & BATCH_D_KMAJOR))
@@ -1875,6 +1929,23 @@ void product_numeric_batched(const ProductPlan &P, const double *Lval, const dou
             ldk = ldt;
         }
         if (ev_start) SDL_HIP(hipEventRecord((hipEvent_t) ev_start, s));
+        if (c.variant == 6)
+        {
+            const int ntiles = (int) ((P.nnz + 255) / 256);
+            const int nchunks = (K + rc - 1) / rc;
+            const int64_t blocks = ((int64_t) ntiles + 7) / 8 * 8 * nchunks;
+            if (rc == 8)
+                hipLaunchKernelGGL(k_num_pull_interleaved_g<8>, dim3((unsigned) blocks), dim3(256),
+                                   0, s, P.nnz, ntiles, nchunks, P.ep.p, P.tA.p, P.tB.p, P.tK.p,
+                                   Lval, Ax, DT, ldk, K, Cx, ldc);
+            else
+                hipLaunchKernelGGL(k_num_pull_interleaved_g<16>, dim3((unsigned) blocks),
+                                   dim3(256), 0, s, P.nnz, ntiles, nchunks, P.ep.p, P.tA.p,
+                                   P.tB.p, P.tK.p, Lval, Ax, DT, ldk, K, Cx, ldc);
+            SDL_HIP(hipGetLastError());
+            if (ev_stop) SDL_HIP(hipEventRecord((hipEvent_t) ev_stop, s));
+            return;
+        }
         const int ept = c.variant == 4 ? 2 : (c.variant == 5 ? 4 : 1);
         const int ntiles = (int) ((P.nnz + 256 * ept - 1) / (256 * ept));
         const int nchunks = (K + rc - 1) / rc;
@@ -2003,7 +2074,7 @@ std::vector<RefillCfg> refill_candidates()
             c.rchunk = rc;
             v.push_back(c);
         }
-    for (int var = 3; var <= 5; var++)
+    for (int var = 3; var <= 6; var++)
         for (int rc : {8, 16})
         {
             c.variant = var;

@@ -50,7 +50,9 @@ struct SymStats
 // variant 0 = one thread per entry over all K refills; 1 = r-chunked grid, one
 // entry per thread; 2 = r-chunked grid, two entries per thread (16-byte stores);
 // 3 = d transposed to k-major and term operands pre-gathered (needs the
-// reserved workspace), r-chunked grid.
+// reserved workspace), r-chunked grid; 4 / 5 = 3 with 2 / 4 entries per thread;
+// 6 = 3 with the operands read through tA / tB in the kernel (no pair array;
+// calls with bound values use 3).
 struct RefillCfg
 {
     int variant = -1;

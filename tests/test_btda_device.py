@@ -152,3 +152,37 @@ def test_btda_config1_bit_exact_single_and_batched():
             assert bits_equal(XB[r], cx)
     finally:
         P.close()
+
+
+def test_every_refill_configuration_bit_exact():
+    """Every candidate launch configuration of the batched refill (including
+    the interleaved kernels that need the reservation) gives the oracle's
+    values for every refill of a K = 16 batch."""
+    from sparsediff_linalg_amd._lib import sdl_refill_cfg
+    lib = load()
+    rng = np.random.default_rng(21)
+    m, nA, nB = 3000, 700, 500
+    Ap, Ai, Ax = random_csr(rng, m, nA, 0.01)
+    Bp, Bi, Bx = random_csr(rng, m, nB, 0.012)
+    K = 16
+    D = np.stack([workloads.uniform(300 + r, m, 0.1, 10.0) for r in range(K)])
+    ref = [O.btda(m, nA, nB, Ap, Ai, Ax, Bp, Bi, Bx, D[r])[2] for r in range(K)]
+    P = DevPlan(m, nA, nB, Ap, Ai, Bp, Bi)
+    try:
+        check(lib.sdl_btda_plan_reserve(P.h, K))
+        cands = (sdl_refill_cfg * 64)()
+        nc = lib.sdl_refill_candidates(cands, 64)
+        assert any(cands[c].variant == 6 for c in range(nc))
+        dAx, dBx, dD = (DeviceArray.from_numpy(v) for v in (Ax, Bx, D))
+        ldc = (P.nnz + 63) // 64 * 64
+        dC = DeviceArray((K, ldc), np.float64)
+        for c in range(nc):
+            cfg = cands[c]
+            check(lib.sdl_btda_refill_batched_cfg(P.h, dAx.ptr, dBx.ptr, dD.ptr, m, K, dC.ptr, ldc,
+                                                  C.byref(cfg), None))
+            lib.sdl_synchronize()
+            X = dC.numpy()[:, :P.nnz]
+            for r in range(K):
+                assert bits_equal(X[r], ref[r]), (cfg.variant, cfg.rchunk, r)
+    finally:
+        P.close()
