@@ -59,10 +59,16 @@ class Group:
         self.ranges = [(int(cp[w["coff"][k] - c0]), int(cp[w["coff"][k + 1] - c0]))
                        for k in range(i0, i1)]
         self.ldc = (max(self.nnz, 1) + 63) // 64 * 64
-        self.C = DeviceArray(K * self.ldc, np.float64)  # ring of K output slots
+        self.C = None
+
+    def finish(self, lib, K):
+        """Refill workspace, bound operand values and the ring of K output
+        slots (after every group's plan, so no symbolic pass runs beside
+        them)."""
         if self.pull:
-            check(lib.sdl_btda_plan_reserve(h, K), "reserve")
-            check(lib.sdl_btda_bind_values(h, self.A[2].ptr, self.B[2].ptr, None), "bind")
+            check(lib.sdl_btda_plan_reserve(self.h, K), "reserve")
+            check(lib.sdl_btda_bind_values(self.h, self.A[2].ptr, self.B[2].ptr, None), "bind")
+        self.C = DeviceArray(K * self.ldc, np.float64)  # ring of K output slots
 
 
 class MultiProduct:
@@ -87,6 +93,9 @@ class MultiProduct:
             acc += int(t)
         self.groups.append((i0, len(inst_terms)))
         self.g = [Group(lib, w, a, b, K) for a, b in self.groups]
+        lib.sdl_synchronize()
+        for g in self.g:
+            g.finish(lib, K)
         self.nnz = sum(g.nnz for g in self.g)
         self.terms = sum(g.terms for g in self.g)
         lib.sdl_synchronize()
