@@ -233,8 +233,8 @@ def main():
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": "batched refill sequence (k_gather_terms + k_transpose_d + "
-                      "k_num_pull_interleaved)",
+            "kernel": "batched refill sequence (k_prep_batch: operand pairs + d "
+                      "transposition; k_num_pull_interleaved)",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",

@@ -927,8 +927,8 @@ __global__ void __launch_bounds__(256)
 // Batch preparation for the interleaved refill.
 // k_gather_terms: each term's operand pair tv[s] = (L value, A value) in term
 // order (depends only on the operand values).
-// k_transpose_d: D ([K][ldd], refill-major) -> DT ([m][ldt], k-major: the K
-// values of one k contiguous), through a 16 x 64 LDS tile.
+// d transposition: D ([K][ldd], refill-major) -> DT ([m][ldt], k-major: the K
+// values of one k contiguous), through a 16 x 64 LDS tile (k_prep_batch).
 constexpr int TR_K = 64;   // k per transpose tile
 constexpr int TR_R = 16;   // refills per transpose tile
 
@@ -945,13 +945,29 @@ __global__ void __launch_bounds__(256)
     *(dvec2 *) (tv + 2 * s) = v;
 }
 
+// Batch preparation in one launch: workgroups [0, ngather) gather operand
+// pairs (as k_gather_terms), the rest transpose d tiles; the two are
+// independent, so they run side by side.
 __global__ void __launch_bounds__(256)
-    k_transpose_d(int m, int K, const double *__restrict__ D, int64_t ldd, double *__restrict__ DT,
-                  int ldt)
+    k_prep_batch(int ngather, int64_t terms, const int *__restrict__ tA,
+                 const int *__restrict__ tB, const double *__restrict__ Lval,
+                 const double *__restrict__ Ax, double *__restrict__ tv, int m, int K,
+                 const double *__restrict__ D, int64_t ldd, double *__restrict__ DT, int ldt)
 {
     __shared__ double tile[TR_R][TR_K + 1];
+    if ((int) blockIdx.x < ngather)
+    {
+        const int64_t s = (int64_t) blockIdx.x * 256 + threadIdx.x;
+        if (s >= terms) return;
+        dvec2 v;
+        v.x = Lval[tB[s]];
+        v.y = Ax[tA[s]];
+        *(dvec2 *) (tv + 2 * s) = v;
+        return;
+    }
+    const int bt = (int) blockIdx.x - ngather;
     const int nkt = (m + TR_K - 1) / TR_K;
-    const int kt = (int) blockIdx.x % nkt, rt = (int) blockIdx.x / nkt;
+    const int kt = bt % nkt, rt = bt / nkt;
     const int k0 = kt * TR_K, r0 = rt * TR_R;
     for (int i = threadIdx.x; i < TR_R * TR_K; i += 256)
     {
@@ -1915,19 +1931,24 @@ void product_numeric_batched(const ProductPlan &P, const double *Lval, const dou
     {
         const int rc = c.rchunk == 8 ? 8 : 16;
         const int ldt = (int) P.batch_ld;
-        if (c.variant != 6 && !(opts & BATCH_VALUES_BOUND)) product_bind_values(P, Lval, Ax, s);
-        const double *DT = D;
-        int ldk = (int) ldd;
-        if (!(opts & BATCH_D_KMAJOR))
+        // operand pairs (unless bound) and d transposed to k-major (unless
+        // given so), in one launch
+        const bool gather = c.variant != 6 && !(opts & BATCH_VALUES_BOUND) && P.terms > 0;
+        const bool transpose = !(opts & BATCH_D_KMAJOR);
+        const int ngather = gather ? (int) ((P.terms + 255) / 256) : 0;
+        const int ntrans = transpose ? (int) (((P.A.Am + TR_K - 1) / TR_K) *
+                                              ((K + TR_R - 1) / TR_R))
+                                     : 0;
+        if (ngather + ntrans > 0)
         {
-            const int nkt = (int) ((P.A.Am + TR_K - 1) / TR_K);
-            const int nrt = (K + TR_R - 1) / TR_R;
-            hipLaunchKernelGGL(k_transpose_d, dim3((unsigned) (nkt * nrt)), dim3(256), 0, s,
-                               (int) P.A.Am, K, D, ldd, P.dT.p, ldt);
+            hipLaunchKernelGGL(k_prep_batch, dim3((unsigned) (ngather + ntrans)), dim3(256), 0, s,
+                               ngather, P.terms, P.tA.p, P.tB.p, Lval, Ax,
+                               const_cast<double *>(P.tv.p), (int) P.A.Am, K, D, ldd, P.dT.p,
+                               ldt);
             SDL_HIP(hipGetLastError());
-            DT = P.dT.p;
-            ldk = ldt;
         }
+        const double *DT = transpose ? P.dT.p : D;
+        const int ldk = transpose ? ldt : (int) ldd;
         if (ev_start) SDL_HIP(hipEventRecord((hipEvent_t) ev_start, s));
         if (c.variant == 6)
         {
