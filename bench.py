@@ -540,11 +540,93 @@ def main_learned(args):
         "exact_terms_per_s": tot_terms / tot_exact,
         "gain_vs_term_bound": tot_exact / tot_pred,
         "roofline": None,
-        "cpu_baseline": None,
+        "cpu_baseline": None if args.no_cpu else symbolic_cpu_baseline(args.cpu_seconds),
     }
     for mlp, _ in nets.values():
         mlp.close()
     print(json.dumps(out), flush=True)
+
+
+def symbolic_cpu_baseline(seconds):
+    """The oracle's single-thread two-pass symbolic Gustavson (mark-array count,
+    fill, per-row sort) over chunks of 4096 output rows of the banded A^T A
+    (2^20 rows, bandwidth 64) at random offsets, until about `seconds`."""
+    import ctypes as Cc
+
+    from oracle import oracle as O
+    from sparsediff_linalg_amd import workloads
+    n = 1 << 20
+    p, i, _ = workloads.banded_csr(n, 64)
+    Lp, Li, _ = O.transpose(n, n, p, i)
+    Lp = np.asarray(Lp, np.int32)
+    Li = np.asarray(Li, np.int32)
+    Ap, Ai = np.ascontiguousarray(p, np.int32), np.ascontiguousarray(i, np.int32)
+    o = O.lib()
+    P = Cc.POINTER(Cc.c_int)
+    lens = np.diff(Ap).astype(np.int64)
+    rng = np.random.default_rng(5)
+    rows = terms = 0
+    t = 0.0
+    while t < seconds:
+        a = int(rng.integers(0, n - 4096))
+        lp = np.ascontiguousarray(Lp[a:a + 4097] - Lp[a], np.int32)
+        li = np.ascontiguousarray(Li[Lp[a]:Lp[a + 4096]], np.int32)
+        terms += int(lens[li].sum())
+        cp = np.zeros(4097, np.int64)
+        t0 = time.perf_counter()
+        nnz = o.orc_spgemm_symbolic(4096, lp.ctypes.data_as(P), li.ctypes.data_as(P), 1, n, n,
+                                    Ap.ctypes.data_as(P), Ai.ctypes.data_as(P),
+                                    cp.ctypes.data_as(Cc.POINTER(Cc.c_longlong)), None)
+        ci = np.zeros(max(nnz, 1), np.int32)
+        o.orc_spgemm_symbolic(4096, lp.ctypes.data_as(P), li.ctypes.data_as(P), 1, n, n,
+                              Ap.ctypes.data_as(P), Ai.ctypes.data_as(P),
+                              cp.ctypes.data_as(Cc.POINTER(Cc.c_longlong)), ci.ctypes.data_as(P))
+        t += time.perf_counter() - t0
+        rows += 4096
+    return {"value": terms / t, "unit": "terms/s", "cores": 1, "kind": "port",
+            "sample": f"{rows} output rows of the banded A^T A (2^20 rows, bandwidth 64) in chunks "
+                      f"of 4096, two-pass symbolic (count, fill + sort), {t:.1f} s on one core "
+                      f"(oracle C)"}
+
+
+def multi_cpu_baseline(w, seconds):
+    """The reference's single-thread path (oracle/_ref when built, else the C
+    restatement) per instance: alloc once, two refills with fresh d, over
+    instances in random order until about `seconds` of refill time."""
+    from oracle import oracle as O
+    from sparsediff_linalg_amd import workloads
+    rng = np.random.default_rng(9)
+    kind = "reference" if O.ref_available() else "port"
+    nnz_total = fills_n = 0
+    fill_s = 0.0
+    done = 0
+    for k in rng.permutation(len(w["m"])):
+        if fill_s >= seconds:
+            break
+        r0, r1 = int(w["roff"][k]), int(w["roff"][k + 1])
+        c0, c1 = int(w["coff"][k]), int(w["coff"][k + 1])
+        ops = []
+        for tag in ("A", "B"):
+            p, i, x = w[tag]
+            a, b = int(p[r0]), int(p[r1])
+            ops.append(((p[r0:r1 + 1] - a).astype(np.int32), (i[a:b] - c0).astype(np.int32),
+                        np.ascontiguousarray(x[a:b])))
+        (Ap, Ai, Ax), (Bp, Bi, Bx) = ops
+        m, nn = r1 - r0, c1 - c0
+        ds = [workloads.uniform(60_000 + 2 * int(k) + j, m, 0.1, 10.0) for j in range(2)]
+        if kind == "reference":
+            _, fills, cx = O.ref_fill_timer(m, nn, nn, Ap, Ai, Ax, Bp, Bi, Bx, ds)
+            nnz = len(cx)
+        else:
+            _, fills = O.port_fill_timer(m, nn, nn, Ap, Ai, Ax, Bp, Bi, Bx, ds)
+            nnz = len(O.btda(m, nn, nn, Ap, Ai, Ax, Bp, Bi, Bx, ds[0])[1])
+        fill_s += sum(fills)
+        nnz_total += nnz * len(fills)
+        fills_n += len(fills)
+        done += 1
+    return {"value": nnz_total / fill_s, "unit": "nnz-out/s", "cores": 1, "kind": kind,
+            "sample": f"{done} instances in random order, alloc once and 2 refills each with fresh "
+                      f"d ({fills_n} refills, {fill_s:.1f} s of refill time on one core)"}
 
 
 def coo_cpu_baseline(rows, cols, x, m, n, count=5_000_000):
@@ -664,7 +746,7 @@ def main_multi(args):
                       "launches_per_round": sum(len(g.ranges) for g in mp.g),
                       "nnz_out_per_s": mp.nnz / (ms_plain_round * 1e-3)},
         "batched_vs_plain": ms_plain_round / (ms_graph / R),
-        "cpu_baseline": None,
+        "cpu_baseline": None if args.no_cpu else multi_cpu_baseline(w, args.cpu_seconds),
     }
     lib.sdl_graph_destroy(graph)
     mp.close()
