@@ -1569,12 +1569,15 @@ void run_symbolic_class(int c, const Bins &B, const ProductPlan &P, int nwords, 
         SDL_HASH_CASE(8, 256, 8192)
     case CLS_BITMAP:
     {
+        // 1024 threads per row where the bitmap leaves room for their static
+        // arrays (more waves to hide the term loads), else 256
         size_t lds = (size_t) nwords * 4;
-        auto fn = k_sym_bitmap<256, PASS, false>;
+        const bool wide = lds + 20 * 1024 <= 160 * 1024;
+        auto fn = wide ? k_sym_bitmap<1024, PASS, false> : k_sym_bitmap<256, PASS, false>;
         SDL_HIP(hipFuncSetAttribute((const void *) fn,
                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds));
-        hipLaunchKernelGGL(fn, dim3(nrows), dim3(256), lds, stream(), rows, nrows, nwords,
-                           L.Lm, L.Lp, L.Li, nL, A.Ap, A.Ai, rowcnt, Cp, Ci,
+        hipLaunchKernelGGL(fn, dim3(nrows), dim3(wide ? 1024 : 256), lds, stream(), rows, nrows,
+                           nwords, L.Lm, L.Lp, L.Li, nL, A.Ap, A.Ai, rowcnt, Cp, Ci,
                            (unsigned *) nullptr);
         break;
     }
