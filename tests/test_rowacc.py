@@ -73,3 +73,13 @@ def test_dense_rows_beyond_rank_width_match_oracle():
     Ap, Ai, Ax = hub_csr(5, m, n, 0, 12, 1500)
     lens = refill_and_compare(m, n, Ap, Ai, Ax, 33)
     assert (lens > 1024).sum() == 1500
+
+
+def test_global_bitmap_symbolic_matches_oracle():
+    # n = 1,300,000 columns: a column bitmap no longer fits LDS, and 8 rows over
+    # the same 5000 columns give every output row 40,000 terms, beyond the
+    # largest hash table, so the symbolic passes use bitmaps in global memory
+    m, n = 8, 1_300_000
+    Ap, Ai, Ax = hub_csr(6, m, n, 0, 8, 5000)
+    lens = refill_and_compare(m, n, Ap, Ai, Ax, 34)
+    assert (lens == 5000).sum() == 5000
