@@ -38,6 +38,8 @@ _SIGS = {
     "orc_da": (None, [C.c_int, ip, dp, dp, dp]),
     "orc_kron_spmv": (None, [C.c_int, C.c_int, ip, ip, dp, dp, dp, C.c_int]),
     "orc_coo_lower": (C.c_int, [C.c_int, ip, ip, dp, ip, ip, dp, ip]),
+    "orc_sum_n_alloc": (C.c_int, [C.c_int, C.c_int, C.c_int, ip, ip, ip, ip, ip, ip]),
+    "orc_sum_n_fill": (None, [C.c_int, C.c_int, ip, ip, dp, dp, ip, C.c_int, dp]),
 }
 
 _lib = None
@@ -171,6 +173,32 @@ def row_reduce(m, n, Ap, Ai, Ax, group, m_out):
     return Cp, Ci[:nnz], Cx[:nnz], mp[:nnzA]
 
 
+def sum_n(m, n, ops, d=None):
+    """Sum of same-shape CSR operands ops = [(p, i, x), ...]: returns (Cp, Ci,
+    Cx, map) with map the output position of every operand entry, operands
+    back to back.  d (len(ops) x m row scales) gives the diag-scaled sum."""
+    o = lib()
+    nops = len(ops)
+    P = np.ascontiguousarray(np.concatenate([np.asarray(p, np.int32) for p, _, _ in ops]),
+                             np.int32)
+    nz = [int(p[m]) for p, _, _ in ops]
+    off = np.zeros(nops + 1, np.int32)
+    off[1:] = np.cumsum(nz)
+    tot = int(off[-1])
+    cat = (lambda arrs, t: np.ascontiguousarray(np.concatenate(arrs) if tot else np.zeros(1, t), t))
+    Iall = cat([np.asarray(i, np.int32)[:k] for (_, i, _), k in zip(ops, nz)], np.int32)
+    Xall = cat([np.asarray(x, np.float64)[:k] for (_, _, x), k in zip(ops, nz)], np.float64)
+    Cp = np.zeros(m + 1, np.int32)
+    Ci = np.zeros(max(tot, 1), np.int32)
+    mp = np.zeros(max(tot, 1), np.int32)
+    nnz = o.orc_sum_n_alloc(m, n, nops, I(P), I(Iall), I(off), I(Cp), I(Ci), I(mp))
+    Cx = np.zeros(max(nnz, 1), np.float64)
+    dd = None if d is None else np.ascontiguousarray(d, np.float64).reshape(-1)
+    o.orc_sum_n_fill(m, nops, I(P), I(off), D(Xall), D(dd) if dd is not None else None, I(mp),
+                     nnz, D(Cx))
+    return Cp, Ci[:nnz], Cx[:nnz], mp[:tot]
+
+
 def row_gather(Ap, Ai, Ax, rmap):
     o = lib()
     rmap = np.ascontiguousarray(rmap, np.int32)
@@ -255,6 +283,14 @@ _REF_SIGS = {
     "sparse_matrix_free": (None, [C.POINTER(_Mat)]),
     "new_sparse_matrix": (C.POINTER(_Mat), [C.POINTER(_CSR)]),
     "new_CSR_matrix": (C.POINTER(_CSR), [C.c_int, C.c_int, C.c_int]),
+    "sum_csr_alloc": (None, [C.POINTER(_CSR), C.POINTER(_CSR), C.POINTER(_CSR)]),
+    "sum_csr_fill_values": (None, [C.POINTER(_CSR), C.POINTER(_CSR), C.POINTER(_CSR)]),
+    "sum_scaled_csr_matrices_fill_values": (None, [C.POINTER(_CSR), C.POINTER(_CSR),
+                                                   C.POINTER(_CSR), dp, dp]),
+    "sum_4_csr_alloc": (C.POINTER(_CSR), [C.POINTER(_CSR), C.POINTER(_CSR), C.POINTER(_CSR),
+                                          C.POINTER(_CSR), C.POINTER(ip)]),
+    "sum_n_csr_alloc": (C.POINTER(_CSR), [C.POINTER(C.POINTER(_CSR)), C.c_int, ip]),
+    "accumulator": (None, [dp, C.c_int, ip, dp]),
 }
 
 _ref = None
@@ -314,6 +350,66 @@ def ref_btda(m, nA, nB, Ap, Ai, Ax, Bp, Bi, Bx, d, timing=None):
     r.sparse_matrix_free(Cm)
     r.free_CSC_matrix(Bc)
     return Cp, Ci, Cx
+
+
+def _ref_out(Cc, m):
+    nnz = Cc.nnz
+    Cp = np.ctypeslib.as_array(Cc.p, (m + 1,)).copy()
+    Ci = np.ctypeslib.as_array(Cc.i, (max(nnz, 1),))[:nnz].copy() if nnz else np.zeros(0, np.int32)
+    return Cp, Ci
+
+
+def ref_sum(m, n, ops, d=None):
+    """The reference's own CSR sums (CSR_sum.c): two operands through
+    sum_csr_alloc and sum_csr_fill_values (sum_scaled_csr_matrices_fill_values
+    with d = (d1, d2)), four through sum_4_csr_alloc and accumulator, any other
+    count through sum_n_csr_alloc and accumulator.  Returns (Cp, Ci, Cx)."""
+    r = ref_lib()
+    keep = []
+    views = [_csr_view(m, n, p, i, x, keep) for p, i, x in ops]
+    if len(ops) == 2:
+        cap = views[0].nnz + views[1].nnz
+        Cc = r.new_CSR_matrix(m, n, max(cap, 1))
+        r.sum_csr_alloc(C.byref(views[0]), C.byref(views[1]), Cc)
+        if d is None:
+            r.sum_csr_fill_values(C.byref(views[0]), C.byref(views[1]), Cc)
+        else:
+            d1, d2 = (np.ascontiguousarray(v, np.float64) for v in d)
+            keep.extend([d1, d2])
+            r.sum_scaled_csr_matrices_fill_values(C.byref(views[0]), C.byref(views[1]), Cc,
+                                                  D(d1), D(d2))
+        Cp, Ci = _ref_out(Cc.contents, m)
+        nnz = int(Cp[m])
+        Cx = np.ctypeslib.as_array(Cc.contents.x, (max(nnz, 1),))[:nnz].copy()
+        r.free_CSR_matrix(Cc)
+        return Cp, Ci, Cx
+    assert d is None
+    tot = sum(v.nnz for v in views)
+    maps = [np.zeros(max(v.nnz, 1), np.int32) for v in views]
+    if len(ops) == 4:
+        marr = (ip * 4)(*[I(mp) for mp in maps])
+        Cc = r.sum_4_csr_alloc(*[C.byref(v) for v in views], marr)
+        # sum_4_csr_alloc allocates the maps itself (CSR_sum.c:170-173)
+        maps = [np.ctypeslib.as_array(marr[k], (views[k].nnz,)).copy() if views[k].nnz
+                else np.zeros(1, np.int32) for k in range(4)]
+    else:
+        flat = np.zeros(max(tot, 1), np.int32)
+        parr = (C.POINTER(_CSR) * len(views))(*[C.pointer(v) for v in views])
+        Cc = r.sum_n_csr_alloc(parr, len(views), I(flat))
+        o = 0
+        for k, v in enumerate(views):
+            maps[k] = flat[o:o + v.nnz].copy()
+            o += v.nnz
+    Cp, Ci = _ref_out(Cc.contents, m)
+    nnz = int(Cp[m])
+    Cx = np.zeros(max(nnz, 1), np.float64)
+    for (p, i, x), v, mp in zip(ops, views, maps):
+        if v.nnz:
+            xv = np.ascontiguousarray(x, np.float64)
+            mp = np.ascontiguousarray(mp, np.int32)
+            r.accumulator(D(xv), v.nnz, I(mp), D(Cx))
+    r.free_CSR_matrix(Cc)
+    return Cp, Ci, Cx[:nnz]
 
 
 def ref_fill_timer(m, nA, nB, Ap, Ai, Ax, Bp, Bi, Bx, ds):

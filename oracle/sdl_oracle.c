@@ -392,3 +392,65 @@ int orc_coo_lower(int m, const int *Ap, const int *Ai, const double *Ax, int *ro
     }
     return cnt;
 }
+
+/* Matrix sums (CSR_sum.c:230-297 sum_n_csr_alloc, matrix_sum.c:112-180).
+   Output row r holds the ascending union of the operands' columns in row r;
+   map[off[k] + t] is the output position of operand k's entry t. */
+int orc_sum_n_alloc(int m, int n, int nops, const int *P, const int *I,
+                    const int *off, int *Cp, int *Ci, int *map)
+{
+    int *where = (int *) malloc((size_t) (n > 0 ? n : 1) * sizeof(int));
+    for (int j = 0; j < n; j++) where[j] = -1;
+    int nnz = 0;
+    Cp[0] = 0;
+    for (int r = 0; r < m; r++)
+    {
+        int row_begin = nnz;
+        for (int k = 0; k < nops; k++)
+        {
+            const int *p = P + (size_t) k * (m + 1);
+            const int *idx = I + off[k];
+            for (int t = p[r]; t < p[r + 1]; t++)
+            {
+                if (where[idx[t]] < row_begin)
+                {
+                    where[idx[t]] = nnz;
+                    Ci[nnz++] = idx[t];
+                }
+            }
+        }
+        qsort(Ci + row_begin, (size_t) (nnz - row_begin), sizeof(int), cmp_int);
+        for (int e = row_begin; e < nnz; e++) where[Ci[e]] = e;
+        for (int k = 0; k < nops; k++)
+        {
+            const int *p = P + (size_t) k * (m + 1);
+            for (int t = p[r]; t < p[r + 1]; t++) map[off[k] + t] = where[I[off[k] + t]];
+        }
+        Cp[r + 1] = nnz;
+    }
+    free(where);
+    return nnz;
+}
+
+/* Cx = 0, then Cx[map[t]] += X[t] over the concatenated operand values in
+   operand order (matrix_sum.c:142-155 with CSR_sum.c:150-157).  With d non-NULL
+   (nops x m row scales) each term is the rounded product d[k*m + r] * X[t]
+   (CSR_sum.c:116-148: val += d1[row] * a; val += d2[row] * b). */
+void orc_sum_n_fill(int m, int nops, const int *P, const int *off, const double *X,
+                    const double *d, const int *map, int nnzC, double *Cx)
+{
+    memset(Cx, 0, (size_t) nnzC * sizeof(double));
+    for (int k = 0; k < nops; k++)
+    {
+        const int *p = P + (size_t) k * (m + 1);
+        for (int r = 0; r < m; r++)
+        {
+            for (int t = p[r]; t < p[r + 1]; t++)
+            {
+                double v = X[off[k] + t];
+                if (d) v = d[(size_t) k * m + r] * v;
+                Cx[map[off[k] + t]] += v;
+            }
+        }
+    }
+}

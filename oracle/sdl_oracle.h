@@ -110,6 +110,20 @@ void orc_kron_spmv(int m, int n, const int *Ap, const int *Ai, const double *Ax,
 int orc_coo_lower(int m, const int *Ap, const int *Ai, const double *Ax, int *rows,
                   int *cols, double *x, int *value_map);
 
+/* ---- matrix sums (src/sparse/CSR_sum.c:26-297, src/sparse/matrix_sum.c) ---- */
+
+/* Union structure of nops operands of one shape.  P holds nops row-pointer
+   arrays of m+1 entries each; I the operands' column indices back to back,
+   operand k starting at off[k] (off has nops+1 entries).  Cp (m+1), Ci
+   (capacity off[nops]) and map (off[nops]: output position of each operand
+   entry).  Returns the output nnz. */
+int orc_sum_n_alloc(int m, int n, int nops, const int *P, const int *I,
+                    const int *off, int *Cp, int *Ci, int *map);
+/* Values: +0.0 plus each operand's entries in operand order; d (nops x m row
+   scales, or NULL) multiplies every term first. */
+void orc_sum_n_fill(int m, int nops, const int *P, const int *off, const double *X,
+                    const double *d, const int *map, int nnzC, double *Cx);
+
 #ifdef __cplusplus
 }
 #endif

@@ -60,6 +60,16 @@ SIGNATURES = {
     "BA_dense_kron_I_matrices_fill_values": (None, [MAT_p, C.c_int, MAT_p, MAT_p]),
     "BA_kron_matrices_alloc": (MAT_p, [MAT_p, C.c_int, MAT_p]),
     "BA_kron_matrices_fill_values": (None, [MAT_p, C.c_int, MAT_p, MAT_p]),
+    # matrix_sum.h
+    "sum_matrices_alloc": (None, [MAT_p, MAT_p, MAT_p]),
+    "sum_matrices_fill_values": (None, [MAT_p, MAT_p, MAT_p]),
+    "sum_scaled_matrices_fill_values": (None, [MAT_p, MAT_p, MAT_p, dp, dp]),
+    "sum_4_matrices_alloc": (MAT_p, [MAT_p, MAT_p, MAT_p, MAT_p]),
+    "sum_4_matrices_fill_values": (None, [MAT_p, MAT_p, MAT_p, MAT_p, MAT_p]),
+    "sum_n_matrices_alloc": (MAT_p, [C.POINTER(MAT_p), C.c_int]),
+    "sum_n_matrices_fill_values": (None, [C.POINTER(MAT_p), C.c_int, MAT_p]),
+    "sum_n_matrices_fill_zero": (None, [MAT_p]),
+    "sum_n_matrices_fill_add": (None, [MAT_p, C.c_int, MAT_p]),
     # matrix_vstack.h
     "vstack_matrices_alloc": (MAT_p, [C.POINTER(MAT_p), C.c_int]),
     "vstack_matrices_fill_values": (None, [C.POINTER(MAT_p), C.c_int, MAT_p]),

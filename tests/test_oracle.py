@@ -212,6 +212,75 @@ def test_da_and_gather_closed_form():
     assert list(Gi) == list(ref_i) and bits_equal(Gx, ref_x)
 
 
+# ------------------------------------------------------------- sums ----
+# test_matmul_dispatchers.h:3001-3012 (test_sum_n_matrices_n1_n5): five 4 x 6
+# operands, M0 twice, one empty, a row empty in all of them.
+SUM5_OPS = [([0, 2, 3, 3, 4], [1, 5, 2, 0], [1, 2, 3, 4]),
+            ([0, 1, 3, 3, 3], [0, 2, 4], [5, 6, 7]),
+            ([0, 2, 3, 3, 4], [1, 5, 2, 0], [1, 2, 3, 4]),
+            ([0, 0, 0, 0, 0], [], []),
+            ([0, 1, 1, 1, 3], [5, 1, 5], [8, 9, 10])]
+
+
+def _dense(m, n, p, i, x):
+    out = np.zeros((m, n))
+    for r in range(m):
+        out[r, np.asarray(i[p[r]:p[r + 1]], int)] = x[p[r]:p[r + 1]]
+    return out
+
+
+def test_sum_reference_literal_n5():
+    Cp, Ci, Cx, _ = O.sum_n(4, 6, SUM5_OPS)
+    assert list(Cp) == [0, 3, 5, 5, 8]
+    assert list(Ci) == [0, 1, 5, 2, 4, 0, 1, 5]
+    ref = sum(_dense(4, 6, *o) for o in SUM5_OPS)
+    assert np.array_equal(_dense(4, 6, Cp, Ci, Cx), ref)
+    # n = 1 copies the operand
+    Cp, Ci, Cx, mp = O.sum_n(4, 6, SUM5_OPS[:1])
+    assert list(Cp) == SUM5_OPS[0][0] and list(Ci) == SUM5_OPS[0][1]
+    assert list(Cx) == SUM5_OPS[0][2] and list(mp) == [0, 1, 2, 3]
+
+
+@pytest.mark.parametrize("nops", [1, 2, 3, 6])
+def test_sum_closed_form(nops):
+    """Structure: the dense union of the patterns.  Values: the dense sum taken
+    operand by operand from zeros (x + 0.0 == x, so a missing entry changes
+    nothing and the dense sum is the fold itself), bit for bit; the map sends
+    every operand entry to its (row, col)."""
+    rng = np.random.default_rng(40 + nops)
+    m, n = 23, 17
+    ops = [random_csr(rng, m, n, 0.05 + 0.05 * k, empty_row=k % 3) for k in range(nops)]
+    Cp, Ci, Cx, mp = O.sum_n(m, n, ops)
+    pat = np.zeros((m, n), bool)
+    acc = np.zeros((m, n))
+    for p, i, x in ops:
+        pat |= _dense(m, n, p, i, np.ones(len(i))) != 0
+        acc = acc + _dense(m, n, p, i, x)
+    rows = np.repeat(np.arange(m), np.diff(Cp))
+    assert np.array_equal(np.argwhere(pat), np.stack([rows, Ci], 1))
+    assert bits_equal(Cx, acc[rows, Ci])
+    o = 0
+    for p, i, x in ops:
+        r = np.repeat(np.arange(m), np.diff(p))
+        assert np.array_equal(rows[mp[o:o + len(i)]], r)
+        assert np.array_equal(Ci[mp[o:o + len(i)]], i)
+        o += len(i)
+    # scaled: each term the rounded product d[row] * x, summed in operand order
+    if nops == 2:
+        d = rng.uniform(-3, 3, (2, m))
+        _, _, Sx, _ = O.sum_n(m, n, ops, d=d)
+        sacc = d[0][:, None] * _dense(m, n, *ops[0]) + d[1][:, None] * _dense(m, n, *ops[1])
+        assert bits_equal(Sx, sacc[rows, Ci])
+
+
+def test_sum_repeated_entries_fold_in_order():
+    """An operand repeating (row, col) contributes each copy in storage order."""
+    ops = [([0, 3], [2, 2, 5], [1e16, 1.0, 2.0]), ([0, 1], [2], [-1e16])]
+    Cp, Ci, Cx, mp = O.sum_n(1, 6, ops)
+    assert list(Ci) == [2, 5] and list(mp) == [0, 0, 1, 0]
+    assert Cx[0] == ((0.0 + 1e16) + 1.0) + -1e16 and Cx[1] == 2.0
+
+
 # -------------------------------------------------- vs oracle/_ref ----
 needs_ref = pytest.mark.skipif(not O.ref_available(),
                                reason="oracle/_ref not built (reference tree absent)")
@@ -286,3 +355,28 @@ def test_spgemm_rows_equal_full_product():
     for r, (cols, vals) in zip(rows, _O.spgemm_rows(rows, nB, Lp, Li, Lx, 1, d, m, nA, Ap, Ai, Ax)):
         assert _np.array_equal(cols, Ci[Cp[r]:Cp[r + 1]])
         assert _np.array_equal(vals.view(_np.int64), Cx[Cp[r]:Cp[r + 1]].view(_np.int64))
+
+
+@needs_ref
+@pytest.mark.parametrize("nops", [1, 2, 3, 4, 5])
+def test_sum_matches_reference_build(nops):
+    """sum_csr_alloc / sum_csr_fill_values (2), sum_4_csr_alloc + accumulator (4)
+    and sum_n_csr_alloc + accumulator (others), bit for bit."""
+    rng = np.random.default_rng(300 + nops)
+    m, n = 80, 60
+    ops = [random_csr(rng, m, n, 0.04 * (k + 1), empty_row=k) for k in range(nops)]
+    Cp, Ci, Cx, _ = O.sum_n(m, n, ops)
+    Rp, Ri, Rx = O.ref_sum(m, n, ops)
+    assert list(Cp) == list(Rp) and list(Ci) == list(Ri) and bits_equal(Cx, Rx)
+    if nops == 2:
+        d1, d2 = rng.uniform(-2, 2, m), rng.uniform(-2, 2, m)
+        _, _, Sx, _ = O.sum_n(m, n, ops, d=np.stack([d1, d2]))
+        _, _, Rs = O.ref_sum(m, n, ops, d=(d1, d2))
+        assert bits_equal(Sx, Rs)
+
+
+@needs_ref
+def test_sum_reference_literal_through_reference_build():
+    Rp, Ri, Rx = O.ref_sum(4, 6, SUM5_OPS)
+    Cp, Ci, Cx, _ = O.sum_n(4, 6, SUM5_OPS)
+    assert list(Rp) == list(Cp) and list(Ri) == list(Ci) and bits_equal(Rx, Cx)
