@@ -56,7 +56,7 @@ def parse():
                     help="multi: refill rounds per launch (even; output ring of this many slots)")
     ap.add_argument("--workload", default="btda",
                     choices=["btda", "blocks-atda", "blocks-btda", "multi", "ata-large", "coo",
-                             "learned"],
+                             "learned", "sum"],
                     help="btda: the sparse refill (default line); blocks-*: the stacked "
                          "permuted-dense products (A^T D A, or B^T D A of two stacks)")
     ap.add_argument("--rmat-scale", type=int, default=18,
@@ -100,6 +100,8 @@ def main():
         return main_coo(args)
     if args.workload == "learned":
         return main_learned(args)
+    if args.workload == "sum":
+        return main_sum(args)
     if args.workload != "btda":
         return main_blocks(args)
     import sparsediff_linalg_amd as sdl
@@ -437,6 +439,125 @@ def main_coo(args):
     }
     lib.sdl_coo_plan_destroy(h)
     print(json.dumps(out_json), flush=True)
+
+
+def main_sum(args):
+    """Hessian-assembly sum: four 1M x 1M operands of ~16 entries per row
+    (each later operand shares half of operand 0's draws), summed into their
+    union (~40M entries from 64M operand entries) from +0.0 in operand order.
+    One step = one value fill through the device sum plan."""
+    import sparsediff_linalg_amd as sdl
+    from oracle import oracle as O
+    from sparsediff_linalg_amd import DeviceArray, check, workloads
+
+    lib = sdl.load()
+    m = n = 1_000_000
+    ops = workloads.sum_operands(6, m, n, 4, 16, 0.5)
+    nops = len(ops)
+    dev = [tuple(DeviceArray.from_numpy(a) for a in o) for o in ops]
+    vp = C.c_void_p
+    dP = (vp * nops)(*[d[0].ptr for d in dev])
+    dI = (vp * nops)(*[d[1].ptr for d in dev])
+    dX = (vp * nops)(*[d[2].ptr for d in dev])
+    nz = (C.c_longlong * nops)(*[len(o[1]) for o in ops])
+    items = int(sum(nz))
+    lib.sdl_synchronize()
+    h = vp()
+    t0 = time.perf_counter()
+    check(lib.sdl_sum_plan_create(nops, m, n, dP, dI, nz, 0, C.byref(h)), "sum plan")
+    lib.sdl_synchronize()
+    t_plan = time.perf_counter() - t0
+    nnz = lib.sdl_sum_plan_nnz(h)
+    masked = bool(lib.sdl_sum_plan_is_masked(h))
+    out = DeviceArray(nnz, np.float64)
+    stream = lib.sdl_library_stream()
+    for _ in range(max(1, args.warmup)):
+        check(lib.sdl_sum_fill(h, dX, None, out.ptr, stream), "fill")
+    lib.sdl_synchronize()
+    e0, e1 = lib.sdl_event_create(), lib.sdl_event_create()
+    steps = max(1, min(args.steps, 100))
+    lib.sdl_event_record(e0, stream)
+    for _ in range(steps):
+        check(lib.sdl_sum_fill(h, dX, None, out.ptr, stream), "fill")
+    lib.sdl_event_record(e1, stream)
+    lib.sdl_synchronize()
+    ms = lib.sdl_event_elapsed_ms(e0, e1) / steps
+    if args.calib_copy_mb > 0:
+        nb = args.calib_copy_mb << 20
+        src, dst = DeviceArray(nb // 8, np.float64), DeviceArray(nb // 8, np.float64)
+        for _ in range(3):
+            check(lib.sdl_copy_kernel(dst.ptr, src.ptr, nb, stream), "copy")
+        lib.sdl_synchronize()
+    # whole output against the oracle, bit for bit
+    Cp, Ci, Cx, _ = O.sum_n(m, n, ops)
+    cp = np.empty(m + 1, np.int64)
+    ci = np.empty(nnz, np.int32)
+    lib.sdl_memcpy_dtoh(cp.ctypes.data, lib.sdl_sum_plan_rowptr(h), cp.nbytes)
+    lib.sdl_memcpy_dtoh(ci.ctypes.data, lib.sdl_sum_plan_colind(h), ci.nbytes)
+    exact = (np.array_equal(cp, Cp) and np.array_equal(ci, Ci)
+             and np.array_equal(out.numpy().view(np.int64), Cx.view(np.int64)))
+    assert exact, "sum differs from the oracle"
+    bytes_ = 8 * items + 8 * nnz               # operand values in, sums out
+    # plan bytes read per fill: operand masks + tile bases, or segments + item order
+    meta = nnz + 4 * nops * ((nnz + 255) // 256) if masked else 4 * (nnz + 1) + 4 * items
+    out_json = {
+        "metric": "Hessian-assembly sum entries/s (output entries filled per second)",
+        "value": nnz / (ms * 1e-3), "unit": "nnz-out/s", "n_gpus": 1, "steps": steps,
+        "warmup": max(1, args.warmup), "ms_per_step": ms, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": ("sum of 4 operands 1M x 1M, ~16 entries per row each, operands "
+                                "1-3 sharing half of operand 0's draws, FP64 N(0,1), seed 6; one "
+                                "step = one value fill of the union"),
+                   "operand_entries": items, "nnz_out": nnz, "plan_s": t_plan,
+                   "bit_exact_vs_oracle": exact},
+        "roofline": {"bound": "hbm",
+                     "kernel": "k_sum_fill_mask" if masked else "k_sum_fill_src<false>",
+                     "achieved": bytes_ / (ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": bytes_ / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                     "traffic": measured_traffic("k_sum_fill_mask_hessian_sum_traffic.json"),
+                     "alg_bytes_per_launch": bytes_, "plan_bytes_per_launch": meta},
+        "cpu_baseline": None if args.no_cpu else sum_cpu_baseline(ops, m, n, args.cpu_seconds),
+    }
+    lib.sdl_sum_plan_destroy(h)
+    print(json.dumps(out_json), flush=True)
+
+
+def sum_cpu_baseline(ops, m, n, seconds):
+    """The reference's own fill (memset, then accumulator over each operand
+    through the maps of sum_n_csr_alloc; matrix_sum.c:142-155) from
+    oracle/_ref on one core, whole workload, repeated for about `seconds`;
+    the oracle's fill when the reference build is absent."""
+    from oracle import oracle as O
+    nops = len(ops)
+    if O.ref_available():
+        r = O.ref_lib()
+        keep = []
+        views = [O._csr_view(m, n, p, i, x, keep) for p, i, x in ops]
+        flat = np.zeros(sum(v.nnz for v in views), np.int32)
+        parr = (C.POINTER(O._CSR) * nops)(*[C.pointer(v) for v in views])
+        Cc = r.sum_n_csr_alloc(parr, nops, O.I(flat))
+        nnz = Cc.contents.nnz
+        Cx = np.zeros(nnz, np.float64)
+        xs = [np.ascontiguousarray(x, np.float64) for _, _, x in ops]
+        offs = np.concatenate([[0], np.cumsum([v.nnz for v in views])])
+        maps = [np.ascontiguousarray(flat[offs[k]:offs[k + 1]]) for k in range(nops)]
+        fills, t = 0, 0.0
+        while t < seconds or fills == 0:
+            t0 = time.perf_counter()
+            Cx.fill(0.0)
+            for k in range(nops):
+                r.accumulator(O.D(xs[k]), views[k].nnz, O.I(maps[k]), O.D(Cx))
+            t += time.perf_counter() - t0
+            fills += 1
+        r.free_CSR_matrix(Cc)
+        kind, what = "reference", "oracle/_ref (the reference's accumulator fill)"
+    else:
+        t0 = time.perf_counter()
+        _, _, Cx, _ = O.sum_n(m, n, ops)
+        t, fills, nnz = time.perf_counter() - t0, 1, len(Cx)
+        kind, what = "port", "the oracle (structure and values)"
+    return {"value": nnz * fills / t, "unit": "nnz-out/s", "cores": 1, "kind": kind,
+            "sample": f"whole workload, {fills} fills in {t:.2f} s on one core, {what}"}
 
 
 def main_learned(args):

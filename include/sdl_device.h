@@ -153,6 +153,39 @@ const int *sdl_coo_plan_colind(const sdl_coo_plan *plan);
 int sdl_coo_fill(const sdl_coo_plan *plan, const double *dx, double *dCx, void *stream);
 void sdl_coo_plan_destroy(sdl_coo_plan *plan);
 
+/* ---- Sums of same-shape sparse matrices on the device ----
+   Device form of the reference's sum_n_matrices_alloc / _fill_values /
+   _fill_add (src/sparse/matrix_sum.c:112-180, CSR_sum.c:230-297): the output
+   holds the sorted union of the operands' columns per row, and every entry is
+   ((+0.0 + x_0) + x_1) + ... over the operands holding it, in operand order
+   (each operand's repeated entries in storage order).  dP[k] / dI[k]: device
+   row pointers (m+1) and columns (nnz[k]) of operand k; the pointer arrays are
+   host arrays.  keep_rows keeps each entry's row (4 bytes per operand entry)
+   for the row-scaled fill. */
+typedef struct sdl_sum_plan sdl_sum_plan;
+int sdl_sum_plan_create(int nops, int m, int n, const int *const *dP, const int *const *dI,
+                        const long long *nnz, int keep_rows, sdl_sum_plan **out);
+long long sdl_sum_plan_nnz(const sdl_sum_plan *plan);
+/* Device row pointer (int64, m+1) and column indices (nnz) of the output. */
+const long long *sdl_sum_plan_rowptr(const sdl_sum_plan *plan);
+const int *sdl_sum_plan_colind(const sdl_sum_plan *plan);
+/* Device int32 map: output position of every operand entry, operands back to
+   back (the reference's bound index maps, matrix_sum.c:90-110). */
+const int *sdl_sum_plan_map(const sdl_sum_plan *plan);
+/* 1 when the unscaled fill streams the operand values through per-entry operand
+   masks (at most 8 operands, each with sorted unique columns per row); 0 when
+   it reads them through the entry -> operand-entry lists. */
+int sdl_sum_plan_is_masked(const sdl_sum_plan *plan);
+/* dCx[nnz] = the sum; dX: host array of nops device value pointers.  dD (device,
+   nops x m, or NULL): row scales, each term the rounded product dD[k*m + row] * x
+   (sum_scaled_matrices_fill_values, CSR_sum.c:116-148, for any count).  Up to 32
+   operands the launch allocates and uploads nothing (graph-capturable). */
+int sdl_sum_fill(sdl_sum_plan *plan, const double *const *dX, const double *dD, double *dCx,
+                 void *stream);
+/* dCx += operand k (device values dx): the incremental fill_add. */
+int sdl_sum_add(sdl_sum_plan *plan, int k, const double *dx, double *dCx, void *stream);
+void sdl_sum_plan_destroy(sdl_sum_plan *plan);
+
 /* ---- HIP graphs of library calls ----
    Work issued on `stream` (NULL = library stream) between sdl_graph_begin
    and sdl_graph_end is captured, not run; sdl_graph_launch replays it.  Only

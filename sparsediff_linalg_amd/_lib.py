@@ -120,6 +120,16 @@ SIGNATURES = {
     "sdl_coo_plan_colind": (vp, [vp]),
     "sdl_coo_fill": (C.c_int, [vp, vp, vp, vp]),
     "sdl_coo_plan_destroy": (None, [vp]),
+    "sdl_sum_plan_create": (C.c_int, [C.c_int, C.c_int, C.c_int, C.POINTER(vp), C.POINTER(vp),
+                                      c_ll_p, C.c_int, C.POINTER(vp)]),
+    "sdl_sum_plan_nnz": (C.c_longlong, [vp]),
+    "sdl_sum_plan_rowptr": (vp, [vp]),
+    "sdl_sum_plan_colind": (vp, [vp]),
+    "sdl_sum_plan_map": (vp, [vp]),
+    "sdl_sum_plan_is_masked": (C.c_int, [vp]),
+    "sdl_sum_fill": (C.c_int, [vp, C.POINTER(vp), vp, vp, vp]),
+    "sdl_sum_add": (C.c_int, [vp, C.c_int, vp, vp, vp]),
+    "sdl_sum_plan_destroy": (None, [vp]),
     "sdl_graph_begin": (C.c_int, [vp]),
     "sdl_graph_end": (C.c_int, [vp, C.POINTER(vp)]),
     "sdl_graph_launch": (C.c_int, [vp, vp]),

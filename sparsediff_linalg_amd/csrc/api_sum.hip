@@ -15,6 +15,8 @@
 #include <cstdlib>
 #include <cstring>
 
+#include "../../include/sdl_device.h"
+
 extern "C" {
 #include "sparsediff_linalg/matrix_sum.h"
 }
@@ -47,34 +49,190 @@ __device__ __forceinline__ int op_of(const int64_t *__restrict__ off, int nops, 
 }
 
 // map[item] = output entry.
-__global__ void k_sum_map(int64_t nnz, const int64_t *__restrict__ seg,
+__global__ void k_sum_map(int64_t nnz, const int *__restrict__ seg,
                           const unsigned *__restrict__ order, int *__restrict__ map)
 {
     int64_t e = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= nnz) return;
-    for (int64_t s = seg[e]; s < seg[e + 1]; s++) map[order[s]] = (int) e;
+    for (int s = seg[e]; s < seg[e + 1]; s++) map[order[s]] = (int) e;
 }
 
 // dup[k] = 1 when two items of operand k land in one entry (repeated (row,
 // col) inside operand k).  Items of an entry are in ascending order, so
 // repeats of one operand are neighbours.
-__global__ void k_sum_dups(int64_t nnz, const int64_t *__restrict__ seg,
+__global__ void k_sum_dups(int64_t nnz, const int *__restrict__ seg,
                            const unsigned *__restrict__ order, const int64_t *__restrict__ off,
                            int nops, int *__restrict__ dup)
 {
     int64_t e = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= nnz) return;
-    for (int64_t s = seg[e] + 1; s < seg[e + 1]; s++)
+    for (int s = seg[e] + 1; s < seg[e + 1]; s++)
     {
         int k = op_of(off, nops, order[s]);
         if (k == op_of(off, nops, order[s - 1])) dup[k] = 1;
     }
 }
 
+// Operand value pointers and item offsets passed by value (up to SUM_ARGS
+// operands): no per-call upload, so the fill can be captured into a graph.
+constexpr int SUM_ARGS = 32;
+struct SumSrc
+{
+    const double *x[SUM_ARGS];
+    int64_t off[SUM_ARGS + 1];
+};
+
 // Cx[e] = ((+0.0 + v_1) + v_2) + ... over entry e's items in item order, with
 // v = x_k[t - off[k]] or, SCALED, the rounded product d[k*m + row[t]] * x.
+// The operand of an item comes from compares against the (wave-uniform)
+// offsets, so the only dependent loads are seg -> order -> value; up to four
+// items' value loads are in flight together.
 template <bool SCALED>
-__global__ void k_sum_fill(int64_t nnz, const int64_t *__restrict__ seg,
+__global__ __launch_bounds__(256) void k_sum_fill_src(int64_t nnz, const int *__restrict__ seg,
+                                                      const unsigned *__restrict__ order,
+                                                      const SumSrc src, int nops,
+                                                      const int *__restrict__ row,
+                                                      const double *__restrict__ d, int m,
+                                                      double *__restrict__ Cx)
+{
+    int64_t e = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nnz) return;
+    const int s1 = seg[e + 1];
+    double acc = 0.0;
+    for (int s = seg[e]; s < s1; s += 4)
+    {
+        const int cnt = s1 - s < 4 ? s1 - s : 4;
+        unsigned t[4];
+        double v[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) t[u] = u < cnt ? order[s + u] : 0u;
+#pragma unroll
+        for (int u = 0; u < 4; u++)
+        {
+            if (u < cnt)
+            {
+                const double *base = src.x[0];
+                int64_t o = 0;
+                int k = 0;
+                for (int j = 1; j < nops; j++)
+                {
+                    if ((int64_t) t[u] >= src.off[j])
+                    {
+                        base = src.x[j];
+                        o = src.off[j];
+                        k = j;
+                    }
+                }
+                v[u] = base[t[u] - o];
+                if (SCALED) v[u] = d[(int64_t) k * m + row[t[u]]] * v[u];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++)
+            if (u < cnt) acc = acc + v[u];
+    }
+    __builtin_nontemporal_store(acc, Cx + e);
+}
+
+// ---- operand-mask fill (up to SUM_MASK_OPS operands, sorted unique columns) --
+// When every operand's entries land in strictly increasing output entries,
+// operand k's value for output entry e is x_k[#(entries before e holding
+// operand k)]: the fill needs one mask byte per entry (bit k: operand k holds
+// it) and, per SUM_TILE-entry tile, each operand's count before the tile; the
+// in-tile counts come from wave ballots.  Plan bytes: 1 per entry instead of
+// 4 per entry + 4 per operand entry, and the value reads stream.
+constexpr int SUM_MASK_OPS = 8;
+constexpr int SUM_TILE = 256;
+
+__global__ void k_sum_masks(int64_t nnz, const int *__restrict__ seg,
+                            const unsigned *__restrict__ order, const int64_t *__restrict__ off,
+                            int nops, unsigned char *__restrict__ mask)
+{
+    int64_t e = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nnz) return;
+    unsigned mk = 0;
+    for (int s = seg[e]; s < seg[e + 1]; s++) mk |= 1u << op_of(off, nops, order[s]);
+    mask[e] = (unsigned char) mk;
+}
+
+// bad[k] = 1 when two consecutive entries of operand k do not land in strictly
+// increasing output entries (unsorted or repeated columns in a row).
+__global__ void k_sum_monotone(int64_t items, const int *__restrict__ map,
+                               const int64_t *__restrict__ off, int nops, int *__restrict__ bad)
+{
+    int64_t t = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= items) return;
+    const int k = op_of(off, nops, t);
+    if (t > off[k] && map[t] <= map[t - 1]) bad[k] = 1;
+}
+
+// base[k * ntiles + tile] = operand k's entries in output entries before
+// tile * SUM_TILE (first j with map_k[j] >= tile * SUM_TILE; map_k increases).
+__global__ void k_sum_tile_base(int ntiles, int nops, const int *__restrict__ map,
+                                const int64_t *__restrict__ off, int *__restrict__ base)
+{
+    int64_t idx = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (int64_t) ntiles * nops) return;
+    const int k = (int) (idx / ntiles), tile = (int) (idx % ntiles);
+    const int64_t e0 = (int64_t) tile * SUM_TILE;
+    int64_t lo = off[k], hi = off[k + 1];
+    while (lo < hi)
+    {
+        const int64_t mid = (lo + hi) >> 1;
+        if (map[mid] < e0) lo = mid + 1;
+        else hi = mid;
+    }
+    base[idx] = (int) (lo - off[k]);
+}
+
+// One workgroup per tile of SUM_TILE entries; Cx[e] = ((+0.0 + x_k0) + x_k1) +
+// ... over the operands in e's mask, ascending k.
+__global__ __launch_bounds__(SUM_TILE) void k_sum_fill_mask(int64_t nnz, int ntiles,
+                                                           const unsigned char *__restrict__ mask,
+                                                           const int *__restrict__ base,
+                                                           const SumSrc src, int nops,
+                                                           double *__restrict__ Cx)
+{
+    __shared__ int wtot[SUM_MASK_OPS][SUM_TILE / 64];
+    const int tile = blockIdx.x;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int64_t e = (int64_t) tile * SUM_TILE + threadIdx.x;
+    const unsigned mk = e < nnz ? mask[e] : 0u;
+    const unsigned long long lt = (1ull << lane) - 1ull;
+    int pos[SUM_MASK_OPS];
+#pragma unroll
+    for (int k = 0; k < SUM_MASK_OPS; k++)
+    {
+        if (k < nops)
+        {
+            const unsigned long long b = __ballot((mk >> k) & 1u);
+            pos[k] = __popcll(b & lt);
+            if (lane == 0) wtot[k][w] = __popcll(b);
+        }
+    }
+    __syncthreads();
+    double v[SUM_MASK_OPS];
+#pragma unroll
+    for (int k = 0; k < SUM_MASK_OPS; k++)
+    {
+        if (k < nops && ((mk >> k) & 1u))
+        {
+            int p = base[(int64_t) k * ntiles + tile] + pos[k];
+            for (int w2 = 0; w2 < w; w2++) p += wtot[k][w2];
+            v[k] = src.x[k][p];
+        }
+    }
+    if (e >= nnz) return;
+    double acc = 0.0;
+#pragma unroll
+    for (int k = 0; k < SUM_MASK_OPS; k++)
+        if (k < nops && ((mk >> k) & 1u)) acc = acc + v[k];
+    __builtin_nontemporal_store(acc, Cx + e);
+}
+
+// The same fill with the pointers and offsets in device memory (any count).
+template <bool SCALED>
+__global__ void k_sum_fill(int64_t nnz, const int *__restrict__ seg,
                            const unsigned *__restrict__ order, const int64_t *__restrict__ off,
                            int nops, const double *const *__restrict__ xs,
                            const int *__restrict__ row, const double *__restrict__ d, int m,
@@ -83,7 +241,7 @@ __global__ void k_sum_fill(int64_t nnz, const int64_t *__restrict__ seg,
     int64_t e = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= nnz) return;
     double acc = 0.0;
-    for (int64_t s = seg[e]; s < seg[e + 1]; s++)
+    for (int s = seg[e]; s < seg[e + 1]; s++)
     {
         const unsigned t = order[s];
         const int k = op_of(off, nops, t);
@@ -105,7 +263,7 @@ __global__ void k_sum_add(int64_t cnt, const int *__restrict__ map, const double
 
 // The same add for an operand with repeated entries: each entry adds its items
 // of operand [lo, hi) in item order.
-__global__ void k_sum_add_ordered(int64_t nnz, const int64_t *__restrict__ seg,
+__global__ void k_sum_add_ordered(int64_t nnz, const int *__restrict__ seg,
                                   const unsigned *__restrict__ order, int64_t lo, int64_t hi,
                                   const double *__restrict__ x, double *__restrict__ Cx)
 {
@@ -113,7 +271,7 @@ __global__ void k_sum_add_ordered(int64_t nnz, const int64_t *__restrict__ seg,
     if (e >= nnz) return;
     double acc = Cx[e];
     bool hit = false;
-    for (int64_t s = seg[e]; s < seg[e + 1]; s++)
+    for (int s = seg[e]; s < seg[e + 1]; s++)
     {
         const int64_t t = order[s];
         if (t >= lo && t < hi)
@@ -125,67 +283,168 @@ __global__ void k_sum_add_ordered(int64_t nnz, const int64_t *__restrict__ seg,
     if (hit) Cx[e] = acc;
 }
 
-struct SumBound : Bound
+// Union plan of nops same-shape operands (device CSR structures).
+struct SumPlan
 {
-    int nops = 0, m = 0;
-    CoalescePlan P;
+    int nops = 0, m = 0, n = 0;
+    CoalescePlan P;                 // structure; its int64 segments are narrowed into seg
+    DBuf<int> seg;                  // nnz + 1: items of entry e are order[seg[e] .. seg[e+1])
     std::vector<int64_t> off;       // nops + 1 item offsets
     DBuf<int64_t> doff;
     DBuf<int> map;                  // item -> output entry
-    DBuf<int> row;                  // item -> row (two-operand sums: the scaled fill)
+    DBuf<int> row;                  // item -> row (kept for the scaled fill)
     std::vector<int> dup;           // operand k repeats an entry
+    bool masked = false;            // the operand-mask fill applies
+    int ntiles = 0;
+    DBuf<unsigned char> mask;       // nnz operand masks (masked plans)
+    DBuf<int> tile_base;            // nops x ntiles (masked plans)
+    DBuf<const double *> xs;        // value pointers beyond SUM_ARGS operands
+};
+
+void sum_plan_build(SumPlan &S, int nops, int m, int n, const int *const *dP,
+                    const int *const *dI, const int64_t *nnz, bool keep_rows)
+{
+    S.nops = nops;
+    S.m = m;
+    S.n = n;
+    S.off.assign((size_t) nops + 1, 0);
+    for (int k = 0; k < nops; k++) S.off[k + 1] = S.off[k] + nnz[k];
+    const int64_t items = S.off[nops];
+    DBuf<int> orow(items), ocol(items);
+    for (int k = 0; k < nops; k++)
+    {
+        if (nnz[k] == 0) continue;
+        expand_rows(m, dP[k], orow.p + S.off[k], stream());
+        d2d(ocol.p + S.off[k], dI[k], (size_t) nnz[k] * sizeof(int));
+    }
+    coalesce_plan(S.P, items, orow.p, ocol.p, m, n);
+    S.seg.alloc((size_t) S.P.nnz + 1);
+    narrow_i64(S.P.nnz + 1, S.P.seg.p, S.seg.p, stream());
+    S.doff = upload(S.off.data(), nops + 1);
+    S.map.alloc(items);
+    DBuf<int> ddup(nops);
+    dzero(ddup.p, ddup.bytes());
+    if (S.P.nnz > 0)
+    {
+        hipLaunchKernelGGL(k_sum_map, dim3(sum_grid(S.P.nnz)), dim3(256), 0, stream(), S.P.nnz,
+                           S.seg.p, S.P.order.p, S.map.p);
+        hipLaunchKernelGGL(k_sum_dups, dim3(sum_grid(S.P.nnz)), dim3(256), 0, stream(), S.P.nnz,
+                           S.seg.p, S.P.order.p, S.doff.p, nops, ddup.p);
+    }
+    SDL_HIP(hipGetLastError());
+    S.dup.assign(nops, 0);
+    d2h(S.dup.data(), ddup.p, (size_t) nops * sizeof(int));
+    S.P.seg.reset();
+    if (nops <= SUM_MASK_OPS && S.P.nnz > 0)
+    {
+        DBuf<int> dbad(nops);
+        dzero(dbad.p, dbad.bytes());
+        hipLaunchKernelGGL(k_sum_monotone, dim3(sum_grid(items)), dim3(256), 0, stream(), items,
+                           S.map.p, S.doff.p, nops, dbad.p);
+        SDL_HIP(hipGetLastError());
+        std::vector<int> bad(nops, 0);
+        d2h(bad.data(), dbad.p, (size_t) nops * sizeof(int));
+        S.masked = true;
+        for (int k = 0; k < nops; k++) S.masked = S.masked && !bad[k];
+    }
+    if (S.masked)
+    {
+        S.ntiles = (int) ((S.P.nnz + SUM_TILE - 1) / SUM_TILE);
+        S.mask.alloc((size_t) S.P.nnz);
+        S.tile_base.alloc((size_t) S.ntiles * nops);
+        hipLaunchKernelGGL(k_sum_masks, dim3(sum_grid(S.P.nnz)), dim3(256), 0, stream(), S.P.nnz,
+                           S.seg.p, S.P.order.p, S.doff.p, nops, S.mask.p);
+        hipLaunchKernelGGL(k_sum_tile_base, dim3(sum_grid((int64_t) S.ntiles * nops)), dim3(256),
+                           0, stream(), S.ntiles, nops, S.map.p, S.doff.p, S.tile_base.p);
+        SDL_HIP(hipGetLastError());
+    }
+    if (keep_rows) S.row = std::move(orow);
+    if (nops > SUM_ARGS) S.xs.alloc(nops);
+}
+
+// dCx = the sum of the operands whose device values are hX[k] (host array of
+// device pointers); dd (nops x m row scales, device) gives the scaled sum.
+// Up to SUM_ARGS operands the launch carries the pointers and allocates and
+// uploads nothing.
+void sum_plan_fill(SumPlan &S, const double *const *hX, const double *dd, double *dCx,
+                   hipStream_t st)
+{
+    if (S.P.nnz <= 0) return;
+    const unsigned grid = sum_grid(S.P.nnz);
+    if (S.nops <= SUM_ARGS)
+    {
+        SumSrc src{};
+        for (int k = 0; k < S.nops; k++) src.x[k] = hX[k];
+        for (int k = 0; k <= S.nops; k++) src.off[k] = S.off[k];
+        if (S.masked && !dd)
+            hipLaunchKernelGGL(k_sum_fill_mask, dim3(S.ntiles), dim3(SUM_TILE), 0, st, S.P.nnz,
+                               S.ntiles, S.mask.p, S.tile_base.p, src, S.nops, dCx);
+        else if (dd)
+            hipLaunchKernelGGL(k_sum_fill_src<true>, dim3(grid), dim3(256), 0, st, S.P.nnz,
+                               S.seg.p, S.P.order.p, src, S.nops, S.row.p, dd, S.m, dCx);
+        else
+            hipLaunchKernelGGL(k_sum_fill_src<false>, dim3(grid), dim3(256), 0, st, S.P.nnz,
+                               S.seg.p, S.P.order.p, src, S.nops, (const int *) nullptr,
+                               (const double *) nullptr, S.m, dCx);
+    }
+    else
+    {
+        h2d(S.xs.p, hX, (size_t) S.nops * sizeof(const double *));
+        if (dd)
+            hipLaunchKernelGGL(k_sum_fill<true>, dim3(grid), dim3(256), 0, st, S.P.nnz, S.seg.p,
+                               S.P.order.p, S.doff.p, S.nops, S.xs.p, S.row.p, dd, S.m, dCx);
+        else
+            hipLaunchKernelGGL(k_sum_fill<false>, dim3(grid), dim3(256), 0, st, S.P.nnz,
+                               S.seg.p, S.P.order.p, S.doff.p, S.nops, S.xs.p,
+                               (const int *) nullptr, (const double *) nullptr, S.m, dCx);
+    }
+    SDL_HIP(hipGetLastError());
+}
+
+// dCx += operand k (device values x), in the reference's accumulator order.
+void sum_plan_add(SumPlan &S, int k, const double *x, double *dCx, hipStream_t st)
+{
+    const int64_t lo = S.off[k], hi = S.off[k + 1];
+    if (hi <= lo) return;
+    if (!S.dup[k])
+        hipLaunchKernelGGL(k_sum_add, dim3(sum_grid(hi - lo)), dim3(256), 0, st, hi - lo,
+                           S.map.p + lo, x, dCx);
+    else
+        hipLaunchKernelGGL(k_sum_add_ordered, dim3(sum_grid(S.P.nnz)), dim3(256), 0, st, S.P.nnz,
+                           S.seg.p, S.P.order.p, lo, hi, x, dCx);
+    SDL_HIP(hipGetLastError());
+}
+
+struct SumBound : Bound
+{
+    SumPlan S;
     std::vector<const double *> hxs;
-    DBuf<const double *> xs;        // operand value pointers (fill-time upload target)
     DBuf<double> d;                 // 2 x m row scales (scaled fill)
 };
 
-// Coalescing plan of the operands' concatenated entries.
+// Plan over the operands' CSR views (any kinds).
 std::unique_ptr<SumBound> sum_plan(const char *fn, matrix *const *ops, int nops, bool keep_rows)
 {
     if (nops < 1) api_exit(fn, "need at least one operand");
     const int m = ops[0]->m, n = ops[0]->n;
-    auto b = std::make_unique<SumBound>();
-    b->nops = nops;
-    b->m = m;
-    b->off.assign((size_t) nops + 1, 0);
+    std::vector<const int *> dP(nops), dI(nops);
+    std::vector<int64_t> nnz(nops);
+    int64_t items = 0;
     for (int k = 0; k < nops; k++)
     {
         if (ops[k]->m != m || ops[k]->n != n) api_exit(fn, "operand shapes differ");
-        b->off[k + 1] = b->off[k] + ops[k]->nnz;
-    }
-    const int64_t items = b->off[nops];
-    if (items > INT_MAX) api_exit(fn, "total operand nnz exceeds INT_MAX.");
-    DBuf<int> orow(items), ocol(items);
-    for (int k = 0; k < nops; k++)
-    {
         SparseM *S = as_sparse(pd_csr_operand(ops[k]));
         sparse_ensure_dev(S);
-        if (ops[k]->nnz == 0) continue;
-        expand_rows(m, S->dp.p, orow.p + b->off[k], stream());
-        d2d(ocol.p + b->off[k], S->di.p, (size_t) ops[k]->nnz * sizeof(int));
+        dP[k] = S->dp.p;
+        dI[k] = S->di.p;
+        nnz[k] = ops[k]->nnz;
+        items += nnz[k];
     }
-    coalesce_plan(b->P, items, orow.p, ocol.p, m, n);
-    b->doff = upload(b->off.data(), nops + 1);
-    b->map.alloc(items);
-    DBuf<int> ddup(nops);
-    dzero(ddup.p, ddup.bytes());
-    if (b->P.nnz > 0)
-    {
-        hipLaunchKernelGGL(k_sum_map, dim3(sum_grid(b->P.nnz)), dim3(256), 0, stream(), b->P.nnz,
-                           b->P.seg.p, b->P.order.p, b->map.p);
-        hipLaunchKernelGGL(k_sum_dups, dim3(sum_grid(b->P.nnz)), dim3(256), 0, stream(),
-                           b->P.nnz, b->P.seg.p, b->P.order.p, b->doff.p, nops, ddup.p);
-    }
-    SDL_HIP(hipGetLastError());
-    b->dup.assign(nops, 0);
-    d2h(b->dup.data(), ddup.p, (size_t) nops * sizeof(int));
-    if (keep_rows)
-    {
-        b->row = std::move(orow);
-        b->d.alloc((size_t) 2 * m);
-    }
+    if (items > INT_MAX) api_exit(fn, "total operand nnz exceeds INT_MAX.");
+    auto b = std::make_unique<SumBound>();
+    sum_plan_build(b->S, nops, m, n, dP.data(), dI.data(), nnz.data(), keep_rows);
+    if (keep_rows) b->d.alloc((size_t) 2 * m);
     b->hxs.assign(nops, nullptr);
-    b->xs.alloc(nops);
     return b;
 }
 
@@ -201,9 +460,9 @@ void write_structure(CSR_matrix *c, const CoalescePlan &P)
 
 matrix *sum_output(const char *fn, std::unique_ptr<SumBound> b, int m, int n)
 {
-    const int nnz = checked_nnz(fn, b->P.nnz);
+    const int nnz = checked_nnz(fn, b->S.P.nnz);
     CSR_matrix *c = new_CSR_matrix(m, n, nnz);
-    write_structure(c, b->P);
+    write_structure(c, b->S.P);
     SparseM *C = as_sparse(make_sparse(c));
     sparse_ensure_dev(C);
     C->bound = std::move(b);
@@ -212,9 +471,9 @@ matrix *sum_output(const char *fn, std::unique_ptr<SumBound> b, int m, int n)
 
 void check_operands(const char *fn, const SumBound *b, const matrix *const *ops, int nops)
 {
-    if (b->nops != nops) api_exit(fn, "operand count differs from the alloc");
+    if (b->S.nops != nops) api_exit(fn, "operand count differs from the alloc");
     for (int k = 0; k < nops; k++)
-        if (ops[k]->nnz != b->off[k + 1] - b->off[k])
+        if (ops[k]->nnz != b->S.off[k + 1] - b->S.off[k])
             api_exit(fn, "operand nnz differs from the alloc");
 }
 
@@ -227,27 +486,15 @@ void sum_fill(const char *fn, matrix *const *ops, int nops, matrix *Cm, const do
     for (int k = 0; k < nops; k++)
         b->hxs[k] = ops[k]->nnz > 0 ? sparse_dev_values(as_sparse(pd_csr_operand_fresh(ops[k])))
                                     : nullptr;
-    h2d(b->xs.p, b->hxs.data(), (size_t) nops * sizeof(const double *));
-    if (b->P.nnz > 0)
+    const double *dd = nullptr;
+    if (d1)
     {
-        if (d1)
-        {
-            if (!b->row.p) api_exit(fn, "output was not produced by sum_matrices_alloc");
-            h2d(b->d.p, d1, (size_t) b->m * sizeof(double));
-            h2d(b->d.p + b->m, d2, (size_t) b->m * sizeof(double));
-            hipLaunchKernelGGL(k_sum_fill<true>, dim3(sum_grid(b->P.nnz)), dim3(256), 0, stream(),
-                               b->P.nnz, b->P.seg.p, b->P.order.p, b->doff.p, nops, b->xs.p,
-                               b->row.p, b->d.p, b->m, C->val.x.p);
-        }
-        else
-        {
-            hipLaunchKernelGGL(k_sum_fill<false>, dim3(sum_grid(b->P.nnz)), dim3(256), 0,
-                               stream(), b->P.nnz, b->P.seg.p, b->P.order.p, b->doff.p, nops,
-                               b->xs.p, (const int *) nullptr, (const double *) nullptr, b->m,
-                               C->val.x.p);
-        }
-        SDL_HIP(hipGetLastError());
+        if (!b->S.row.p) api_exit(fn, "output was not produced by sum_matrices_alloc");
+        h2d(b->d.p, d1, (size_t) b->S.m * sizeof(double));
+        h2d(b->d.p + b->S.m, d2, (size_t) b->S.m * sizeof(double));
+        dd = b->d.p;
     }
+    sum_plan_fill(b->S, b->hxs.data(), dd, C->val.x.p, stream());
     sparse_publish(C);
 }
 
@@ -256,6 +503,11 @@ void sum_fill(const char *fn, matrix *const *ops, int nops, matrix *Cm, const do
 } // namespace sdl
 
 using namespace sdl;
+
+struct sdl_sum_plan
+{
+    SumPlan S;
+};
 
 extern "C" {
 #pragma GCC visibility push(default)
@@ -269,7 +521,7 @@ void sum_matrices_alloc(matrix *A, matrix *B, matrix *Cm)
     if (Cm->m != A->m || Cm->n != A->n) api_exit(fn, "C's shape differs from the operands'");
     matrix *ops[2] = {A, B};
     std::unique_ptr<SumBound> b = sum_plan(fn, ops, 2, true);
-    const int nnz = checked_nnz(fn, b->P.nnz);
+    const int nnz = checked_nnz(fn, b->S.P.nnz);
     // C's CSR keeps its struct and row-pointer array; column and value arrays
     // are sized to the union (the reference's CSR_trim, CSR_matrix.c:56-63).
     CSR_matrix *c = C->csr;
@@ -280,7 +532,7 @@ void sum_matrices_alloc(matrix *A, matrix *B, matrix *Cm)
     if (!ni || !nx) api_exit(fn, "host allocation failed");
     c->i = ni;
     c->x = nx;
-    write_structure(c, b->P);
+    write_structure(c, b->S.P);
     Cm->x = c->x;
     Cm->nnz = nnz;
     // the device mirror follows the new structure
@@ -358,24 +610,93 @@ void sum_n_matrices_fill_add(const matrix *op, int k, matrix *Cm)
     const char *fn = "sum_n_matrices_fill_add";
     SparseM *C = as_sparse(Cm);
     SumBound *b = bound_as<SumBound>(C, fn);
-    if (k < 0 || k >= b->nops) api_exit(fn, "operand index out of range");
-    const int64_t lo = b->off[k], hi = b->off[k + 1];
-    if (op->nnz != hi - lo) api_exit(fn, "operand nnz differs from the alloc");
-    if (hi > lo)
+    if (k < 0 || k >= b->S.nops) api_exit(fn, "operand index out of range");
+    if (op->nnz != b->S.off[k + 1] - b->S.off[k]) api_exit(fn, "operand nnz differs from the alloc");
+    if (op->nnz > 0)
     {
         // C's device values current (the caller may have written C->x)
         double *cx = const_cast<double *>(sparse_dev_values(C));
         const double *x =
             sparse_dev_values(as_sparse(pd_csr_operand_fresh(const_cast<matrix *>(op))));
-        if (!b->dup[k])
-            hipLaunchKernelGGL(k_sum_add, dim3(sum_grid(hi - lo)), dim3(256), 0, stream(),
-                               hi - lo, b->map.p + lo, x, cx);
-        else
-            hipLaunchKernelGGL(k_sum_add_ordered, dim3(sum_grid(b->P.nnz)), dim3(256), 0,
-                               stream(), b->P.nnz, b->P.seg.p, b->P.order.p, lo, hi, x, cx);
-        SDL_HIP(hipGetLastError());
+        sum_plan_add(b->S, k, x, cx, stream());
     }
     sparse_publish(C);
+}
+
+// ---- device-resident sums ----
+int sdl_sum_plan_create(int nops, int m, int n, const int *const *dP, const int *const *dI,
+                        const long long *nnz, int keep_rows, sdl_sum_plan **out)
+{
+    if (!out || nops < 1 || m < 0 || n < 0 || !dP || !dI || !nnz)
+    {
+        set_error("sdl_sum_plan_create: invalid arguments");
+        return 1;
+    }
+    long long items = 0;
+    for (int k = 0; k < nops; k++)
+    {
+        if (nnz[k] < 0 || (nnz[k] > 0 && (!dP[k] || !dI[k])) || (m > 0 && !dP[k]))
+        {
+            set_error("sdl_sum_plan_create: operand %d: invalid arguments", k);
+            return 1;
+        }
+        items += nnz[k];
+    }
+    if (items > INT_MAX)
+    {
+        set_error("sdl_sum_plan_create: total operand nnz exceeds INT_MAX");
+        return 1;
+    }
+    sdl_sum_plan *p = new (std::nothrow) sdl_sum_plan();
+    if (!p) return 1;
+    std::vector<int64_t> nz(nnz, nnz + nops);
+    sum_plan_build(p->S, nops, m, n, dP, dI, nz.data(), keep_rows != 0);
+    *out = p;
+    return 0;
+}
+
+long long sdl_sum_plan_nnz(const sdl_sum_plan *p) { return p ? p->S.P.nnz : -1; }
+const long long *sdl_sum_plan_rowptr(const sdl_sum_plan *p)
+{
+    return p ? (const long long *) p->S.P.Cp.p : nullptr;
+}
+const int *sdl_sum_plan_colind(const sdl_sum_plan *p) { return p ? p->S.P.Ci.p : nullptr; }
+const int *sdl_sum_plan_map(const sdl_sum_plan *p) { return p ? p->S.map.p : nullptr; }
+int sdl_sum_plan_is_masked(const sdl_sum_plan *p) { return p && p->S.masked ? 1 : 0; }
+
+int sdl_sum_fill(sdl_sum_plan *p, const double *const *dX, const double *dD, double *dCx,
+                 void *stream)
+{
+    if (!p || (p->S.P.nnz > 0 && (!dX || !dCx)))
+    {
+        set_error("sdl_sum_fill: invalid arguments");
+        return 1;
+    }
+    if (dD && !p->S.row.p)
+    {
+        set_error("sdl_sum_fill: row scales need a plan created with keep_rows");
+        return 1;
+    }
+    sum_plan_fill(p->S, dX, dD, dCx, stream ? (hipStream_t) stream : sdl::stream());
+    return 0;
+}
+
+int sdl_sum_add(sdl_sum_plan *p, int k, const double *dx, double *dCx, void *stream)
+{
+    if (!p || k < 0 || k >= p->S.nops)
+    {
+        set_error("sdl_sum_add: invalid arguments");
+        return 1;
+    }
+    sum_plan_add(p->S, k, dx, dCx, stream ? (hipStream_t) stream : sdl::stream());
+    return 0;
+}
+
+void sdl_sum_plan_destroy(sdl_sum_plan *p)
+{
+    if (!p) return;
+    sdl::sync();
+    delete p;
 }
 
 #pragma GCC visibility pop

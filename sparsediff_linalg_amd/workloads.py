@@ -248,3 +248,28 @@ def coo_triplets(seed: int = 2, n_items: int = 50_000_000, unique: int = 35_000_
     cols = (allk % n).astype(np.int32)
     x = normal(seed * 1000 + 11, n_items)
     return rows, cols, x
+
+
+def sum_operands(seed: int = 6, m: int = 1_000_000, n: int = 1_000_000, nops: int = 4,
+                 per_row: int = 16, shared: float = 0.5):
+    """Operands of a Hessian-assembly sum: nops m x n CSR matrices with about
+    per_row entries per row.  Operand 0 draws its columns uniformly; every
+    later operand keeps each of operand 0's draws with probability `shared`
+    and draws the others afresh (repeated draws in a row collapse).  Values
+    N(0,1).  Returns [(p int32, i int32, x float64), ...]."""
+    g = np.random.Generator(np.random.PCG64(seed))
+    rows = np.repeat(np.arange(m, dtype=np.int64), per_row)
+    base = rows * n + g.integers(0, n, m * per_row, dtype=np.int64)
+    ops = []
+    for k in range(nops):
+        if k == 0:
+            keys = base
+        else:
+            keep = g.random(m * per_row) < shared
+            keys = np.where(keep, base, rows * n + g.integers(0, n, m * per_row, dtype=np.int64))
+        keys = np.unique(keys)
+        p = np.zeros(m + 1, np.int64)
+        np.cumsum(np.bincount(keys // n, minlength=m), out=p[1:])
+        i = (keys % n).astype(np.int32)
+        ops.append((p.astype(np.int32), i, normal(seed * 100 + k, len(i))))
+    return ops
