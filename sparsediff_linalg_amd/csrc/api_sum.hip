@@ -138,11 +138,12 @@ __global__ __launch_bounds__(256) void k_sum_fill_src(int64_t nnz, const int *__
 // When every operand's entries land in strictly increasing output entries,
 // operand k's value for output entry e is x_k[#(entries before e holding
 // operand k)]: the fill needs one mask byte per entry (bit k: operand k holds
-// it) and, per SUM_TILE-entry tile, each operand's count before the tile; the
-// in-tile counts come from wave ballots.  Plan bytes: 1 per entry instead of
+// it) and, per SUM_TILE-entry tile (one wave's), each operand's count before
+// the tile; the in-tile counts come from wave ballots.  Plan bytes: 1 per entry instead of
 // 4 per entry + 4 per operand entry, and the value reads stream.
 constexpr int SUM_MASK_OPS = 8;
-constexpr int SUM_TILE = 256;
+constexpr int SUM_PER = 4;                  // entries per lane
+constexpr int SUM_TILE = 64 * SUM_PER;      // entries per wave
 
 __global__ void k_sum_masks(int64_t nnz, const int *__restrict__ seg,
                             const unsigned *__restrict__ order, const int64_t *__restrict__ off,
@@ -185,49 +186,64 @@ __global__ void k_sum_tile_base(int ntiles, int nops, const int *__restrict__ ma
     base[idx] = (int) (lo - off[k]);
 }
 
-// One workgroup per tile of SUM_TILE entries; Cx[e] = ((+0.0 + x_k0) + x_k1) +
-// ... over the operands in e's mask, ascending k.
-__global__ __launch_bounds__(SUM_TILE) void k_sum_fill_mask(int64_t nnz, int ntiles,
-                                                           const unsigned char *__restrict__ mask,
-                                                           const int *__restrict__ base,
-                                                           const SumSrc src, int nops,
-                                                           double *__restrict__ Cx)
+// One wave per tile of SUM_TILE entries: SUM_PER rows of 64, lane i holding
+// entry i of each (every load and store coalesced), no LDS and no barrier.
+// Cx[e] = ((+0.0 + x_k0) + x_k1) + ... over the operands in e's mask, ascending
+// k.  NOPS is the operand count, so the loops have no branches and every value
+// load is issued unconditionally (a lane without operand k reads the mask
+// buffer's first word instead and drops it): all SUM_PER * NOPS loads of a
+// lane are in flight together.
+template <int NOPS>
+__global__ __launch_bounds__(256) void k_sum_fill_mask(int64_t nnz, int ntiles,
+                                                      const unsigned char *__restrict__ mask,
+                                                      const int *__restrict__ base,
+                                                      const SumSrc src,
+                                                      double *__restrict__ Cx)
 {
-    __shared__ int wtot[SUM_MASK_OPS][SUM_TILE / 64];
-    const int tile = blockIdx.x;
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int64_t e = (int64_t) tile * SUM_TILE + threadIdx.x;
-    const unsigned mk = e < nnz ? mask[e] : 0u;
+    const int lane = threadIdx.x & 63;
+    const int tile = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (threadIdx.x >> 6));
+    if (tile >= ntiles) return;  // whole waves
+    const int64_t t0 = (int64_t) tile * SUM_TILE + lane;
     const unsigned long long lt = (1ull << lane) - 1ull;
-    int pos[SUM_MASK_OPS];
+    const double *dummy = (const double *) mask;
+    unsigned mk[SUM_PER];
 #pragma unroll
-    for (int k = 0; k < SUM_MASK_OPS; k++)
+    for (int u = 0; u < SUM_PER; u++) mk[u] = mask[t0 + 64 * u];  // padded to whole tiles
+    int run[NOPS];
+#pragma unroll
+    for (int k = 0; k < NOPS; k++) run[k] = base[(int64_t) k * ntiles + tile];
+    double v[SUM_PER][NOPS];
+#pragma unroll
+    for (int u = 0; u < SUM_PER; u++)
     {
-        if (k < nops)
+#pragma unroll
+        for (int k = 0; k < NOPS; k++)
         {
-            const unsigned long long b = __ballot((mk >> k) & 1u);
-            pos[k] = __popcll(b & lt);
-            if (lane == 0) wtot[k][w] = __popcll(b);
+            const bool hit = (mk[u] >> k) & 1u;
+            const unsigned long long b = __ballot(hit);
+            const double *a = hit ? src.x[k] + run[k] + __popcll(b & lt) : dummy;
+            v[u][k] = *a;
+            run[k] += __popcll(b);
         }
     }
-    __syncthreads();
-    double v[SUM_MASK_OPS];
 #pragma unroll
-    for (int k = 0; k < SUM_MASK_OPS; k++)
+    for (int u = 0; u < SUM_PER; u++)
     {
-        if (k < nops && ((mk >> k) & 1u))
-        {
-            int p = base[(int64_t) k * ntiles + tile] + pos[k];
-            for (int w2 = 0; w2 < w; w2++) p += wtot[k][w2];
-            v[k] = src.x[k][p];
-        }
-    }
-    if (e >= nnz) return;
-    double acc = 0.0;
+        double acc = 0.0;
 #pragma unroll
-    for (int k = 0; k < SUM_MASK_OPS; k++)
-        if (k < nops && ((mk >> k) & 1u)) acc = acc + v[k];
-    __builtin_nontemporal_store(acc, Cx + e);
+        for (int k = 0; k < NOPS; k++)
+            if ((mk[u] >> k) & 1u) acc = acc + v[u][k];
+        const int64_t e = t0 + 64 * u;
+        if (e < nnz) __builtin_nontemporal_store(acc, Cx + e);
+    }
+}
+
+template <int NOPS>
+void launch_fill_mask(unsigned blocks, hipStream_t st, int64_t nnz, int ntiles,
+                      const unsigned char *mask, const int *base, const SumSrc &src, double *Cx)
+{
+    hipLaunchKernelGGL(k_sum_fill_mask<NOPS>, dim3(blocks), dim3(256), 0, st, nnz, ntiles, mask,
+                       base, src, Cx);
 }
 
 // The same fill with the pointers and offsets in device memory (any count).
@@ -350,7 +366,9 @@ void sum_plan_build(SumPlan &S, int nops, int m, int n, const int *const *dP,
     if (S.masked)
     {
         S.ntiles = (int) ((S.P.nnz + SUM_TILE - 1) / SUM_TILE);
-        S.mask.alloc((size_t) S.P.nnz);
+        // padded to whole tiles (zero masks), read as 32-bit words
+        S.mask.alloc((size_t) S.ntiles * SUM_TILE);
+        dzero(S.mask.p, S.mask.bytes());
         S.tile_base.alloc((size_t) S.ntiles * nops);
         hipLaunchKernelGGL(k_sum_masks, dim3(sum_grid(S.P.nnz)), dim3(256), 0, stream(), S.P.nnz,
                            S.seg.p, S.P.order.p, S.doff.p, nops, S.mask.p);
@@ -377,8 +395,22 @@ void sum_plan_fill(SumPlan &S, const double *const *hX, const double *dd, double
         for (int k = 0; k < S.nops; k++) src.x[k] = hX[k];
         for (int k = 0; k <= S.nops; k++) src.off[k] = S.off[k];
         if (S.masked && !dd)
-            hipLaunchKernelGGL(k_sum_fill_mask, dim3(S.ntiles), dim3(SUM_TILE), 0, st, S.P.nnz,
-                               S.ntiles, S.mask.p, S.tile_base.p, src, S.nops, dCx);
+        {
+            const unsigned blocks = (unsigned) ((S.ntiles + 3) / 4);
+            auto *fn = &launch_fill_mask<1>;
+            switch (S.nops)
+            {
+            case 2: fn = &launch_fill_mask<2>; break;
+            case 3: fn = &launch_fill_mask<3>; break;
+            case 4: fn = &launch_fill_mask<4>; break;
+            case 5: fn = &launch_fill_mask<5>; break;
+            case 6: fn = &launch_fill_mask<6>; break;
+            case 7: fn = &launch_fill_mask<7>; break;
+            case 8: fn = &launch_fill_mask<8>; break;
+            default: break;
+            }
+            fn(blocks, st, S.P.nnz, S.ntiles, S.mask.p, S.tile_base.p, src, dCx);
+        }
         else if (dd)
             hipLaunchKernelGGL(k_sum_fill_src<true>, dim3(grid), dim3(256), 0, st, S.P.nnz,
                                S.seg.p, S.P.order.p, src, S.nops, S.row.p, dd, S.m, dCx);
