@@ -186,6 +186,16 @@ int sdl_sum_fill(sdl_sum_plan *plan, const double *const *dX, const double *dD, 
 int sdl_sum_add(sdl_sum_plan *plan, int k, const double *dx, double *dCx, void *stream);
 void sdl_sum_plan_destroy(sdl_sum_plan *plan);
 
+/* ---- Dense Kronecker-vector products on device arrays ----
+   Device forms of the reference's Y_kron_I_vec / I_kron_XT_vec
+   (src/support/mini_numpy.c:74-105): same index conventions, every output a
+   left-to-right sum from +0.0 of individually rounded products.  No
+   allocation; stream NULL = library stream. */
+int sdl_y_kron_i_vec(int m, int k, int n, const double *dY, const double *dw, double *dv,
+                     void *stream);
+int sdl_i_kron_xt_vec(int m, int k, int n, const double *dX, const double *dw, double *dv,
+                      void *stream);
+
 /* ---- HIP graphs of library calls ----
    Work issued on `stream` (NULL = library stream) between sdl_graph_begin
    and sdl_graph_end is captured, not run; sdl_graph_launch replays it.  Only

@@ -15,7 +15,8 @@ fi
 mkdir -p "$OUT"
 SRCS="$REF/src/sparse/gustavson.c $REF/src/sparse/CSC_matrix.c $REF/src/sparse/CSR_matrix.c \
       $REF/src/sparse/COO_matrix.c $REF/src/sparse/CSR_sum.c $REF/src/sparse/sparse_matrix.c \
-      $REF/src/sparse/linalg_sparse_matmuls.c $REF/src/support/utils.c $REF/src/support/tracked_alloc.c"
+      $REF/src/sparse/linalg_sparse_matmuls.c $REF/src/support/utils.c $REF/src/support/tracked_alloc.c \
+      $REF/src/support/mini_numpy.c"
 gcc -std=c99 -O3 -DNDEBUG -fPIC -shared -Wl,-Bsymbolic -I"$REF/include" -I"$REF/src" $SRCS \
     -o "$OUT/libref_sparse.so.tmp"
 mv "$OUT/libref_sparse.so.tmp" "$OUT/libref_sparse.so"

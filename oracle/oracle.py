@@ -38,6 +38,8 @@ _SIGS = {
     "orc_da": (None, [C.c_int, ip, dp, dp, dp]),
     "orc_kron_spmv": (None, [C.c_int, C.c_int, ip, ip, dp, dp, dp, C.c_int]),
     "orc_coo_lower": (C.c_int, [C.c_int, ip, ip, dp, ip, ip, dp, ip]),
+    "orc_y_kron_i_vec": (None, [C.c_int, C.c_int, C.c_int, dp, dp, dp]),
+    "orc_i_kron_xt_vec": (None, [C.c_int, C.c_int, C.c_int, dp, dp, dp]),
     "orc_sum_n_alloc": (C.c_int, [C.c_int, C.c_int, C.c_int, ip, ip, ip, ip, ip, ip]),
     "orc_sum_n_fill": (None, [C.c_int, C.c_int, ip, ip, dp, dp, ip, C.c_int, dp]),
 }
@@ -199,6 +201,25 @@ def sum_n(m, n, ops, d=None):
     return Cp, Ci[:nnz], Cx[:nnz], mp[:tot]
 
 
+def y_kron_i_vec(m, k, n, Y, w, lib_fn=None):
+    """v = (Y kron I_m) w (Y k x n column-major, len(w) = m*n), the oracle's C
+    restatement; lib_fn = ref_lib().Y_kron_I_vec runs the reference build."""
+    Y = np.ascontiguousarray(Y, np.float64).reshape(-1)
+    w = np.ascontiguousarray(w, np.float64).reshape(-1)
+    v = np.zeros(max(m * k, 1))
+    (lib_fn or lib().orc_y_kron_i_vec)(m, k, n, D(Y), D(w), D(v))
+    return v[:m * k]
+
+
+def i_kron_xt_vec(m, k, n, X, w, lib_fn=None):
+    """v = (I_n kron X^T) w (X m x k column-major, len(w) = m*n)."""
+    X = np.ascontiguousarray(X, np.float64).reshape(-1)
+    w = np.ascontiguousarray(w, np.float64).reshape(-1)
+    v = np.zeros(max(k * n, 1))
+    (lib_fn or lib().orc_i_kron_xt_vec)(m, k, n, D(X), D(w), D(v))
+    return v[:k * n]
+
+
 def row_gather(Ap, Ai, Ax, rmap):
     o = lib()
     rmap = np.ascontiguousarray(rmap, np.int32)
@@ -291,6 +312,8 @@ _REF_SIGS = {
                                           C.POINTER(_CSR), C.POINTER(ip)]),
     "sum_n_csr_alloc": (C.POINTER(_CSR), [C.POINTER(C.POINTER(_CSR)), C.c_int, ip]),
     "accumulator": (None, [dp, C.c_int, ip, dp]),
+    "Y_kron_I_vec": (None, [C.c_int, C.c_int, C.c_int, dp, dp, dp]),
+    "I_kron_XT_vec": (None, [C.c_int, C.c_int, C.c_int, dp, dp, dp]),
 }
 
 _ref = None

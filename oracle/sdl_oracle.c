@@ -454,3 +454,29 @@ void orc_sum_n_fill(int m, int nops, const int *P, const int *off, const double 
         }
     }
 }
+
+/* mini_numpy.c:74-89: v[row + j*m] = sum over col ascending of
+   Y[j + col*k] * w[row + col*m], from 0.0. */
+void orc_y_kron_i_vec(int m, int k, int n, const double *Y, const double *w, double *v)
+{
+    for (int j = 0; j < k; j++)
+        for (int row = 0; row < m; row++)
+        {
+            double s = 0.0;
+            for (int col = 0; col < n; col++) s += Y[j + (size_t) col * k] * w[row + (size_t) col * m];
+            v[row + (size_t) j * m] = s;
+        }
+}
+
+/* mini_numpy.c:91-105: v[j + col*k] = sum over row ascending of
+   X[row + j*m] * w[row + col*m], from 0.0. */
+void orc_i_kron_xt_vec(int m, int k, int n, const double *X, const double *w, double *v)
+{
+    for (int col = 0; col < n; col++)
+        for (int j = 0; j < k; j++)
+        {
+            double s = 0.0;
+            for (int row = 0; row < m; row++) s += X[row + (size_t) j * m] * w[row + (size_t) col * m];
+            v[j + (size_t) col * k] = s;
+        }
+}

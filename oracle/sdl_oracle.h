@@ -124,6 +124,10 @@ int orc_sum_n_alloc(int m, int n, int nops, const int *P, const int *I,
 void orc_sum_n_fill(int m, int nops, const int *P, const int *off, const double *X,
                     const double *d, const int *map, int nnzC, double *Cx);
 
+/* ---- dense Kronecker-vector products (src/support/mini_numpy.c:74-105) ---- */
+void orc_y_kron_i_vec(int m, int k, int n, const double *Y, const double *w, double *v);
+void orc_i_kron_xt_vec(int m, int k, int n, const double *X, const double *w, double *v);
+
 #ifdef __cplusplus
 }
 #endif

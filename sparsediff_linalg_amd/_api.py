@@ -60,6 +60,13 @@ SIGNATURES = {
     "BA_dense_kron_I_matrices_fill_values": (None, [MAT_p, C.c_int, MAT_p, MAT_p]),
     "BA_kron_matrices_alloc": (MAT_p, [MAT_p, C.c_int, MAT_p]),
     "BA_kron_matrices_fill_values": (None, [MAT_p, C.c_int, MAT_p, MAT_p]),
+    # dense.h
+    "repeat": (None, [dp, dp, C.c_int, C.c_int]),
+    "tile_double": (None, [dp, dp, C.c_int, C.c_int]),
+    "scaled_ones": (None, [dp, C.c_int, C.c_double]),
+    "A_transpose": (None, [dp, dp, C.c_int, C.c_int]),
+    "Y_kron_I_vec": (None, [C.c_int, C.c_int, C.c_int, dp, dp, dp]),
+    "I_kron_XT_vec": (None, [C.c_int, C.c_int, C.c_int, dp, dp, dp]),
     # matrix_sum.h
     "sum_matrices_alloc": (None, [MAT_p, MAT_p, MAT_p]),
     "sum_matrices_fill_values": (None, [MAT_p, MAT_p, MAT_p]),

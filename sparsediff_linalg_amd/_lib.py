@@ -130,6 +130,8 @@ SIGNATURES = {
     "sdl_sum_fill": (C.c_int, [vp, C.POINTER(vp), vp, vp, vp]),
     "sdl_sum_add": (C.c_int, [vp, C.c_int, vp, vp, vp]),
     "sdl_sum_plan_destroy": (None, [vp]),
+    "sdl_y_kron_i_vec": (C.c_int, [C.c_int, C.c_int, C.c_int, vp, vp, vp, vp]),
+    "sdl_i_kron_xt_vec": (C.c_int, [C.c_int, C.c_int, C.c_int, vp, vp, vp, vp]),
     "sdl_graph_begin": (C.c_int, [vp]),
     "sdl_graph_end": (C.c_int, [vp, C.POINTER(vp)]),
     "sdl_graph_launch": (C.c_int, [vp, vp]),

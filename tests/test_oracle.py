@@ -380,3 +380,43 @@ def test_sum_reference_literal_through_reference_build():
     Rp, Ri, Rx = O.ref_sum(4, 6, SUM5_OPS)
     Cp, Ci, Cx, _ = O.sum_n(4, 6, SUM5_OPS)
     assert list(Rp) == list(Cp) and list(Ri) == list(Ci) and bits_equal(Rx, Cx)
+
+
+# ------------------------------------------------------- dense helpers ----
+def _kron_case(rng, m, k, n, integer=False):
+    if integer:
+        return (rng.integers(-4, 5, (k, n)).astype(float), rng.integers(-4, 5, (m, k)).astype(float),
+                rng.integers(-4, 5, m * n).astype(float))
+    return rng.standard_normal((k, n)), rng.standard_normal((m, k)), rng.standard_normal(m * n)
+
+
+@pytest.mark.parametrize("shape", [(3, 2, 4), (7, 5, 1), (1, 1, 9), (40, 13, 37)])
+def test_dense_kron_closed_form(shape):
+    """(Y kron I_m) w and (I_n kron X^T) w against numpy's kron on column-major
+    reshapes: exact on small integers, 1e-13 relative on normal values."""
+    m, k, n = shape
+    rng = np.random.default_rng(sum(shape))
+    for integer in (True, False):
+        Y, X, w = _kron_case(rng, m, k, n, integer)
+        v1 = O.y_kron_i_vec(m, k, n, Y.reshape(-1, order="F"), w)
+        r1 = np.kron(Y, np.eye(m)) @ w
+        v2 = O.i_kron_xt_vec(m, k, n, X.reshape(-1, order="F"), w)
+        r2 = np.kron(np.eye(n), X.T) @ w
+        if integer:
+            assert np.array_equal(v1, r1) and np.array_equal(v2, r2)
+        else:
+            assert np.allclose(v1, r1, rtol=1e-13, atol=1e-13 * np.abs(r1).max())
+            assert np.allclose(v2, r2, rtol=1e-13, atol=1e-13 * np.abs(r2).max())
+
+
+@needs_ref
+@pytest.mark.parametrize("shape", [(3, 2, 4), (37, 19, 70), (64, 1, 5), (5, 3, 0)])
+def test_dense_kron_matches_reference_build(shape):
+    m, k, n = shape
+    rng = np.random.default_rng(7 * m + k)
+    Y, X, w = _kron_case(rng, m, k, n)
+    r = O.ref_lib()
+    Yf, Xf = Y.reshape(-1, order="F"), X.reshape(-1, order="F")
+    assert bits_equal(O.y_kron_i_vec(m, k, n, Yf, w), O.y_kron_i_vec(m, k, n, Yf, w, r.Y_kron_I_vec))
+    assert bits_equal(O.i_kron_xt_vec(m, k, n, Xf, w),
+                      O.i_kron_xt_vec(m, k, n, Xf, w, r.I_kron_XT_vec))
