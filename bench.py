@@ -56,7 +56,7 @@ def parse():
                     help="multi: refill rounds per launch (even; output ring of this many slots)")
     ap.add_argument("--workload", default="btda",
                     choices=["btda", "blocks-atda", "blocks-btda", "multi", "ata-large", "coo",
-                             "learned", "sum"],
+                             "learned", "sum", "dense-kron"],
                     help="btda: the sparse refill (default line); blocks-*: the stacked "
                          "permuted-dense products (A^T D A, or B^T D A of two stacks)")
     ap.add_argument("--rmat-scale", type=int, default=18,
@@ -102,6 +102,8 @@ def main():
         return main_learned(args)
     if args.workload == "sum":
         return main_sum(args)
+    if args.workload == "dense-kron":
+        return main_dense_kron(args)
     if args.workload != "btda":
         return main_blocks(args)
     import sparsediff_linalg_amd as sdl
@@ -520,6 +522,94 @@ def main_sum(args):
     }
     lib.sdl_sum_plan_destroy(h)
     print(json.dumps(out_json), flush=True)
+
+
+def main_dense_kron(args):
+    """The dense Kronecker-vector products of dense.h on device arrays:
+    v = (Y kron I_m) w and v = (I_n kron X^T) w with m = 4096, k = 256,
+    n = 4096 (FP64 N(0,1)); one step = both products.  Each output is an
+    ordered sum of rounded products (the reference's arithmetic), so the
+    kernels use neither MFMA nor fused multiply-add."""
+    import sparsediff_linalg_amd as sdl
+    from oracle import oracle as O
+    from sparsediff_linalg_amd import DeviceArray, check
+
+    lib = sdl.load()
+    m, k, n = 4096, 256, 4096
+    rng = np.random.default_rng(12)
+    Y, X, w = rng.standard_normal(k * n), rng.standard_normal(m * k), rng.standard_normal(m * n)
+    dY, dX, dw = (DeviceArray.from_numpy(a) for a in (Y, X, w))
+    dv1, dv2 = DeviceArray(m * k, np.float64), DeviceArray(k * n, np.float64)
+    stream = lib.sdl_library_stream()
+
+    def step():
+        check(lib.sdl_y_kron_i_vec(m, k, n, dY.ptr, dw.ptr, dv1.ptr, stream), "y_kron_i")
+        check(lib.sdl_i_kron_xt_vec(m, k, n, dX.ptr, dw.ptr, dv2.ptr, stream), "i_kron_xt")
+
+    for _ in range(max(1, args.warmup)):
+        step()
+    lib.sdl_synchronize()
+    e0, e1 = lib.sdl_event_create(), lib.sdl_event_create()
+    steps = max(1, min(args.steps, 50))
+    lib.sdl_event_record(e0, stream)
+    for _ in range(steps):
+        step()
+    lib.sdl_event_record(e1, stream)
+    lib.sdl_synchronize()
+    ms = lib.sdl_event_elapsed_ms(e0, e1) / steps
+    # sampled outputs against the oracle, bit for bit (64 rows of W / columns)
+    v1, v2 = dv1.numpy(), dv2.numpy()
+    rows = np.arange(0, m, m // 64)
+    Ys = Y.reshape(n, k).T  # Y[j + col*k]
+    W = w.reshape(n, m).T   # W[row, col] = w[row + col*m]
+    exact = True
+    for r in rows:
+        ref = np.zeros(k)
+        for c in range(n):
+            ref = ref + Ys[:, c] * W[r, c]
+        exact = exact and np.array_equal(v1[r + np.arange(k) * m].view(np.int64), ref.view(np.int64))
+    cols = np.arange(0, n, n // 16)
+    Xm = X.reshape(k, m).T  # X[row + j*m]
+    for c in cols:
+        ref = np.zeros(k)
+        for r in range(m):
+            ref = ref + Xm[r, :] * W[r, c]
+        exact = exact and np.array_equal(v2[np.arange(k) + c * k].view(np.int64), ref.view(np.int64))
+    assert exact, "kron products differ from the ordered sums"
+    flops = 2 * 2.0 * m * k * n
+    out_json = {
+        "metric": "dense Kronecker-vector products FP64 GFLOP/s (ordered sums)",
+        "value": flops / (ms * 1e-3) / 1e9, "unit": "GFLOP/s", "n_gpus": 1, "steps": steps,
+        "warmup": max(1, args.warmup), "ms_per_step": ms, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": ("v = (Y kron I_m) w and v = (I_n kron X^T) w, m = 4096, k = 256, "
+                                "n = 4096, FP64 N(0,1), seed 12; one step = both products"),
+                   "sampled_outputs_bit_exact": exact},
+        "roofline": {"bound": "fp64", "kernel": "k_ordered_dots<2,16,32>",
+                     "achieved": flops / (ms * 1e-3) / 1e12, "peak": 78.6, "unit": "TFLOP/s",
+                     "frac": flops / (ms * 1e-3) / 1e12 / 78.6, "traffic": None,
+                     "note": ("separate FP64 multiply and add per term (the reference's rounding): "
+                              "the attainable ceiling is half the FMA peak")},
+        "cpu_baseline": None if args.no_cpu else dense_kron_cpu_baseline(m, k, n, Y, X, w),
+    }
+    print(json.dumps(out_json), flush=True)
+
+
+def dense_kron_cpu_baseline(m, k, n, Y, X, w):
+    """The reference's own Y_kron_I_vec / I_kron_XT_vec (oracle/_ref, one core)
+    on the first 512 columns of W (the same per-output work; n scaled 1/8)."""
+    from oracle import oracle as O
+    ns = 512
+    Ys, ws = Y[:k * ns], w[:m * ns]
+    fn1 = O.ref_lib().Y_kron_I_vec if O.ref_available() else None
+    fn2 = O.ref_lib().I_kron_XT_vec if O.ref_available() else None
+    t0 = time.perf_counter()
+    O.y_kron_i_vec(m, k, ns, Ys, ws, fn1)
+    O.i_kron_xt_vec(m, k, ns, X, ws, fn2)
+    t = time.perf_counter() - t0
+    return {"value": 2 * 2.0 * m * k * ns / t / 1e9, "unit": "GFLOP/s", "cores": 1,
+            "kind": "reference" if fn1 else "port",
+            "sample": f"both products with n = {ns} (1/8 of the columns) in {t:.2f} s on one core"}
 
 
 def sum_cpu_baseline(ops, m, n, seconds):

@@ -69,34 +69,42 @@ __global__ __launch_bounds__(TT * 8) void k_transpose_dense(int m, int n,
 
 // Ordered dot-product tiles: out[p * op + q * oq] = sum over s ascending of
 // P(p, s) * Q(q, s), from +0.0, with P(p, s) = P[p * pp + s * psd] and
-// Q(q, s) = Q[q * qq + s * qsd].  A 16 x 16 output tile per workgroup; slices of DS
-// reduction steps of P and Q go through LDS, so the order of each output's sum
-// is the reference's whatever the tiling.
-constexpr int DT = 16, DS = 32;
-__global__ __launch_bounds__(DT * DT) void k_ordered_dots(int np, int nq, int ns,
-                                                        const double *__restrict__ P,
-                                                        int64_t pp, int64_t psd,
-                                                        const double *__restrict__ Q,
-                                                        int64_t qq, int64_t qsd,
-                                                        double *__restrict__ out, int64_t op,
-                                                        int64_t oq)
+// Q(q, s) = Q[q * qq + s * qsd].  A DT x DT output tile per workgroup of TS x TS
+// threads, RB x RB outputs per thread (p = tile + tp + TS i, q = tile + tq +
+// TS j); slices of DS reduction steps of P and Q go through LDS.  Each output
+// keeps its own accumulator and adds its products in ascending s, so the sum is
+// the reference's whatever the tiling (no fused multiply-add: built with
+// -ffp-contract=off).  Few outputs and long sums take the smaller tiles, so
+// the work still spreads over the CUs, and deeper slices, so each global-load
+// round trip feeds more steps.
+template <int RB, int TS, int DS>
+__global__ __launch_bounds__(TS * TS) void k_ordered_dots(int np, int nq, int ns,
+                                                     const double *__restrict__ P, int64_t pp,
+                                                     int64_t psd, const double *__restrict__ Q,
+                                                     int64_t qq, int64_t qsd,
+                                                     double *__restrict__ out, int64_t op,
+                                                     int64_t oq)
 {
+    constexpr int DT = TS * RB, NT = TS * TS;
     __shared__ double Ps[DS][DT + 1], Qs[DS][DT + 1];
     const int ntp = (np + DT - 1) / DT;
     const int bp = (int) (blockIdx.x % ntp), bq = (int) (blockIdx.x / ntp);
-    const int tp = threadIdx.x % DT, tq = threadIdx.x / DT;
-    const int p = bp * DT + tp, q = bq * DT + tq;
-    double acc = 0.0;
+    const int tp = threadIdx.x % TS, tq = threadIdx.x / TS;
+    double acc[RB][RB];
+#pragma unroll
+    for (int i = 0; i < RB; i++)
+#pragma unroll
+        for (int j = 0; j < RB; j++) acc[i][j] = 0.0;
     for (int s0 = 0; s0 < ns; s0 += DS)
     {
         // consecutive threads walk the unit-stride index of each operand
-        for (int idx = threadIdx.x; idx < DS * DT; idx += DT * DT)
+        for (int idx = threadIdx.x; idx < DS * DT; idx += NT)
         {
             const int s = psd == 1 ? idx % DS : idx / DT, c = psd == 1 ? idx / DS : idx % DT;
             const int gp = bp * DT + c, gs = s0 + s;
             Ps[s][c] = (gp < np && gs < ns) ? P[gp * pp + gs * psd] : 0.0;
         }
-        for (int idx = threadIdx.x; idx < DS * DT; idx += DT * DT)
+        for (int idx = threadIdx.x; idx < DS * DT; idx += NT)
         {
             const int s = qsd == 1 ? idx % DS : idx / DT, c = qsd == 1 ? idx / DS : idx % DT;
             const int gq = bq * DT + c, gs = s0 + s;
@@ -104,14 +112,36 @@ __global__ __launch_bounds__(DT * DT) void k_ordered_dots(int np, int nq, int ns
         }
         __syncthreads();
         const int lim = ns - s0 < DS ? ns - s0 : DS;
+#pragma unroll 4
         for (int s = 0; s < lim; s++)
         {
-            const double prod = Ps[s][tp] * Qs[s][tq];
-            acc = acc + prod;
+            double a[RB], b[RB];
+#pragma unroll
+            for (int i = 0; i < RB; i++) a[i] = Ps[s][tp + TS * i];
+#pragma unroll
+            for (int j = 0; j < RB; j++) b[j] = Qs[s][tq + TS * j];
+#pragma unroll
+            for (int i = 0; i < RB; i++)
+#pragma unroll
+                for (int j = 0; j < RB; j++)
+                {
+                    const double prod = a[i] * b[j];
+                    acc[i][j] = acc[i][j] + prod;
+                }
         }
         __syncthreads();
     }
-    if (p < np && q < nq) out[p * op + q * oq] = acc;
+#pragma unroll
+    for (int i = 0; i < RB; i++)
+    {
+        const int p = bp * DT + tp + TS * i;
+#pragma unroll
+        for (int j = 0; j < RB; j++)
+        {
+            const int q = bq * DT + tq + TS * j;
+            if (p < np && q < nq) out[p * op + q * oq] = acc[i][j];
+        }
+    }
 }
 
 void ordered_dots(int np, int nq, int ns, const double *P, int64_t pp, int64_t psd,
@@ -119,9 +149,20 @@ void ordered_dots(int np, int nq, int ns, const double *P, int64_t pp, int64_t p
                   hipStream_t st)
 {
     if (np <= 0 || nq <= 0) return;
-    const unsigned blocks = (unsigned) (((np + DT - 1) / DT) * (int64_t) ((nq + DT - 1) / DT));
-    hipLaunchKernelGGL(k_ordered_dots, dim3(blocks), dim3(DT * DT), 0, st, np, nq, ns, P, pp, psd,
-                       Q, qq, qsd, out, op, oq);
+    auto tiles = [&](int dt) { return ((np + dt - 1) / dt) * (int64_t) ((nq + dt - 1) / dt); };
+    constexpr int64_t enough = 512;  // two workgroups per CU
+    if (tiles(64) >= enough)
+        hipLaunchKernelGGL((k_ordered_dots<4, 16, 16>), dim3((unsigned) tiles(64)), dim3(256), 0, st,
+                           np, nq, ns, P, pp, psd, Q, qq, qsd, out, op, oq);
+    else if (tiles(32) >= enough)
+        hipLaunchKernelGGL((k_ordered_dots<2, 16, 32>), dim3((unsigned) tiles(32)), dim3(256), 0, st,
+                           np, nq, ns, P, pp, psd, Q, qq, qsd, out, op, oq);
+    else if (tiles(16) >= enough)
+        hipLaunchKernelGGL((k_ordered_dots<1, 16, 64>), dim3((unsigned) tiles(16)), dim3(256), 0, st,
+                           np, nq, ns, P, pp, psd, Q, qq, qsd, out, op, oq);
+    else
+        hipLaunchKernelGGL((k_ordered_dots<1, 8, 256>), dim3((unsigned) tiles(8)), dim3(64), 0, st,
+                           np, nq, ns, P, pp, psd, Q, qq, qsd, out, op, oq);
     SDL_HIP(hipGetLastError());
 }
 

@@ -49,7 +49,8 @@ def test_a_transpose(mn):
 
 
 @pytest.mark.parametrize("shape", [(3, 2, 4), (37, 19, 70), (1, 1, 1), (64, 5, 33), (5, 3, 0),
-                                   (130, 40, 129)])
+                                   (130, 40, 129), (512, 256, 3), (2048, 1024, 2),
+                                   (3, 600, 900), (20000, 16, 64), (1024, 512, 2)])
 def test_kron_vec_products(shape):
     L = D.lib()
     m, k, n = shape
