@@ -56,7 +56,7 @@ def parse():
                     help="multi: refill rounds per launch (even; output ring of this many slots)")
     ap.add_argument("--workload", default="btda",
                     choices=["btda", "blocks-atda", "blocks-btda", "multi", "ata-large", "coo",
-                             "learned", "sum", "dense-kron"],
+                             "learned", "sum", "dense-kron", "lower-coo"],
                     help="btda: the sparse refill (default line); blocks-*: the stacked "
                          "permuted-dense products (A^T D A, or B^T D A of two stacks)")
     ap.add_argument("--rmat-scale", type=int, default=18,
@@ -104,6 +104,8 @@ def main():
         return main_sum(args)
     if args.workload == "dense-kron":
         return main_dense_kron(args)
+    if args.workload == "lower-coo":
+        return main_lower_coo(args)
     if args.workload != "btda":
         return main_blocks(args)
     import sparsediff_linalg_amd as sdl
@@ -521,6 +523,98 @@ def main_sum(args):
         "cpu_baseline": None if args.no_cpu else sum_cpu_baseline(ops, m, n, args.cpu_seconds),
     }
     lib.sdl_sum_plan_destroy(h)
+    print(json.dumps(out_json), flush=True)
+
+
+def main_lower_coo(args):
+    """Solver hand-off: the lower-triangular COO of a Hessian-shaped product
+    output kept in HBM.  A = 2^20 x 2^20 banded (bandwidth 32, FP64 N(0,1)),
+    C = A^T diag(d) A (2^20 rows, ~129 entries per row); one step = one refresh
+    of the exported values from C.x (sdl_lower_coo_refresh)."""
+    import sparsediff_linalg_amd as sdl
+    from oracle import oracle as O
+    from sparsediff_linalg_amd import DeviceArray, check, workloads
+    from sparsediff_linalg_amd.predictor import Plan
+
+    lib = sdl.load()
+    m, bw = 1 << 20, 32
+    Ap, Ai, Ax = workloads.banded_csr(m, bw)
+    pl = Plan((Ap, Ai), None, m, m)
+    nnz = pl.nnz()
+    dA = DeviceArray.from_numpy(Ax)
+    dd = DeviceArray.from_numpy(workloads.uniform(31, m, 0.1, 10.0))
+    Cx = DeviceArray(nnz, np.float64)
+    check(lib.sdl_btda_refill(pl.h, dA.ptr, dA.ptr, dd.ptr, Cx.ptr, None), "refill")
+    lib.sdl_synchronize()
+    h = C.c_void_p()
+    t0 = time.perf_counter()
+    check(lib.sdl_lower_coo_create(m, lib.sdl_btda_plan_rowptr(pl.h), lib.sdl_btda_plan_colind(pl.h),
+                                   C.byref(h)), "lower coo")
+    t_create = time.perf_counter() - t0
+    cnt = lib.sdl_lower_coo_nnz(h)
+    x = DeviceArray(cnt, np.float64)
+    stream = lib.sdl_library_stream()
+    for _ in range(max(1, args.warmup)):
+        check(lib.sdl_lower_coo_refresh(h, Cx.ptr, x.ptr, stream), "refresh")
+    lib.sdl_synchronize()
+    e0, e1 = lib.sdl_event_create(), lib.sdl_event_create()
+    steps = max(1, min(args.steps, 100))
+    lib.sdl_event_record(e0, stream)
+    for _ in range(steps):
+        check(lib.sdl_lower_coo_refresh(h, Cx.ptr, x.ptr, stream), "refresh")
+    lib.sdl_event_record(e1, stream)
+    lib.sdl_synchronize()
+    ms = lib.sdl_event_elapsed_ms(e0, e1) / steps
+    # whole export against the oracle's (rows, cols, value_map, values)
+    cp = np.empty(m + 1, np.int64)
+    lib.sdl_memcpy_dtoh(cp.ctypes.data, lib.sdl_btda_plan_rowptr(pl.h), cp.nbytes)
+    ci = np.empty(nnz, np.int32)
+    lib.sdl_memcpy_dtoh(ci.ctypes.data, lib.sdl_btda_plan_colind(pl.h), ci.nbytes)
+    cx = Cx.numpy()
+    r, c, v, vm = O.coo_lower(m, cp.astype(np.int32), ci, cx)
+
+    def dev(ptr, n, dt):
+        a = np.empty(n, dt)
+        lib.sdl_memcpy_dtoh(a.ctypes.data, ptr, a.nbytes)
+        return a
+
+    exact = (cnt == len(r) and np.array_equal(dev(lib.sdl_lower_coo_rows(h), cnt, np.int32), r)
+             and np.array_equal(dev(lib.sdl_lower_coo_cols(h), cnt, np.int32), c)
+             and np.array_equal(dev(lib.sdl_lower_coo_value_map(h), cnt, np.int64), vm)
+             and np.array_equal(x.numpy().view(np.int64), v.view(np.int64)))
+    assert exact, "lower-triangular export differs from the oracle"
+    bytes_ = 16 * cnt  # values read from C.x, written to the export
+    cpu = None
+    if not args.no_cpu:
+        if O.ref_available():
+            k, fills, t = O.ref_lower_coo_timer(m, m, cp.astype(np.int32), ci, cx, args.cpu_seconds)
+            cpu = {"value": k * fills / t, "unit": "entries/s", "cores": 1, "kind": "reference",
+                   "sample": (f"whole export, {fills} refreshes in {t:.2f} s on one core "
+                              "(oracle/_ref refresh_lower_triangular_coo)")}
+        else:
+            t0 = time.perf_counter()
+            O.coo_lower(m, cp.astype(np.int32), ci, cx)
+            t = time.perf_counter() - t0
+            cpu = {"value": cnt / t, "unit": "entries/s", "cores": 1, "kind": "port",
+                   "sample": f"whole export (structure and values) in {t:.2f} s, the oracle"}
+    out_json = {
+        "metric": "lower-triangular COO refresh entries/s (solver hand-off)",
+        "value": cnt / (ms * 1e-3), "unit": "entries/s", "n_gpus": 1, "steps": steps,
+        "warmup": max(1, args.warmup), "ms_per_step": ms, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": ("lower-triangular COO of C = A^T diag(d) A, A 2^20 x 2^20 banded "
+                                "(bandwidth 32), C with ~129 entries per row; one step = one "
+                                "refresh of the exported values"),
+                   "nnz_C": nnz, "exported": cnt, "create_s": t_create,
+                   "bit_exact_vs_oracle": exact},
+        "roofline": {"bound": "hbm", "kernel": "k_lower_refresh",
+                     "achieved": bytes_ / (ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": bytes_ / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, "traffic": None,
+                     "alg_bytes_per_launch": bytes_},
+        "cpu_baseline": cpu,
+    }
+    lib.sdl_lower_coo_destroy(h)
+    pl.close()
     print(json.dumps(out_json), flush=True)
 
 

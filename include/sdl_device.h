@@ -186,6 +186,22 @@ int sdl_sum_fill(sdl_sum_plan *plan, const double *const *dX, const double *dD, 
 int sdl_sum_add(sdl_sum_plan *plan, int k, const double *dx, double *dCx, void *stream);
 void sdl_sum_plan_destroy(sdl_sum_plan *plan);
 
+/* ---- Solver hand-off: lower-triangular COO of an HBM-resident output ----
+   Device form of new_COO_matrix_lower_triangular / refresh_lower_triangular_coo
+   (COO_matrix.c:48-98): the entries with col <= row in CSR order, value_map =
+   their CSR positions (int64).  dCp (int64, m+1) and dCi describe a CSR with
+   sorted columns per row, e.g. sdl_btda_plan_rowptr / _colind of a product
+   plan; the refresh copies the selected values of dCx into dx (nnz of the
+   export) without reading value_map, allocates nothing and can be captured. */
+typedef struct sdl_lower_coo sdl_lower_coo;
+int sdl_lower_coo_create(int m, const long long *dCp, const int *dCi, sdl_lower_coo **out);
+long long sdl_lower_coo_nnz(const sdl_lower_coo *coo);
+const int *sdl_lower_coo_rows(const sdl_lower_coo *coo);
+const int *sdl_lower_coo_cols(const sdl_lower_coo *coo);
+const long long *sdl_lower_coo_value_map(const sdl_lower_coo *coo);
+int sdl_lower_coo_refresh(const sdl_lower_coo *coo, const double *dCx, double *dx, void *stream);
+void sdl_lower_coo_destroy(sdl_lower_coo *coo);
+
 /* ---- Dense Kronecker-vector products on device arrays ----
    Device forms of the reference's Y_kron_I_vec / I_kron_XT_vec
    (src/support/mini_numpy.c:74-105): same index conventions, every output a

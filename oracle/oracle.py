@@ -273,6 +273,11 @@ class _CSR(C.Structure):
                 ("nnz", C.c_int)]
 
 
+class _COO(C.Structure):  # COO_matrix, include/sparsediff_linalg/COO_matrix.h:35-44
+    _fields_ = [("rows", ip), ("cols", ip), ("x", dp), ("value_map", ip), ("m", C.c_int),
+                ("n", C.c_int), ("nnz", C.c_int)]
+
+
 class _Rows(C.Structure):  # sparse_rows, src/internal/gustavson.h:26-32
     _fields_ = [("p", ip), ("i", ip), ("x", dp), ("m", C.c_int)]
 
@@ -314,6 +319,9 @@ _REF_SIGS = {
     "accumulator": (None, [dp, C.c_int, ip, dp]),
     "Y_kron_I_vec": (None, [C.c_int, C.c_int, C.c_int, dp, dp, dp]),
     "I_kron_XT_vec": (None, [C.c_int, C.c_int, C.c_int, dp, dp, dp]),
+    "new_COO_matrix_lower_triangular": (C.POINTER(_COO), [C.POINTER(_CSR)]),
+    "refresh_lower_triangular_coo": (None, [C.POINTER(_COO), dp]),
+    "free_COO_matrix": (None, [C.POINTER(_COO)]),
 }
 
 _ref = None
@@ -433,6 +441,27 @@ def ref_sum(m, n, ops, d=None):
             r.accumulator(D(xv), v.nnz, I(mp), D(Cx))
     r.free_CSR_matrix(Cc)
     return Cp, Ci, Cx[:nnz]
+
+
+def ref_lower_coo_timer(m, n, p, i, x, seconds):
+    """The reference's lower-triangular export (COO_matrix.c:48-90) once, then
+    its refresh (COO_matrix.c:92-98) repeated for about `seconds` on one core.
+    Returns (exported entries, refreshes, refresh seconds)."""
+    import time
+    r = ref_lib()
+    keep = []
+    A = _csr_view(m, n, p, i, x, keep)
+    coo = r.new_COO_matrix_lower_triangular(C.byref(A))
+    cnt = coo.contents.nnz
+    xv = keep[2]
+    fills, t = 0, 0.0
+    while t < seconds or fills == 0:
+        t0 = time.perf_counter()
+        r.refresh_lower_triangular_coo(coo, D(xv))
+        t += time.perf_counter() - t0
+        fills += 1
+    r.free_COO_matrix(coo)
+    return cnt, fills, t
 
 
 def ref_fill_timer(m, nA, nB, Ap, Ai, Ax, Bp, Bi, Bx, ds):

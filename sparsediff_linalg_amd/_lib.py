@@ -130,6 +130,13 @@ SIGNATURES = {
     "sdl_sum_fill": (C.c_int, [vp, C.POINTER(vp), vp, vp, vp]),
     "sdl_sum_add": (C.c_int, [vp, C.c_int, vp, vp, vp]),
     "sdl_sum_plan_destroy": (None, [vp]),
+    "sdl_lower_coo_create": (C.c_int, [C.c_int, vp, vp, C.POINTER(vp)]),
+    "sdl_lower_coo_nnz": (C.c_longlong, [vp]),
+    "sdl_lower_coo_rows": (vp, [vp]),
+    "sdl_lower_coo_cols": (vp, [vp]),
+    "sdl_lower_coo_value_map": (vp, [vp]),
+    "sdl_lower_coo_refresh": (C.c_int, [vp, vp, vp, vp]),
+    "sdl_lower_coo_destroy": (None, [vp]),
     "sdl_y_kron_i_vec": (C.c_int, [C.c_int, C.c_int, C.c_int, vp, vp, vp, vp]),
     "sdl_i_kron_xt_vec": (C.c_int, [C.c_int, C.c_int, C.c_int, vp, vp, vp, vp]),
     "sdl_graph_begin": (C.c_int, [vp]),
