@@ -101,15 +101,34 @@ __global__ void k_coalesce_entries(int64_t n, const K *__restrict__ keys,
     ekeys[e] = keys[s];
 }
 
-__global__ void k_coalesce_fill(int64_t nnz, const int64_t *__restrict__ seg,
-                                const unsigned *__restrict__ order, const double *__restrict__ x,
-                                double *__restrict__ Cx)
+// Cx[e] = ((+0.0 + x[t1]) + x[t2]) + ... over entry e's items in order.  Items
+// go four at a time with unconditional loads (indices past the entry's last
+// item repeat its first, whose value is then dropped), so up to four random
+// gathers of x are in flight per thread instead of one.
+__global__ __launch_bounds__(256) void k_coalesce_fill(int64_t nnz,
+                                                      const int64_t *__restrict__ seg,
+                                                      const unsigned *__restrict__ order,
+                                                      const double *__restrict__ x,
+                                                      double *__restrict__ Cx)
 {
     int64_t e = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= nnz) return;
+    const int64_t s1 = seg[e + 1];
     double acc = 0.0;
-    for (int64_t s = seg[e]; s < seg[e + 1]; s++) acc = acc + x[order[s]];
-    Cx[e] = acc;
+    for (int64_t s = seg[e]; s < s1; s += 4)
+    {
+        const int64_t cnt = s1 - s;
+        unsigned t[4];
+        double v[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) t[u] = order[u < cnt ? s + u : s];
+#pragma unroll
+        for (int u = 0; u < 4; u++) v[u] = x[t[u]];
+#pragma unroll
+        for (int u = 0; u < 4; u++)
+            if (u < cnt) acc = acc + v[u];
+    }
+    __builtin_nontemporal_store(acc, Cx + e);
 }
 
 __global__ void k_set_i64(int64_t *p, int64_t v) { *p = v; }
