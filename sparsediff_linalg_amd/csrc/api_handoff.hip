@@ -58,22 +58,40 @@ __global__ void k_lower_entries(int64_t total, int m, const int64_t *__restrict_
     vmap[o] = t;
 }
 
-// x[off[r] + i] = Cx[Cp[r] + i] for i < off[r+1] - off[r]: one wave per row,
-// reads and writes contiguous.  (Measured slower: a wave per 256 export
+// x[off[r] + i] = Cx[Cp[r] + i] for i < off[r+1] - off[r]: LG lanes per row
+// (four rows per wave), LU entries per lane per pass; every load of a pass is
+// issued unconditionally (indices past the row's end re-read its last entry,
+// whose copy is dropped), so a lane keeps LU loads in flight.  Reads and writes
+// are contiguous runs of LG values.  (Measured slower: one wave per row, whose
+// second pass leaves most lanes idle on ~65-entry rows; a wave per 256 export
 // entries with a per-lane row walk, whose data-dependent walk serialises the
-// loads, and two loads per lane per pass, which splits short rows' lanes over
-// two code paths.)
+// loads.)
+constexpr int LG = 16, LU = 8;
 __global__ __launch_bounds__(256) void k_lower_refresh(int m, const int64_t *__restrict__ off,
                                                       const int64_t *__restrict__ Cp,
                                                       const double *__restrict__ Cx,
                                                       double *__restrict__ x)
 {
-    const int r = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (threadIdx.x >> 6));
-    if (r >= m) return;
-    const int lane = threadIdx.x & 63;
+    const int64_t r = ((int64_t) blockIdx.x * 256 + threadIdx.x) / LG;
+    if (r >= m) return;  // no cross-lane operations below
+    const int gl = threadIdx.x % LG;
     const int64_t d0 = off[r], len = off[r + 1] - d0, s0 = Cp[r];
-    for (int64_t i = lane; i < len; i += 64)
-        __builtin_nontemporal_store(Cx[s0 + i], x + d0 + i);
+    for (int64_t b = 0; b < len; b += LG * LU)
+    {
+        double v[LU];
+#pragma unroll
+        for (int u = 0; u < LU; u++)
+        {
+            const int64_t i = b + gl + LG * u;
+            v[u] = Cx[s0 + (i < len ? i : len - 1)];
+        }
+#pragma unroll
+        for (int u = 0; u < LU; u++)
+        {
+            const int64_t i = b + gl + LG * u;
+            if (i < len) __builtin_nontemporal_store(v[u], x + d0 + i);
+        }
+    }
 }
 
 } // namespace
@@ -143,7 +161,8 @@ int sdl_lower_coo_refresh(const sdl_lower_coo *h, const double *dCx, double *dx,
         return 1;
     }
     if (h->m > 0 && h->nnz > 0)
-        hipLaunchKernelGGL(k_lower_refresh, dim3((unsigned) ((h->m + 3) / 4)), dim3(256), 0,
+        hipLaunchKernelGGL(k_lower_refresh,
+                           dim3((unsigned) (((int64_t) h->m * LG + 255) / 256)), dim3(256), 0,
                            stream ? (hipStream_t) stream : sdl::stream(), h->m, h->off.p, h->Cp.p,
                            dCx, dx);
     SDL_HIP(hipGetLastError());
