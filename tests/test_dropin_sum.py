@@ -211,3 +211,30 @@ def test_sum_n_random_with_repeated_entries():
     L.free_matrix(C)
     for h in hs[:5]:
         L.free_matrix(h)
+
+
+def test_sum_edge_cases():
+    """All operands empty (sums allocate an empty union and every fill is a
+    no-op), zero rows, and sum_matrices_alloc into a zero-capacity C."""
+    L = D.lib()
+    empt = ([0] * 6, [], [])
+    hs = [D.sparse(*empt, 5, 4) for _ in range(3)]
+    arr = D.ops_array(hs)
+    C = L.sum_n_matrices_alloc(arr, 3)
+    L.sum_n_matrices_fill_values(arr, 3, C)
+    L.sum_n_matrices_fill_zero(C)
+    for k in range(3):
+        L.sum_n_matrices_fill_add(hs[k], k, C)
+    p, i, x = D.mat_csr(C)
+    assert list(p) == [0] * 6 and len(i) == 0 and C.contents.nnz == 0
+    S = L.new_sparse_matrix_alloc(5, 4, 0)
+    L.sum_matrices_alloc(hs[0], hs[1], S)
+    d = np.ones(5)
+    L.sum_scaled_matrices_fill_values(hs[0], hs[1], S, D.dptr(d), D.dptr(d))
+    assert S.contents.nnz == 0
+    z = [D.sparse([0], [], [], 0, 3) for _ in range(2)]
+    Cz = L.sum_n_matrices_alloc(D.ops_array(z), 2)
+    L.sum_n_matrices_fill_values(D.ops_array(z), 2, Cz)
+    assert Cz.contents.m == 0 and Cz.contents.nnz == 0
+    for M in (C, S, Cz, *hs, *z):
+        L.free_matrix(M)

@@ -91,3 +91,13 @@ def test_lower_coo_of_product(case):
     lib.sdl_graph_destroy(g)
     lib.sdl_lower_coo_destroy(h)
     pl.close()
+
+
+def test_lower_coo_zero_rows():
+    lib = load()
+    dp = DeviceArray.from_numpy(np.zeros(1, np.int64))
+    h = vp()
+    check(lib.sdl_lower_coo_create(0, dp.ptr, None, C.byref(h)), "lower coo")
+    assert lib.sdl_lower_coo_nnz(h) == 0
+    check(lib.sdl_lower_coo_refresh(h, None, None, None), "refresh")
+    lib.sdl_lower_coo_destroy(h)
