@@ -501,6 +501,25 @@ def main_sum(args):
     exact = (np.array_equal(cp, Cp) and np.array_equal(ci, Ci)
              and np.array_equal(out.numpy().view(np.int64), Cx.view(np.int64)))
     assert exact, "sum differs from the oracle"
+    # the row-scaled fill (sum_scaled_matrices semantics for four operands): a
+    # plan that keeps the rows
+    h2 = vp()
+    check(lib.sdl_sum_plan_create(nops, m, n, dP, dI, nz, 1, C.byref(h2)), "sum plan (rows)")
+    dscale = np.random.default_rng(7).uniform(0.5, 2.0, (nops, m))
+    dD = DeviceArray.from_numpy(dscale)
+    out2 = DeviceArray(nnz, np.float64)
+    check(lib.sdl_sum_fill(h2, dX, dD.ptr, out2.ptr, stream), "scaled fill")
+    lib.sdl_synchronize()
+    lib.sdl_event_record(e0, stream)
+    for _ in range(steps):
+        check(lib.sdl_sum_fill(h2, dX, dD.ptr, out2.ptr, stream), "scaled fill")
+    lib.sdl_event_record(e1, stream)
+    lib.sdl_synchronize()
+    ms_scaled = lib.sdl_event_elapsed_ms(e0, e1) / steps
+    _, _, Sx, _ = O.sum_n(m, n, ops, d=dscale)
+    exact_scaled = np.array_equal(out2.numpy().view(np.int64), Sx.view(np.int64))
+    assert exact_scaled, "scaled sum differs from the oracle"
+    lib.sdl_sum_plan_destroy(h2)
     bytes_ = 8 * items + 8 * nnz               # operand values in, sums out
     # plan bytes read per fill: operand masks + tile bases, or segments + item order
     meta = nnz + 4 * nops * ((nnz + 255) // 256) if masked else 4 * (nnz + 1) + 4 * items
@@ -514,6 +533,10 @@ def main_sum(args):
                                 "step = one value fill of the union"),
                    "operand_entries": items, "nnz_out": nnz, "plan_s": t_plan,
                    "bit_exact_vs_oracle": exact},
+        "row_scaled_fill": {"kernel": "k_sum_fill_mask<4,true>", "ms": ms_scaled,
+                            "hbm_frac_alg": (8 * items + 8 * nnz + 8 * nops * m + 4 * nnz)
+                            / (ms_scaled * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                            "bit_exact_vs_oracle": exact_scaled},
         "roofline": {"bound": "hbm",
                      "kernel": "k_sum_fill_mask" if masked else "k_sum_fill_src<false>",
                      "achieved": bytes_ / (ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,

@@ -2,11 +2,13 @@
 // src/sparse/matrix_sum.c, src/sparse/CSR_sum.c).  The alloc concatenates the
 // operands' entries (operand 0 first, each in CSR order) as items, coalesces
 // them on the GPU into the union structure and keeps the plan; the fill sums
-// each output entry's items in that order from +0.0, one thread per entry, so
-// C[r, c] = ((+0.0 + x_0) + x_1) + ... over the operands holding (r, c) -- the
+// each output entry's items in that order from +0.0, so C[r, c] =
+// ((+0.0 + x_0) + x_1) + ... over the operands holding (r, c) -- the
 // reference's accumulator order (CSR_sum.c:150-157 applied operand by
 // operand, matrix_sum.c:142-155; the merge fills CSR_sum.c:84-148 give the
-// same sums for canonical operands).
+// same sums for canonical operands).  Up to 8 operands with sorted unique
+// columns fill through per-entry operand masks (values stream); otherwise
+// through the entry -> item lists.
 #include "api_matrix.h"
 #include "api_pd.h"
 #include "sparse_ops.h"
@@ -85,8 +87,9 @@ struct SumSrc
 // Cx[e] = ((+0.0 + v_1) + v_2) + ... over entry e's items in item order, with
 // v = x_k[t - off[k]] or, SCALED, the rounded product d[k*m + row[t]] * x.
 // The operand of an item comes from compares against the (wave-uniform)
-// offsets, so the only dependent loads are seg -> order -> value; up to four
-// items' value loads are in flight together.
+// offsets, so the only dependent loads are seg -> order -> value.  Items go
+// four at a time with unconditional loads (slots past the entry's last item
+// repeat its first item and are dropped), so four loads are in flight.
 template <bool SCALED>
 __global__ __launch_bounds__(256) void k_sum_fill_src(int64_t nnz, const int *__restrict__ seg,
                                                       const unsigned *__restrict__ order,
@@ -105,27 +108,24 @@ __global__ __launch_bounds__(256) void k_sum_fill_src(int64_t nnz, const int *__
         unsigned t[4];
         double v[4];
 #pragma unroll
-        for (int u = 0; u < 4; u++) t[u] = u < cnt ? order[s + u] : 0u;
+        for (int u = 0; u < 4; u++) t[u] = order[u < cnt ? s + u : s];
 #pragma unroll
         for (int u = 0; u < 4; u++)
         {
-            if (u < cnt)
+            const double *base = src.x[0];
+            int64_t o = 0;
+            int k = 0;
+            for (int j = 1; j < nops; j++)
             {
-                const double *base = src.x[0];
-                int64_t o = 0;
-                int k = 0;
-                for (int j = 1; j < nops; j++)
+                if ((int64_t) t[u] >= src.off[j])
                 {
-                    if ((int64_t) t[u] >= src.off[j])
-                    {
-                        base = src.x[j];
-                        o = src.off[j];
-                        k = j;
-                    }
+                    base = src.x[j];
+                    o = src.off[j];
+                    k = j;
                 }
-                v[u] = base[t[u] - o];
-                if (SCALED) v[u] = d[(int64_t) k * m + row[t[u]]] * v[u];
             }
+            v[u] = base[t[u] - o];
+            if (SCALED) v[u] = d[(int64_t) k * m + row[t[u]]] * v[u];
         }
 #pragma unroll
         for (int u = 0; u < 4; u++)
@@ -193,11 +193,15 @@ __global__ void k_sum_tile_base(int ntiles, int nops, const int *__restrict__ ma
 // load is issued unconditionally (a lane without operand k reads the mask
 // buffer's first word instead and drops it): all SUM_PER * NOPS loads of a
 // lane are in flight together.
-template <int NOPS>
+// SCALED: each term is the rounded product d[k*m + row] * x, the row read from
+// erow (one int per entry).
+template <int NOPS, bool SCALED>
 __global__ __launch_bounds__(256) void k_sum_fill_mask(int64_t nnz, int ntiles,
                                                       const unsigned char *__restrict__ mask,
                                                       const int *__restrict__ base,
                                                       const SumSrc src,
+                                                      const int *__restrict__ erow,
+                                                      const double *__restrict__ d, int m,
                                                       double *__restrict__ Cx)
 {
     const int lane = threadIdx.x & 63;
@@ -226,6 +230,17 @@ __global__ __launch_bounds__(256) void k_sum_fill_mask(int64_t nnz, int ntiles,
             run[k] += __popcll(b);
         }
     }
+    if (SCALED)
+    {
+#pragma unroll
+        for (int u = 0; u < SUM_PER; u++)
+        {
+            const int64_t e = t0 + 64 * u;
+            const int64_t r = erow[e < nnz ? e : nnz - 1];
+#pragma unroll
+            for (int k = 0; k < NOPS; k++) v[u][k] = d[(int64_t) k * m + r] * v[u][k];
+        }
+    }
 #pragma unroll
     for (int u = 0; u < SUM_PER; u++)
     {
@@ -240,10 +255,31 @@ __global__ __launch_bounds__(256) void k_sum_fill_mask(int64_t nnz, int ntiles,
 
 template <int NOPS>
 void launch_fill_mask(unsigned blocks, hipStream_t st, int64_t nnz, int ntiles,
-                      const unsigned char *mask, const int *base, const SumSrc &src, double *Cx)
+                      const unsigned char *mask, const int *base, const SumSrc &src,
+                      const int *erow, const double *d, int m, double *Cx)
 {
-    hipLaunchKernelGGL(k_sum_fill_mask<NOPS>, dim3(blocks), dim3(256), 0, st, nnz, ntiles, mask,
-                       base, src, Cx);
+    if (d)
+        hipLaunchKernelGGL((k_sum_fill_mask<NOPS, true>), dim3(blocks), dim3(256), 0, st, nnz,
+                           ntiles, mask, base, src, erow, d, m, Cx);
+    else
+        hipLaunchKernelGGL((k_sum_fill_mask<NOPS, false>), dim3(blocks), dim3(256), 0, st, nnz,
+                           ntiles, mask, base, src, erow, d, m, Cx);
+}
+
+// erow[e] = row of output entry e (the largest r with Cp[r] <= e).
+__global__ void k_sum_entry_rows(int64_t nnz, int m, const int64_t *__restrict__ Cp,
+                                 int *__restrict__ erow)
+{
+    int64_t e = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nnz) return;
+    int lo = 0, hi = m - 1;
+    while (lo < hi)
+    {
+        const int mid = (lo + hi + 1) >> 1;
+        if (Cp[mid] <= e) lo = mid;
+        else hi = mid - 1;
+    }
+    erow[e] = lo;
 }
 
 // The same fill with the pointers and offsets in device memory (any count).
@@ -314,6 +350,7 @@ struct SumPlan
     int ntiles = 0;
     DBuf<unsigned char> mask;       // nnz operand masks (masked plans)
     DBuf<int> tile_base;            // nops x ntiles (masked plans)
+    DBuf<int> erow;                 // row of every output entry (masked plans keeping rows)
     DBuf<const double *> xs;        // value pointers beyond SUM_ARGS operands
 };
 
@@ -376,6 +413,13 @@ void sum_plan_build(SumPlan &S, int nops, int m, int n, const int *const *dP,
                            0, stream(), S.ntiles, nops, S.map.p, S.doff.p, S.tile_base.p);
         SDL_HIP(hipGetLastError());
     }
+    if (S.masked && keep_rows)
+    {
+        S.erow.alloc((size_t) S.P.nnz);
+        hipLaunchKernelGGL(k_sum_entry_rows, dim3(sum_grid(S.P.nnz)), dim3(256), 0, stream(),
+                           S.P.nnz, m, S.P.Cp.p, S.erow.p);
+        SDL_HIP(hipGetLastError());
+    }
     if (keep_rows) S.row = std::move(orow);
     if (nops > SUM_ARGS) S.xs.alloc(nops);
 }
@@ -394,7 +438,7 @@ void sum_plan_fill(SumPlan &S, const double *const *hX, const double *dd, double
         SumSrc src{};
         for (int k = 0; k < S.nops; k++) src.x[k] = hX[k];
         for (int k = 0; k <= S.nops; k++) src.off[k] = S.off[k];
-        if (S.masked && !dd)
+        if (S.masked)
         {
             const unsigned blocks = (unsigned) ((S.ntiles + 3) / 4);
             auto *fn = &launch_fill_mask<1>;
@@ -409,7 +453,8 @@ void sum_plan_fill(SumPlan &S, const double *const *hX, const double *dd, double
             case 8: fn = &launch_fill_mask<8>; break;
             default: break;
             }
-            fn(blocks, st, S.P.nnz, S.ntiles, S.mask.p, S.tile_base.p, src, dCx);
+            fn(blocks, st, S.P.nnz, S.ntiles, S.mask.p, S.tile_base.p, src, S.erow.p, dd, S.m,
+               dCx);
         }
         else if (dd)
             hipLaunchKernelGGL(k_sum_fill_src<true>, dim3(grid), dim3(256), 0, st, S.P.nnz,

@@ -104,10 +104,14 @@ def test_sum_plan_operand_masks(nops):
     rng = np.random.default_rng(40 + nops)
     m, n = 5000, 3000
     ops = make_ops(rng, m, n, nops, empty_op=1 if nops > 2 else None)
-    S = DevSum(m, n, ops, keep_rows=0)
+    S = DevSum(m, n, ops, keep_rows=1)
     assert lib.sdl_sum_plan_is_masked(S.h)
     check(lib.sdl_sum_fill(S.h, S.dX, None, S.out.ptr, None), "fill")
     assert bits_equal(S.values(), O.sum_n(m, n, ops)[2])
+    d = rng.uniform(-2, 2, (nops, m))
+    dd = DeviceArray.from_numpy(d)
+    check(lib.sdl_sum_fill(S.h, S.dX, dd.ptr, S.out.ptr, None), "scaled fill")
+    assert bits_equal(S.values(), O.sum_n(m, n, ops, d=d)[2])
     S.close()
 
 
