@@ -59,7 +59,7 @@ def parse():
                              "learned", "sum", "dense-kron", "lower-coo"],
                     help="btda: the sparse refill (default line); blocks-*: the stacked "
                          "permuted-dense products (A^T D A, or B^T D A of two stacks)")
-    ap.add_argument("--rmat-scale", type=int, default=18,
+    ap.add_argument("--rmat-scale", type=int, default=20,
                     help="learned: R-MAT scale of the A^T A evaluated beside the banded one")
     ap.add_argument("--epochs", type=int, default=100, help="learned: training epochs")
     ap.add_argument("--taus", type=float, nargs="*", default=[0.99, 0.999],

@@ -199,17 +199,19 @@ template <int S> __device__ __forceinline__ unsigned hash_slot(int col)
     return ((unsigned) col * 2654435761u) >> (32 - LOG2);
 }
 
-// Insert col; returns false if the probe sequence ran over the whole table.
-template <int S> __device__ __forceinline__ bool hash_insert(int *tab, int col)
+// Insert col; returns 1 when col took an empty slot, 0 when it was there
+// already, -1 when the probe sequence ran over the whole table.
+template <int S> __device__ __forceinline__ int hash_insert(int *tab, int col)
 {
     unsigned h = hash_slot<S>(col);
     for (int probe = 0; probe < S; probe++)
     {
         int prev = atomicCAS(&tab[h], -1, col);
-        if (prev == -1 || prev == col) return true;
+        if (prev == -1) return 1;
+        if (prev == col) return 0;
         h = (h + 1) & (S - 1);
     }
-    return false;
+    return -1;
 }
 
 // Ascending bitonic sort of S unsigned keys in LDS by G threads.
@@ -357,12 +359,28 @@ __global__ void __launch_bounds__(G)
     __shared__ int tab[S];
     __shared__ Expand<G> ex;
     __shared__ long long red[G / 64];
+    __shared__ int filled;
     const int row = rows[blockIdx.x];
     for (int i = threadIdx.x; i < S; i += G) tab[i] = -1;
+    if (threadIdx.x == 0) filled = 0;
     __syncthreads();
     RowCtx c = row_ctx(row, Lm, Lp, nL);
-    for_each_term<G>(ex, c.q0, c.q1, c.koff, Li, Ap,
-                     [&](int t, int, int) { hash_insert<S>(tab, Ai[t]); });
+    // Tables hold at least twice their row's bound.  In the count pass a row
+    // past S/2 distinct columns has exceeded its (predicted) bound and will be
+    // counted again from its term bound, so its remaining terms are skipped:
+    // without this, a row far above its table would probe the full table for
+    // every new column.  Exact bounds never get there.
+    for_each_term<G>(ex, c.q0, c.q1, c.koff, Li, Ap, [&](int t, int, int) {
+        if (PASS == 0)
+        {
+            if (*(volatile int *) &filled > S / 2) return;
+            if (hash_insert<S>(tab, Ai[t]) == 1) atomicAdd(&filled, 1);
+        }
+        else
+        {
+            hash_insert<S>(tab, Ai[t]);
+        }
+    });
     if (PASS == 0)
     {
         long long cnt = 0;
