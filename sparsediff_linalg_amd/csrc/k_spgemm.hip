@@ -1309,6 +1309,11 @@ constexpr int RANK_MAX_WORDS = 16384;      // 524,288 columns
 constexpr int RANK_MAX_ENTRIES = 65536;
 constexpr int RANK_LIST = 1024;
 constexpr int RU = 16;                     // terms per thread per step of the rank passes
+constexpr int RU2 = 8;                     // the same in pass 2 (more registers per term)
+template <bool B> struct BoolC
+{
+    static constexpr bool value = B;
+};
 
 __device__ __forceinline__ int rank_of(const unsigned *bm, const unsigned short *sub,
                                        const unsigned *sup, int col)
@@ -1364,7 +1369,8 @@ __global__ void __launch_bounds__(G)
             for (int u = 0; u < RU; u++)
             {
                 const int i = i0 + u * G;
-                col[u] = i < cnt ? Ci[cbase + i] : -1;
+                const int cc = Ci[cbase + (i < cnt ? i : cnt - 1)];
+                col[u] = i < cnt ? cc : -1;
             }
 #pragma unroll
             for (int u = 0; u < RU; u++)
@@ -1388,7 +1394,8 @@ __global__ void __launch_bounds__(G)
         __syncthreads();
         RowCtx c = row_ctx(row, Lm, Lp, nL);
         // pass 1: entries hit once / more than once (RU terms per thread at a
-        // time, their loads issued together)
+        // time, their loads issued together: slots past the chunk repeat its
+        // last term and are dropped, so no load sits behind a branch)
         for (int qb = c.q0; qb < c.q1; qb += G)
         {
             const int tot = expand_chunk<G>(ex, qb, c.q1, c.koff, Li, Ap);
@@ -1401,8 +1408,9 @@ __global__ void __launch_bounds__(G)
                 {
                     const int fi = f0 + u * G;
                     int t = 0, q, k;
-                    if (fi < tot) expand_term<G>(ex, nq, fi, t, q, k);
-                    col[u] = fi < tot ? Ai[t] : -1;
+                    expand_term<G>(ex, nq, fi < tot ? fi : tot - 1, t, q, k);
+                    const int a = Ai[t];
+                    col[u] = fi < tot ? a : -1;
                 }
 #pragma unroll
                 for (int u = 0; u < RU; u++)
@@ -1420,29 +1428,42 @@ __global__ void __launch_bounds__(G)
         {
             const int tot = expand_chunk<G>(ex, qb, c.q1, c.koff, Li, Ap);
             const int nq = min(G, c.q1 - qb);
-            for (int f0 = threadIdx.x; f0 < tot; f0 += RU * G)
+            for (int f0 = threadIdx.x; f0 < tot; f0 += RU2 * G)
             {
-                int col[RU], qq[RU];
-                double x[RU];
+                int col[RU2], qq[RU2];
+                double x[RU2];
+                // unconditional loads (slots past the chunk repeat its last
+                // term); the permutation / scaling choice is made outside the
+                // unrolled loop so no branch separates the terms' loads
+                auto load_terms = [&](auto perm_c, auto d_c) {
+                    constexpr bool HP = decltype(perm_c)::value, HD = decltype(d_c)::value;
 #pragma unroll
-                for (int u = 0; u < RU; u++)
-                {
-                    const int fi = f0 + u * G;
-                    int t = 0, q = 0, k = 0;
-                    if (fi < tot) expand_term<G>(ex, nq, fi, t, q, k);
-                    col[u] = -1;
-                    qq[u] = q;
-                    x[u] = 0.0;
-                    if (fi < tot)
+                    for (int u = 0; u < RU2; u++)
                     {
-                        col[u] = Ai[t];
-                        double v = Lval[Lperm ? Lperm[q] : q];
-                        if (d) v = v * d[k];
-                        x[u] = v * Ax[t];
+                        const int fi = f0 + u * G;
+                        int t = 0, q = 0, k = 0;
+                        expand_term<G>(ex, nq, fi < tot ? fi : tot - 1, t, q, k);
+                        const int a = Ai[t];
+                        double v = Lval[HP ? Lperm[q] : q];
+                        if (HD) v = v * d[k];
+                        const double xv = v * Ax[t];
+                        col[u] = fi < tot ? a : -1;
+                        qq[u] = q;
+                        x[u] = xv;
                     }
+                };
+                if (Lperm)
+                {
+                    if (d) load_terms(BoolC<true>{}, BoolC<true>{});
+                    else load_terms(BoolC<true>{}, BoolC<false>{});
+                }
+                else
+                {
+                    if (d) load_terms(BoolC<false>{}, BoolC<true>{});
+                    else load_terms(BoolC<false>{}, BoolC<false>{});
                 }
 #pragma unroll
-                for (int u = 0; u < RU; u++)
+                for (int u = 0; u < RU2; u++)
                 {
                     if (col[u] < 0) continue;
                     const int pos = rank_of(bm, sub, sup, col[u]);
