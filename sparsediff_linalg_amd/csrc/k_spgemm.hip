@@ -193,6 +193,40 @@ __device__ __forceinline__ void expand_term(const Expand<G> &ex, int nq, int fi,
     k = ex.k[lo];
 }
 
+// Calls f(col) for the column of every term of output row (b, r), U terms
+// per thread at a time with their column loads issued together (slots past
+// a chunk repeat its last term and are dropped, so no load sits behind a
+// branch).  All G threads must call it.
+template <int G, int U, typename F>
+__device__ __forceinline__ void for_each_col(Expand<G> &ex, int q0, int q1, int koff,
+                                             const int *__restrict__ Li,
+                                             const int *__restrict__ Ap,
+                                             const int *__restrict__ Ai, F &&f)
+{
+    for (int qb = q0; qb < q1; qb += G)
+    {
+        const int tot = expand_chunk<G>(ex, qb, q1, koff, Li, Ap);
+        const int nq = min(G, q1 - qb);
+        for (int f0 = (int) threadIdx.x; f0 < tot; f0 += U * G)
+        {
+            int col[U];
+#pragma unroll
+            for (int u = 0; u < U; u++)
+            {
+                const int fi = f0 + u * G;
+                int t, q, k;
+                expand_term<G>(ex, nq, fi < tot ? fi : tot - 1, t, q, k);
+                const int a = Ai[t];
+                col[u] = fi < tot ? a : -1;
+            }
+#pragma unroll
+            for (int u = 0; u < U; u++)
+                if (col[u] >= 0) f(col[u]);
+        }
+        __syncthreads();
+    }
+}
+
 template <int S> __device__ __forceinline__ unsigned hash_slot(int col)
 {
     constexpr int LOG2 = __builtin_ctz(S);
@@ -370,15 +404,15 @@ __global__ void __launch_bounds__(G)
     // counted again from its term bound, so its remaining terms are skipped:
     // without this, a row far above its table would probe the full table for
     // every new column.  Exact bounds never get there.
-    for_each_term<G>(ex, c.q0, c.q1, c.koff, Li, Ap, [&](int t, int, int) {
+    for_each_col<G, 4>(ex, c.q0, c.q1, c.koff, Li, Ap, Ai, [&](int col) {
         if (PASS == 0)
         {
             if (*(volatile int *) &filled > S / 2) return;
-            if (hash_insert<S>(tab, Ai[t]) == 1) atomicAdd(&filled, 1);
+            if (hash_insert<S>(tab, col) == 1) atomicAdd(&filled, 1);
         }
         else
         {
-            hash_insert<S>(tab, Ai[t]);
+            hash_insert<S>(tab, col);
         }
     });
     if (PASS == 0)
@@ -430,10 +464,10 @@ __global__ void __launch_bounds__(G)
         if (GLOBAL) __threadfence();
         __syncthreads();
         RowCtx c = row_ctx(row, Lm, Lp, nL);
-        for_each_term<G>(ex, c.q0, c.q1, c.koff, Li, Ap, [&](int t, int, int) {
-            int col = Ai[t];
-            atomicOr(&bm[col >> 5], 1u << (col & 31));
-        });
+        // (four loads in flight for LDS bitmaps; one for global ones, whose L2
+        // atomics measured slower batched)
+        for_each_col<G, GLOBAL ? 1 : 4>(ex, c.q0, c.q1, c.koff, Li, Ap, Ai,
+                                        [&](int col) { atomicOr(&bm[col >> 5], 1u << (col & 31)); });
         if (GLOBAL) __threadfence();
         __syncthreads();
         // contiguous word range per thread keeps the output ascending
