@@ -18,15 +18,33 @@ typedef double d4 __attribute__((ext_vector_type(4)));
 
 unsigned grid_for(int64_t n, int b = 256) { return (unsigned) std::max<int64_t>(1, (n + b - 1) / b); }
 
-// pos null: the listed outputs are all outputs, in order
-__global__ void k_pull_sum(int n, const int *__restrict__ pos, const int *__restrict__ seg,
-                           const int *__restrict__ idx, const double *__restrict__ src,
-                           double *__restrict__ out)
+// pos null: the listed outputs are all outputs, in order.  Sources go four at
+// a time with unconditional loads (indices past the output's last source
+// repeat its first, whose value is then dropped), so four gathers are in
+// flight per thread.
+__global__ __launch_bounds__(256) void k_pull_sum(int n, const int *__restrict__ pos,
+                                                 const int *__restrict__ seg,
+                                                 const int *__restrict__ idx,
+                                                 const double *__restrict__ src,
+                                                 double *__restrict__ out)
 {
     int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= n) return;
+    const int s1 = seg[e + 1];
     double acc = 0.0;
-    for (int s = seg[e]; s < seg[e + 1]; s++) acc = acc + src[idx[s]];
+    for (int s = seg[e]; s < s1; s += 4)
+    {
+        const int cnt = s1 - s;
+        int ix[4];
+        double v[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) ix[u] = idx[u < cnt ? s + u : s];
+#pragma unroll
+        for (int u = 0; u < 4; u++) v[u] = src[ix[u]];
+#pragma unroll
+        for (int u = 0; u < 4; u++)
+            if (u < cnt) acc = acc + v[u];
+    }
     out[pos ? pos[e] : e] = acc;
 }
 
