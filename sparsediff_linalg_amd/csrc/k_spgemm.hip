@@ -1268,18 +1268,57 @@ __global__ void __launch_bounds__(G)
     if (CAP == 0) __threadfence_block();
     __syncthreads();
     RowCtx c = row_ctx(row, Lm, Lp, nL);
-    for (int q = c.q0; q < c.q1; q++)
+    // L entries go G at a time: their A-row ranges and scale factors are
+    // loaded together into LDS, and each A row's first G terms are loaded
+    // before the previous row's updates (unconditionally: index 0 stands in
+    // for a lane without a term), so the dependent loads of consecutive k
+    // overlap the barrier between them.  Updates stay in ascending k.
+    __shared__ int m_a[G], m_b[G];
+    __shared__ double m_v[G];
+    for (int qb = c.q0; qb < c.q1; qb += G)
     {
-        int k = Li[q] + c.koff;
-        double v = Lval[Lperm ? Lperm[q] : q];
-        if (d) v = v * d[k];
-        for (int t = Ap[k] + (int) threadIdx.x; t < Ap[k + 1]; t += G)
+        const int nqc = min(G, c.q1 - qb);
+        if ((int) threadIdx.x < nqc)
         {
-            int pos = lower_bound_int(cols, cnt, Ai[t]);
-            val[pos] = val[pos] + v * Ax[t];
+            const int q = qb + (int) threadIdx.x;
+            const int k = Li[q] + c.koff;
+            double v = Lval[Lperm ? Lperm[q] : q];
+            if (d) v = v * d[k];
+            m_a[threadIdx.x] = Ap[k];
+            m_b[threadIdx.x] = Ap[k + 1];
+            m_v[threadIdx.x] = v;
         }
-        if (CAP == 0) __threadfence_block();
         __syncthreads();
+        int tc = m_a[0] + (int) threadIdx.x;
+        bool hc = tc < m_b[0];
+        int colc = Ai[hc ? tc : 0];
+        double xc = Ax[hc ? tc : 0];
+        for (int j = 0; j < nqc; j++)
+        {
+            const int b = m_b[j];
+            const double v = m_v[j];
+            const bool more = j + 1 < nqc;
+            const int tn = (more ? m_a[j + 1] : 0) + (int) threadIdx.x;
+            const bool hn = more && tn < m_b[j + 1];
+            const int coln = Ai[hn ? tn : 0];
+            const double xn = Ax[hn ? tn : 0];
+            if (hc)
+            {
+                const int pos = lower_bound_int(cols, cnt, colc);
+                val[pos] = val[pos] + v * xc;
+            }
+            for (int t = tc + G; t < b; t += G)  // A rows longer than G
+            {
+                const int pos = lower_bound_int(cols, cnt, Ai[t]);
+                val[pos] = val[pos] + v * Ax[t];
+            }
+            if (CAP == 0) __threadfence_block();
+            __syncthreads();
+            tc = tn;
+            hc = hn;
+            colc = coln;
+            xc = xn;
+        }
     }
     if (CAP > 0)
         for (int i = threadIdx.x; i < cnt; i += G) Cx[cbase + i] = s_val[i];
