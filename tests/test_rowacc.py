@@ -1,5 +1,6 @@
-"""GPU: the row-accumulator refills of products whose output rows are long
-(more than 1024 entries), whole products bit for bit against the oracle.
+"""GPU: the row-accumulator refills, whole products bit for bit against the
+oracle: output rows longer than 1024 entries, and short rows built from A
+rows longer than a workgroup.
 Rows of up to 65536 entries with n <= 524,288 columns go by rank (column
 bitmap + ranks in LDS); rows whose multi-term list overflows, and every long
 row when n is larger, take the dense per-workgroup accumulator."""
@@ -83,3 +84,13 @@ def test_global_bitmap_symbolic_matches_oracle():
     Ap, Ai, Ax = hub_csr(6, m, n, 0, 8, 5000)
     lens = refill_and_compare(m, n, Ap, Ai, Ax, 34)
     assert (lens == 5000).sum() == 5000
+
+
+def test_short_rows_with_long_a_rows_match_oracle():
+    """Output rows of at most 1000 entries (the LDS row accumulator) built from
+    A rows of 300 entries, longer than the workgroup: each k's first 256
+    terms come from the pipelined loads, the rest from the tail loop."""
+    m, n = 400, 1000
+    Ap, Ai, Ax = hub_csr(8, m, n, 300, 0, 0)
+    lens = refill_and_compare(m, n, Ap, Ai, Ax, 81)
+    assert lens.max() <= 1024 and lens.min() > 256
