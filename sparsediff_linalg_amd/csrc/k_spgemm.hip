@@ -1341,6 +1341,8 @@ __global__ void __launch_bounds__(G)
                        double *__restrict__ scratch, int64_t n)
 {
     double *acc = scratch + (int64_t) blockIdx.x * n;
+    __shared__ int m_a[G], m_b[G];
+    __shared__ double m_v[G];
     if (nrows_dev) nrows = *nrows_dev;
     for (int ri = blockIdx.x; ri < nrows; ri += gridDim.x)
     {
@@ -1349,17 +1351,47 @@ __global__ void __launch_bounds__(G)
         for (int64_t i = threadIdx.x; i < cnt; i += G) acc[Ci[cbase + i]] = 0.0;
         __syncthreads();
         RowCtx c = row_ctx(row, Lm, Lp, nL);
-        for (int q = c.q0; q < c.q1; q++)
+        // as k_num_rowacc: the chunk's A-row ranges and scale factors in LDS,
+        // each A row's first G terms loaded before the previous row's updates
+        for (int qb = c.q0; qb < c.q1; qb += G)
         {
-            const int k = Li[q] + c.koff;
-            double v = Lval[Lperm ? Lperm[q] : q];
-            if (d) v = v * d[k];
-            for (int t = Ap[k] + (int) threadIdx.x; t < Ap[k + 1]; t += G)
+            const int nqc = min(G, c.q1 - qb);
+            if ((int) threadIdx.x < nqc)
             {
-                const int j = Ai[t];
-                acc[j] = acc[j] + v * Ax[t];
+                const int q = qb + (int) threadIdx.x;
+                const int k = Li[q] + c.koff;
+                double v = Lval[Lperm ? Lperm[q] : q];
+                if (d) v = v * d[k];
+                m_a[threadIdx.x] = Ap[k];
+                m_b[threadIdx.x] = Ap[k + 1];
+                m_v[threadIdx.x] = v;
             }
             __syncthreads();
+            int tc = m_a[0] + (int) threadIdx.x;
+            bool hc = tc < m_b[0];
+            int colc = Ai[hc ? tc : 0];
+            double xc = Ax[hc ? tc : 0];
+            for (int jq = 0; jq < nqc; jq++)
+            {
+                const int b = m_b[jq];
+                const double v = m_v[jq];
+                const bool more = jq + 1 < nqc;
+                const int tn = (more ? m_a[jq + 1] : 0) + (int) threadIdx.x;
+                const bool hn = more && tn < m_b[jq + 1];
+                const int coln = Ai[hn ? tn : 0];
+                const double xn = Ax[hn ? tn : 0];
+                if (hc) acc[colc] = acc[colc] + v * xc;
+                for (int t = tc + G; t < b; t += G)  // A rows longer than G
+                {
+                    const int j = Ai[t];
+                    acc[j] = acc[j] + v * Ax[t];
+                }
+                __syncthreads();
+                tc = tn;
+                hc = hn;
+                colc = coln;
+                xc = xn;
+            }
         }
         for (int64_t i = threadIdx.x; i < cnt; i += G) Cx[cbase + i] = acc[Ci[cbase + i]];
         __syncthreads();
