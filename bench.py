@@ -114,7 +114,7 @@ def main():
 
     lib = sdl.load()
     K = args.K
-    w = workloads.btda_config1(1)
+    w = workloads.btda_20000x5000(1)
     m, nA, nB = w["m"], w["nA"], w["nB"]
     nnzA, nnzB = int(w["Ap"][-1]), int(w["Bp"][-1])
 
@@ -215,7 +215,7 @@ def main():
     value = K * nnz / (ms_step * 1e-3)
     achieved = bytes_K / (ms_step * 1e-3) / 1e9
 
-    traffic = measured_traffic("btda_refill_config1_traffic.json")
+    traffic = measured_traffic("btda_refill_20000x5000_traffic.json")
 
     cpu = None
     if not args.no_cpu:
@@ -235,7 +235,7 @@ def main():
         "dtype": "f64",
         "data": "synthetic",
         "config": {
-            "workload": ("config 1: C = B^T diag(d) A, A and B 20000x5000 CSR, 6 distinct uniform "
+            "workload": ("C = B^T diag(d) A, A and B 20000x5000 CSR, 6 distinct uniform "
                          "columns/row, FP64 N(0,1), d U[0.1,10], seed 1; one step = one batched "
                          f"refill of C for K={K} fresh d vectors"),
             "K": K, "nnz_out": nnz, "terms": terms, "m": m, "n_A": nA, "n_B": nB,
@@ -1096,7 +1096,7 @@ def main_blocks(args):
     lib = sdl.load()
     if args.gemm_slice:
         check(0 if lib.sdl_block_gemm_slice(args.gemm_slice) > 0 else 1, "gemm slice")
-    w = workloads.blocks_config3(3)
+    w = workloads.stacked_pd_blocks(3)
     m, N = w["m"], w["n"]
     btda = args.workload == "blocks-btda"
 
@@ -1272,7 +1272,7 @@ def cpu_baseline(w, nnz, seconds):
         "unit": "nnz-out/s",
         "cores": 1,
         "kind": kind,
-        "sample": (f"config 1 product, alloc (CSC mirror + Gustavson + plan) once per pass of 64 "
+        "sample": (f"20000x5000 B^T D A product, alloc (CSC mirror + Gustavson + plan) once per pass of 64 "
                    f"d vectors, {count} single-thread refills with fresh d; fill time "
                    f"{fill_s:.2f} s, alloc {alloc_s * 1e3:.1f} ms"),
         "us_per_refill": fill_s / count * 1e6,

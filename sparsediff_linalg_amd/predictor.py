@@ -156,13 +156,13 @@ def training_products(small=False, structured=False):
     banded matrices and of an R-MAT graph."""
     from . import workloads as W
     out = []
-    w1 = W.btda_config1(1, m=4000 if small else 20000, n=1000 if small else 5000)
+    w1 = W.btda_20000x5000(1, m=4000 if small else 20000, n=1000 if small else 5000)
     out.append(("btda", (w1["Ap"], w1["Ai"]), (w1["Bp"], w1["Bi"]), w1["nA"], w1["nB"]))
     for m, n in ((20000, 5000), (50000, 20000)) if small else ((200000, 50000), (100000, 10000),
                                                                (400000, 200000)):
         p, i, _ = W.zipf_csr(2 + m // 1000, m, n)
         out.append((f"zipf{m}x{n}", (p, i), None, n, n))
-    wb = W.blocks_config3(3, nblocks=32 if small else 256)
+    wb = W.stacked_pd_blocks(3, nblocks=32 if small else 256)
     p, i, _ = W.blocks_as_csr(wb)
     out.append(("blocks", (p, i), None, wb["n"], wb["n"]))
     if structured:

@@ -31,7 +31,7 @@ def normal(seed: int, count: int):
     return out
 
 
-def btda_config1(seed: int = 1, m: int = 20000, n: int = 5000, nnz_row: int = 6):
+def btda_20000x5000(seed: int = 1, m: int = 20000, n: int = 5000, nnz_row: int = 6):
     """A and B independent m x n CSR with exactly nnz_row distinct uniform columns
     per row (values N(0,1)); d uniform [0.1, 10]."""
     Ap, Ai, Ax = csr_uniform(seed * 1000 + 1, m, n, nnz_row, nnz_row)
@@ -40,7 +40,7 @@ def btda_config1(seed: int = 1, m: int = 20000, n: int = 5000, nnz_row: int = 6)
     return dict(m=m, nA=n, nB=n, Ap=Ap, Ai=Ai, Ax=Ax, Bp=Bp, Bi=Bi, Bx=Bx, d=d)
 
 
-def blocks_config3(seed: int = 3, nblocks: int = 256, lo: int = 64, hi: int = 512,
+def stacked_pd_blocks(seed: int = 3, nblocks: int = 256, lo: int = 64, hi: int = 512,
                    ncols: int = 4096):
     """Stacked permuted-dense operands: nblocks dense FP64 N(0,1) blocks, m0 and
     n0 independently log-uniform on [lo, hi] (rounded); the rows are a random
@@ -170,7 +170,7 @@ def rmat_csr(scale: int, edges_per_node: int = 16, a: float = 0.57, b: float = 0
 
 
 def blocks_as_csr(w):
-    """CSR (m x n) of a stacked permuted-dense operand from blocks_config3."""
+    """CSR (m x n) of a stacked permuted-dense operand from stacked_pd_blocks."""
     m, n = w["m"], w["n"]
     rows = [None] * m
     for rp, cp, X in w["A"]:

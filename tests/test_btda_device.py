@@ -131,8 +131,8 @@ def test_btda_empty_operands():
     P.close()
 
 
-def test_btda_config1_bit_exact_single_and_batched():
-    w = workloads.btda_config1(1)
+def test_btda_20000x5000_bit_exact_single_and_batched():
+    w = workloads.btda_20000x5000(1)
     m, nA, nB = w["m"], w["nA"], w["nB"]
     Cp, Ci, Cx, terms = O.btda(m, nA, nB, w["Ap"], w["Ai"], w["Ax"], w["Bp"], w["Bi"], w["Bx"],
                                w["d"])

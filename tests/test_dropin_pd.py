@@ -261,7 +261,7 @@ def test_dense_kron(jk, layout):
     L.free_matrix(Ah)
 
 
-def test_config3_scale_atda_spd():
+def test_stacked_blocks_scale_atda_spd():
     """256 blocks with sizes log-uniform 64..512 (the config-3 shape, smaller
     column space so blocks overlap): A^T D A of the stack against dense."""
     L = D.lib()
@@ -357,13 +357,13 @@ def test_pd_reference_literal(c):
 
 
 @pytest.mark.parametrize("case", ["atda", "btda"])
-def test_config3_full_size(case):
+def test_stacked_blocks_full_size(case):
     """The full stacked permuted-dense workload (256 blocks, m0/n0 log-uniform
     64..512, col_perms in [0, 4096)): output structure exactly the oracle's,
     values on 2000 sampled cells within 1e-13 * sum |terms|."""
     from sparsediff_linalg_amd import workloads
     L = D.lib()
-    w = workloads.blocks_config3(3)
+    w = workloads.stacked_pd_blocks(3)
     m, N, d = w["m"], w["n"], w["d"]
     Ao = PO.spd(m, N, [PO.pd(m, N, rp.tolist(), cp.tolist(), X) for rp, cp, X in w["A"]])
     Ah = make(Ao)

@@ -300,9 +300,9 @@ def test_btda_matches_reference_build(seed):
 
 
 @needs_ref
-def test_config1_matches_reference_build():
+def test_btda_20000x5000_matches_reference_build():
     from sparsediff_linalg_amd import workloads
-    w = workloads.btda_config1(1)
+    w = workloads.btda_20000x5000(1)
     Cp, Ci, Cx, terms = O.btda(w["m"], w["nA"], w["nB"], w["Ap"], w["Ai"], w["Ax"], w["Bp"],
                                w["Bi"], w["Bx"], w["d"])
     Rp, Ri, Rx = O.ref_btda(w["m"], w["nA"], w["nB"], w["Ap"], w["Ai"], w["Ax"], w["Bp"],

@@ -111,7 +111,7 @@ def cases():
     yield "zipf", (p, i), None, 4000, 4000
     p, i, _ = W.rmat_csr(12)
     yield "rmat12", (p, i), None, 1 << 12, 1 << 12
-    wb = W.blocks_config3(5, nblocks=12, lo=16, hi=64, ncols=512)
+    wb = W.stacked_pd_blocks(5, nblocks=12, lo=16, hi=64, ncols=512)
     p, i, _ = W.blocks_as_csr(wb)
     yield "blocks", (p, i), None, 512, 512
     # wide: the term bound asks for 128 KB column bitmaps, so predictions apply
