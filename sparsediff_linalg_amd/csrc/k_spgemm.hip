@@ -1456,6 +1456,7 @@ __global__ void __launch_bounds__(G)
     for (int ri = blockIdx.x; ri < nrows; ri += gridDim.x)
     {
         const int row = rows[ri];
+        const RowCtx c = row_ctx(row, Lm, Lp, nL);   // loads issued with Cp's
         const int64_t cbase = Cp[row];
         const int cnt = (int) (Cp[row + 1] - cbase);
         const int swords = (cnt + 31) >> 5;
@@ -1497,13 +1498,16 @@ __global__ void __launch_bounds__(G)
             }
         }
         __syncthreads();
-        RowCtx c = row_ctx(row, Lm, Lp, nL);
+        // a row of one L chunk keeps pass 1's expansion for pass 2
+        const bool one_chunk = c.q1 - c.q0 <= G;
+        int tot1 = 0;
         // pass 1: entries hit once / more than once (RU terms per thread at a
         // time, their loads issued together: slots past the chunk repeat its
         // last term and are dropped, so no load sits behind a branch)
         for (int qb = c.q0; qb < c.q1; qb += G)
         {
             const int tot = expand_chunk<G>(ex, qb, c.q1, c.koff, Li, Ap);
+            tot1 = tot;
             const int nq = min(G, c.q1 - qb);
             for (int f0 = threadIdx.x; f0 < tot; f0 += RU * G)
             {
@@ -1531,7 +1535,7 @@ __global__ void __launch_bounds__(G)
         // pass 2: single-term entries written, the others' terms listed
         for (int qb = c.q0; qb < c.q1; qb += G)
         {
-            const int tot = expand_chunk<G>(ex, qb, c.q1, c.koff, Li, Ap);
+            const int tot = one_chunk ? tot1 : expand_chunk<G>(ex, qb, c.q1, c.koff, Li, Ap);
             const int nq = min(G, c.q1 - qb);
             for (int f0 = threadIdx.x; f0 < tot; f0 += RU2 * G)
             {
