@@ -1142,7 +1142,7 @@ def main_blocks(args):
     lib.sdl_synchronize()
     lib.sdl_event_record(e0, stream)
     for i in range(args.steps):
-        a, b = ev[i]
+        a, b = ev[i] if i < len(ev) else (None, None)
         check(lib.sdl_block_refill_device(Cm, left, A, pools[i % NPOOL].ptr, a, b), "refill")
     lib.sdl_event_record(e1, stream)
     lib.sdl_synchronize()
