@@ -65,7 +65,10 @@ __global__ void k_lower_entries(int64_t total, int m, const int64_t *__restrict_
 // are contiguous runs of LG values.  (Measured slower: one wave per row, whose
 // second pass leaves most lanes idle on ~65-entry rows; a wave per 256 export
 // entries with a per-lane row walk, whose data-dependent walk serialises the
-// loads.)
+// loads; non-temporal stores, 19 % slower: a row's segment shares its first and
+// last 128-byte lines with its neighbours', and ordinary stores let L2 merge
+// those partial lines before they reach HBM; 32 lanes x 4 or 16 lanes x 4
+// entries, 6 % slower.)
 constexpr int LG = 16, LU = 8;
 __global__ __launch_bounds__(256) void k_lower_refresh(int m, const int64_t *__restrict__ off,
                                                       const int64_t *__restrict__ Cp,
@@ -89,7 +92,7 @@ __global__ __launch_bounds__(256) void k_lower_refresh(int m, const int64_t *__r
         for (int u = 0; u < LU; u++)
         {
             const int64_t i = b + gl + LG * u;
-            if (i < len) __builtin_nontemporal_store(v[u], x + d0 + i);
+            if (i < len) x[d0 + i] = v[u];
         }
     }
 }
