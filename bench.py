@@ -73,6 +73,22 @@ def parse():
     return ap.parse_args()
 
 
+def pinned_cpu(fn, *a, pin=True, **kw):
+    """Runs a CPU-baseline timing, pinned to one core when the baseline is
+    single-threaded, and adds the host facts (CPU model, nproc, pinning) to the
+    record it returns."""
+    from oracle import cpu_env
+    if not pin:
+        rec = fn(*a, **kw)
+        rec.update({"cpu_model": cpu_env.cpu_model(), "nproc": cpu_env.nproc(),
+                    "pinning": "none (multi-threaded)"})
+        return rec
+    with cpu_env.pinned() as facts:
+        rec = fn(*a, **kw)
+    rec.update(facts)
+    return rec
+
+
 def measured_traffic(name):
     """HBM bytes per launch measured by rocprofv3 counters (FETCH_SIZE and
     WRITE_SIZE runs, scaled by a streaming-copy calibration), from the summary
@@ -351,8 +367,8 @@ def main_ata_large(args):
                      "alg_bytes_per_launch": bytes_},
         "gflops": (nnzA + 2 * terms) / t / 1e9,
         "symbolic_terms_per_s": terms / t_sym,
-        "cpu_baseline": None if args.no_cpu else ata_cpu_baseline(m, n, Ap, Ai, Ax, d_last,
-                                                                  args.cpu_seconds),
+        "cpu_baseline": None if args.no_cpu else pinned_cpu(ata_cpu_baseline, m, n, Ap, Ai, Ax,
+                                                             d_last, args.cpu_seconds),
     }
     pl.close()
     print(json.dumps(out), flush=True)
@@ -439,7 +455,7 @@ def main_coo(args):
                      "traffic": None, "alg_bytes_per_launch": bytes_},
         "plan_triplets_per_s": nin / t_plan,
         "alg_bytes_whole_coalesce": bytes_all,
-        "cpu_baseline": None if args.no_cpu else coo_cpu_baseline(rows, cols, x, m, n),
+        "cpu_baseline": None if args.no_cpu else pinned_cpu(coo_cpu_baseline, rows, cols, x, m, n),
     }
     lib.sdl_coo_plan_destroy(h)
     print(json.dumps(out_json), flush=True)
@@ -543,7 +559,8 @@ def main_sum(args):
                      "unit": "GB/s", "frac": bytes_ / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
                      "traffic": measured_traffic("k_sum_fill_mask_hessian_sum_traffic.json"),
                      "alg_bytes_per_launch": bytes_, "plan_bytes_per_launch": meta},
-        "cpu_baseline": None if args.no_cpu else sum_cpu_baseline(ops, m, n, args.cpu_seconds),
+        "cpu_baseline": None if args.no_cpu else pinned_cpu(sum_cpu_baseline, ops, m, n,
+                                                             args.cpu_seconds),
     }
     lib.sdl_sum_plan_destroy(h)
     print(json.dumps(out_json), flush=True)
@@ -707,7 +724,8 @@ def main_dense_kron(args):
                      "frac": flops / (ms * 1e-3) / 1e12 / 78.6, "traffic": None,
                      "note": ("separate FP64 multiply and add per term (the reference's rounding): "
                               "the attainable ceiling is half the FMA peak")},
-        "cpu_baseline": None if args.no_cpu else dense_kron_cpu_baseline(m, k, n, Y, X, w),
+        "cpu_baseline": None if args.no_cpu else pinned_cpu(dense_kron_cpu_baseline, m, k, n, Y, X,
+                                                             w),
     }
     print(json.dumps(out_json), flush=True)
 
@@ -868,7 +886,8 @@ def main_learned(args):
         "exact_terms_per_s": tot_terms / tot_exact,
         "gain_vs_term_bound": tot_exact / tot_pred,
         "roofline": None,
-        "cpu_baseline": None if args.no_cpu else symbolic_cpu_baseline(args.cpu_seconds),
+        "cpu_baseline": None if args.no_cpu else pinned_cpu(symbolic_cpu_baseline,
+                                                             args.cpu_seconds),
     }
     for mlp, _ in nets.values():
         mlp.close()
@@ -1074,7 +1093,7 @@ def main_multi(args):
                       "launches_per_round": sum(len(g.ranges) for g in mp.g),
                       "nnz_out_per_s": mp.nnz / (ms_plain_round * 1e-3)},
         "batched_vs_plain": ms_plain_round / (ms_graph / R),
-        "cpu_baseline": None if args.no_cpu else multi_cpu_baseline(w, args.cpu_seconds),
+        "cpu_baseline": None if args.no_cpu else pinned_cpu(multi_cpu_baseline, w, args.cpu_seconds),
     }
     lib.sdl_graph_destroy(graph)
     mp.close()
@@ -1185,7 +1204,8 @@ def main_blocks(args):
     bytes_alg = 8 * (XA_cells + XB_cells + m + out_cells)
     tfs = flops / (ms_step * 1e-3) / 1e12
     gbs = bytes_alg / (ms_step * 1e-3) / 1e9
-    cpu = None if args.no_cpu else blocks_cpu_baseline(w, btda, args.cpu_seconds)
+    cpu = None if args.no_cpu else pinned_cpu(blocks_cpu_baseline, w, btda, args.cpu_seconds,
+                                              pin=False)
     out = {
         "metric": "stacked permuted-dense B^T·D·A GFLOP/s + % FP64 MFMA peak",
         "value": flops / (ms_step * 1e-3) / 1e9,
@@ -1243,39 +1263,44 @@ def blocks_cpu_baseline(w, btda, seconds):
 
 
 def cpu_baseline(w, nnz, seconds):
-    """Single-thread CPU path on a bounded sample: one alloc per pass, then
-    fills with fresh d (same product as the GPU workload), until about
-    `seconds` of fill time."""
+    """Single-thread CPU path on a bounded sample: the C restatement of the
+    reference's sparse path (oracle/liboracle.so; its bit-identity and timing
+    against the reference's own sources are recorded in
+    tests/golden/oracle_faithfulness.json), pinned to one core.  One alloc per
+    pass of 64 fresh d vectors, then one fill per d, until about `seconds` of
+    fill time; the value is from the median fill time (after one warm-up
+    pass)."""
     try:
+        from oracle import cpu_env
         from oracle import oracle as O
     except Exception as e:  # pragma: no cover
         return {"error": str(e)}
+    import statistics
+
     from sparsediff_linalg_amd import workloads
     ds = [workloads.uniform(50_000 + r, w["m"], 0.1, 10.0) for r in range(64)]
-    kind = "reference" if O.ref_available() else "port"
-    fill_s = 0.0
-    count = 0
-    alloc_s = None
-    t_start = time.perf_counter()
-    while fill_s < seconds and time.perf_counter() - t_start < 3 * seconds:
-        if kind == "reference":
-            a, fills, _ = O.ref_fill_timer(w["m"], w["nA"], w["nB"], w["Ap"], w["Ai"], w["Ax"],
-                                            w["Bp"], w["Bi"], w["Bx"], ds)
-        else:
-            a, fills = O.port_fill_timer(w["m"], w["nA"], w["nB"], w["Ap"], w["Ai"], w["Ax"],
-                                         w["Bp"], w["Bi"], w["Bx"], ds)
-        alloc_s = a if alloc_s is None else min(alloc_s, a)
-        fill_s += sum(fills)
-        count += len(fills)
+    args_w = (w["m"], w["nA"], w["nB"], w["Ap"], w["Ai"], w["Ax"], w["Bp"], w["Bi"], w["Bx"])
+    fills_all, allocs = [], []
+    with cpu_env.pinned() as facts:
+        O.port_fill_timer(*args_w, ds[:4])  # warm-up
+        t_start = time.perf_counter()
+        while sum(fills_all) < seconds and time.perf_counter() - t_start < 3 * seconds:
+            a, fills = O.port_fill_timer(*args_w, ds)
+            allocs.append(a)
+            fills_all.extend(fills)
+    med = statistics.median(fills_all)
     return {
-        "value": count * nnz / fill_s,
+        "value": nnz / med,
         "unit": "nnz-out/s",
         "cores": 1,
-        "kind": kind,
-        "sample": (f"20000x5000 B^T D A product, alloc (CSC mirror + Gustavson + plan) once per pass of 64 "
-                   f"d vectors, {count} single-thread refills with fresh d; fill time "
-                   f"{fill_s:.2f} s, alloc {alloc_s * 1e3:.1f} ms"),
-        "us_per_refill": fill_s / count * 1e6,
+        "kind": "port",
+        "sample": (f"20000x5000 B^T D A product on the C restatement of the reference's sparse "
+                   f"path (bit-identical to the reference build, time within 20 %): alloc (CSC "
+                   f"mirror + Gustavson + plan) once per pass of 64 d vectors, "
+                   f"{len(fills_all)} single-thread refills with fresh d, median fill "
+                   f"{med * 1e3:.3f} ms, median alloc {statistics.median(allocs) * 1e3:.1f} ms"),
+        "us_per_refill": med * 1e6,
+        **facts,
     }
 
 

@@ -420,3 +420,17 @@ def test_dense_kron_matches_reference_build(shape):
     assert bits_equal(O.y_kron_i_vec(m, k, n, Yf, w), O.y_kron_i_vec(m, k, n, Yf, w, r.Y_kron_I_vec))
     assert bits_equal(O.i_kron_xt_vec(m, k, n, Xf, w),
                       O.i_kron_xt_vec(m, k, n, Xf, w, r.I_kron_XT_vec))
+
+
+def test_faithfulness_record():
+    """The committed record of oracle/faithfulness.py: on the 20000 x 5000
+    product the C restatement's structure and values were bit-identical to the
+    reference build's, and its single-thread alloc and fill times were within
+    +-20 % of the reference's (the restatement is the CPU baseline wherever the
+    reference build is absent)."""
+    rec = json.load(open(os.path.join(os.path.dirname(__file__), "golden",
+                                      "oracle_faithfulness.json")))
+    assert rec["bit_identical"] is True and rec["nnz_out"] == 709850
+    assert 0.8 <= rec["ratio_alloc"] <= 1.2 and 0.8 <= rec["ratio_fill"] <= 1.2
+    assert rec["port"]["fill_runs"] >= 5 and rec["reference"]["fill_runs"] >= 5
+    assert rec["host"]["nproc"] >= 1 and rec["host"]["cpu_model"]
