@@ -56,7 +56,7 @@ def parse():
                     help="multi: refill rounds per launch (even; output ring of this many slots)")
     ap.add_argument("--workload", default="btda",
                     choices=["btda", "blocks-atda", "blocks-btda", "multi", "ata-large", "coo",
-                             "learned", "sum", "dense-kron", "lower-coo"],
+                             "learned", "sum", "dense-kron", "lower-coo", "write-ceiling"],
                     help="btda: the sparse refill (default line); blocks-*: the stacked "
                          "permuted-dense products (A^T D A, or B^T D A of two stacks)")
     ap.add_argument("--rmat-scale", type=int, default=20,
@@ -67,6 +67,8 @@ def parse():
     ap.add_argument("--gemm-slice", type=int, default=0,
                     help="blocks-*: reduction slice depth of the block GEMM (16 or 32; 0 = "
                          "library default)")
+    ap.add_argument("--sweep", action="store_true",
+                    help="btda: also time every refill launch configuration (stderr table)")
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="also run the streaming copy kernel on this many MiB (HBM counter "
                          "calibration under rocprofv3 --pmc)")
@@ -122,6 +124,8 @@ def main():
         return main_dense_kron(args)
     if args.workload == "lower-coo":
         return main_lower_coo(args)
+    if args.workload == "write-ceiling":
+        return main_write_ceiling(args)
     if args.workload != "btda":
         return main_blocks(args)
     import sparsediff_linalg_amd as sdl
@@ -165,6 +169,9 @@ def main():
                                     C.byref(best), C.byref(best_us)), "autotune")
 
     stream = lib.sdl_library_stream()
+    sweep = refill_sweep(lib, h, dAx, dBx, pools[0], m, K, dC, ldc) if args.sweep else None
+    if sweep is not None:
+        check(lib.sdl_btda_set_refill_cfg(h, C.byref(best)), "set cfg")
     ev = [(lib.sdl_event_create(), lib.sdl_event_create()) for _ in range(min(args.steps, 50))]
     e_all0, e_all1 = lib.sdl_event_create(), lib.sdl_event_create()
 
@@ -256,12 +263,15 @@ def main():
                          f"refill of C for K={K} fresh d vectors"),
             "K": K, "nnz_out": nnz, "terms": terms, "m": m, "n_A": nA, "n_B": nB,
             "refill_cfg": {"variant": best.variant, "rchunk": best.rchunk},
+            "symbolic_mode": "exact",
+            "symbolic_note": ("the symbolic phase (exact or predicted) is the untimed alloc "
+                              "phase; it yields the same structure and refill schedule, so the "
+                              "timed refills are the same for either"),
             "wall_s": t_wall,
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": "batched refill sequence (k_prep_batch: operand pairs + d "
-                      "transposition; k_num_pull_interleaved)",
+            "kernel": refill_kernel_name(best),
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
@@ -284,8 +294,104 @@ def main():
         "alloc_s": t_alloc,
         "cpu_baseline": cpu,
     }
+    if sweep is not None:
+        out["refill_sweep_us"] = sweep
     lib.sdl_btda_plan_destroy(h)
     print(json.dumps(out), flush=True)
+
+
+def refill_kernel_name(cfg):
+    if cfg.variant >= 3:
+        return "k_prep_batch (operand pairs + d transposition) + k_num_pull_interleaved"
+    return "k_num_pull_rchunk / k_num_pull_batched (direct gathers)"
+
+
+def refill_sweep(lib, h, dAx, dBx, D, m, K, dC, ldc, reps=20):
+    """Event time per batched refill of every candidate launch configuration
+    (after two warm-ups); returns {"variant/rchunk": us}."""
+    from sparsediff_linalg_amd import check
+    from sparsediff_linalg_amd._lib import sdl_refill_cfg
+    cands = (sdl_refill_cfg * 64)()
+    nc = min(64, lib.sdl_refill_candidates(cands, 64))
+    stream = lib.sdl_library_stream()
+    e0, e1 = lib.sdl_event_create(), lib.sdl_event_create()
+    res = {}
+    for c in range(nc):
+        cfg = cands[c]
+        for _ in range(2):
+            check(lib.sdl_btda_refill_batched_cfg(h, dAx.ptr, dBx.ptr, D.ptr, m, K, dC.ptr, ldc,
+                                                  C.byref(cfg), stream), "sweep")
+        lib.sdl_event_record(e0, stream)
+        for _ in range(reps):
+            check(lib.sdl_btda_refill_batched_cfg(h, dAx.ptr, dBx.ptr, D.ptr, m, K, dC.ptr, ldc,
+                                                  C.byref(cfg), stream), "sweep")
+        lib.sdl_event_record(e1, stream)
+        lib.sdl_synchronize()
+        us = lib.sdl_event_elapsed_ms(e0, e1) * 1e3 / reps
+        key = f"{cfg.variant}/{cfg.rchunk}"
+        res[key] = round(us, 2)
+        print(f"sweep {key}: {us:.2f} us", file=sys.stderr, flush=True)
+    return res
+
+
+def main_write_ceiling(args):
+    """Streaming-write calibration on the output size of the default line
+    (K = 64 refills of the 20000 x 5000 product: 64 rows of 709,888 doubles,
+    363 MB): constant non-temporal 16-byte and 8-byte stores, ordinary 16-byte
+    stores, a refill's store pattern (all 64 rows per thread, or 8), and the
+    streaming copy of the same bytes.  The best write rate is the ceiling a
+    write-bound refill can reach on this GPU."""
+    import sparsediff_linalg_amd as sdl
+    from sparsediff_linalg_amd import DeviceArray, check
+    lib = sdl.load()
+    K, nnz = args.K, 709_850
+    ldc = (nnz + 63) // 64 * 64
+    nbytes = K * ldc * 8
+    dst = DeviceArray((K, ldc), np.float64)
+    src = DeviceArray((K, ldc), np.float64)
+    stream = lib.sdl_library_stream()
+    e0, e1 = lib.sdl_event_create(), lib.sdl_event_create()
+    steps = max(args.steps, 10)
+
+    def timed(fn):
+        for _ in range(3):
+            check(fn(), "write")
+        lib.sdl_event_record(e0, stream)
+        for _ in range(steps):
+            check(fn(), "write")
+        lib.sdl_event_record(e1, stream)
+        lib.sdl_synchronize()
+        return lib.sdl_event_elapsed_ms(e0, e1) / steps
+
+    rates = {}
+    for name, fn in (
+            ("store16_nt", lambda: lib.sdl_write_kernel(dst.ptr, nbytes, 0, stream)),
+            ("store16", lambda: lib.sdl_write_kernel(dst.ptr, nbytes, 1, stream)),
+            ("store8_nt", lambda: lib.sdl_write_kernel(dst.ptr, nbytes, 2, stream)),
+            ("refill_rows_g1", lambda: lib.sdl_write_rows_kernel(dst.ptr, ldc, K, nnz, 1, stream)),
+            ("refill_rows_g8", lambda: lib.sdl_write_rows_kernel(dst.ptr, ldc, K, nnz, 8, stream))):
+        ms = timed(fn)
+        b = nbytes if not name.startswith("refill") else K * nnz * 8
+        rates[name] = {"ms": ms, "GB_s": b / (ms * 1e-3) / 1e9}
+    ms = timed(lambda: lib.sdl_copy_kernel(dst.ptr, src.ptr, nbytes, stream))
+    rates["copy"] = {"ms": ms, "GB_s": 2 * nbytes / (ms * 1e-3) / 1e9}
+    best = max(v["GB_s"] for k, v in rates.items() if k != "copy")
+    out = {
+        "metric": "streaming write GB/s (calibration of the write-bound refill)",
+        "value": best, "unit": "GB/s", "n_gpus": 1, "steps": steps, "warmup": 3,
+        "ms_per_step": min(v["ms"] for k, v in rates.items() if k != "copy"),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": (f"constant stores over {K} x {ldc} doubles ({nbytes / 1e6:.1f} MB),"
+                                " the output of the default batched refill"), "K": K},
+        "rates": rates,
+        "roofline": {"bound": "hbm", "kernel": "k_stream_write16 / k_write_rows",
+                     "achieved": best, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": best / HBM_PEAK_GBS, "traffic": None},
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+
 
 
 def main_ata_large(args):

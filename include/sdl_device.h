@@ -37,6 +37,14 @@ int sdl_device_name(char *buf, int len);
 /* Streaming device copy kernel (16-byte aligned), the HBM-counter calibration
    reference: exactly `bytes` read and `bytes` written. */
 int sdl_copy_kernel(void *dst, const void *src, size_t bytes, void *stream);
+/* Streaming-write calibration: `bytes` of constant stores (mode 0: 16 bytes
+   per lane, non-temporal; 1: 16 bytes, ordinary; 2: 8 bytes, non-temporal). */
+int sdl_write_kernel(void *dst, size_t bytes, int mode, void *stream);
+/* A batched refill's store pattern alone: K rows of n doubles at stride ldc,
+   thread e of a 256-entry tile storing column e of `K / groups` consecutive
+   rows (groups = 1: every row), one 8-byte non-temporal store per row. */
+int sdl_write_rows_kernel(double *dC, long long ldc, int K, long long n, int groups,
+                          void *stream);
 
 /* ---- C = B^T diag(d) A on device-resident CSR operands (the hot path of
    BTA_matrices_alloc / BTDA_matrices_fill_values for two sparse operands,

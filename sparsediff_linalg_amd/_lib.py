@@ -89,6 +89,8 @@ SIGNATURES = {
     "sdl_event_destroy": (None, [vp]),
     "sdl_device_name": (C.c_int, [C.c_char_p, C.c_int]),
     "sdl_copy_kernel": (C.c_int, [vp, vp, C.c_size_t, vp]),
+    "sdl_write_kernel": (C.c_int, [vp, C.c_size_t, C.c_int, vp]),
+    "sdl_write_rows_kernel": (C.c_int, [vp, C.c_longlong, C.c_int, C.c_longlong, C.c_int, vp]),
     # device-resident B^T D A plans
     "sdl_btda_plan_create": (C.c_int, [C.c_int, C.c_int, C.c_int, vp, vp, vp, vp, C.c_int,
                                        C.POINTER(vp)]),

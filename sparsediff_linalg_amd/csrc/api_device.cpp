@@ -265,8 +265,11 @@ int sdl_btda_refill_batched_ex(const sdl_btda_plan *plan, const double *dAx, con
         sdl::set_error("sdl_btda_refill_batched_ex: invalid arguments");
         return 1;
     }
-    // a k-major D is read in whole runs of 16 refills per row
-    if ((flags & SDL_BATCH_D_KMAJOR) ? ldd < ((long long) K + 15) / 16 * 16 : ldd < plan->m)
+    // a k-major D is read in whole runs of 16 refills per row, two values
+    // (16 bytes) per load
+    if ((flags & SDL_BATCH_D_KMAJOR)
+            ? (ldd < ((long long) K + 15) / 16 * 16 || (ldd & 1) || ((uintptr_t) dD & 15))
+            : ldd < plan->m)
     {
         sdl::set_error("sdl_btda_refill_batched_ex: ldd too small for the layout");
         return 1;

@@ -741,14 +741,20 @@ __global__ void __launch_bounds__(G)
             atomicOr(err, 1);
             return;
         }
+        // tK holds the L index q until the lists are sorted (below)
         tA[slot] = t;
         tB[slot] = Lperm ? Lperm[q] : q;
-        tK[slot] = k;
+        tK[slot] = q;
     });
     off_fence<CAP>();
     __threadfence_block();
     __syncthreads();
-    // off[e] is now the end of entry e's list; sort each list by k
+    // off[e] is now the end of entry e's list.  Sort each list by q, L's
+    // storage order, which is the order the reference folds an entry's terms
+    // in (gustavson.c:127-135 walks L's row ascending); q is unique within an
+    // entry (A rows have distinct columns).  q order is ascending k whenever
+    // L's rows are sorted; it also fixes the order for unsorted or repeated
+    // row indices.  Then replace q by k.
     for (int e = threadIdx.x; e < cnt; e += G)
     {
         int64_t s1 = rbase + off_ld<CAP>(off, e);
@@ -760,19 +766,20 @@ __global__ void __launch_bounds__(G)
         }
         for (int64_t i = s0 + 1; i < s1; i++)
         {
-            int kk = tK[i], aa = tA[i], bb = tB[i];
+            int qq = tK[i], aa = tA[i], bb = tB[i];
             int64_t j = i - 1;
-            while (j >= s0 && tK[j] > kk)
+            while (j >= s0 && tK[j] > qq)
             {
                 tK[j + 1] = tK[j];
                 tA[j + 1] = tA[j];
                 tB[j + 1] = tB[j];
                 j--;
             }
-            tK[j + 1] = kk;
+            tK[j + 1] = qq;
             tA[j + 1] = aa;
             tB[j + 1] = bb;
         }
+        for (int64_t i = s0; i < s1; i++) tK[i] = Li[tK[i]] + c.koff;
     }
 }
 
@@ -2049,9 +2056,14 @@ void product_bind_values(const ProductPlan &P, const double *Lval, const double 
 void product_numeric_batched(const ProductPlan &P, const double *Lval, const double *D,
                              int64_t ldd, int K, const double *Ax, double *Cx, int64_t ldc,
                              hipStream_t s, const RefillCfg *cfg, unsigned opts, void *ev_start,
-                             void *ev_stop)
+                             void *ev_stop, const BatchScratch *ws)
 {
     if (P.nnz == 0 || K <= 0) return;
+    // workspace: the plan's reservation, or the caller's
+    double *tvp = ws ? ws->tv : P.tv.p;
+    double *dTp = ws ? ws->dT : P.dT.p;
+    const int64_t wld = ws ? ws->ld : P.batch_ld;
+    const int cap = ws ? (int) std::min<int64_t>(ws->ld, INT_MAX) : P.batch_cap;
     if (!P.pull)
     {
         for (int r = 0; r < K; r++)
@@ -2062,15 +2074,20 @@ void product_numeric_batched(const ProductPlan &P, const double *Lval, const dou
     const bool kmajor = (opts & BATCH_D_KMAJOR) != 0;
     // only the interleaved kernels read a k-major D
     if (kmajor && c.variant < 3) c = RefillCfg{3, c.rchunk == 16 ? 16 : 8};
-    SDL_REQUIRE(!kmajor || K <= P.batch_cap, "k-major refill beyond the reservation");
-    if (c.variant >= 3 && K > P.batch_cap) c = RefillCfg{2, 8};
+    SDL_REQUIRE(!kmajor || K <= cap, "k-major refill beyond the reservation");
+    if (c.variant >= 3 && K > cap) c = RefillCfg{2, 8};
     if (c.variant >= 3 && K == 1 && !kmajor)
     {
         // a single refill: d read in place, no transposition
-        if (!(opts & BATCH_VALUES_BOUND)) product_bind_values(P, Lval, Ax, s);
+        if (!(opts & BATCH_VALUES_BOUND) && P.terms > 0)
+        {
+            hipLaunchKernelGGL(k_gather_terms, dim3((unsigned) ((P.terms + 255) / 256)), dim3(256),
+                               0, s, P.terms, P.tA.p, P.tB.p, Lval, Ax, tvp);
+            SDL_HIP(hipGetLastError());
+        }
         if (ev_start) SDL_HIP(hipEventRecord((hipEvent_t) ev_start, s));
         hipLaunchKernelGGL(k_num_pull_tv1, dim3((unsigned) ((P.nnz + 255) / 256)), dim3(256), 0,
-                           s, P.nnz, P.ep.p, P.tK.p, P.tv.p, D, Cx);
+                           s, P.nnz, P.ep.p, P.tK.p, tvp, D, Cx);
         SDL_HIP(hipGetLastError());
         if (ev_stop) SDL_HIP(hipEventRecord((hipEvent_t) ev_stop, s));
         return;
@@ -2081,7 +2098,7 @@ void product_numeric_batched(const ProductPlan &P, const double *Lval, const dou
     if (c.variant >= 3)
     {
         const int rc = c.rchunk == 8 ? 8 : 16;
-        const int ldt = (int) P.batch_ld;
+        const int ldt = (int) wld;
         // operand pairs (unless bound) and d transposed to k-major (unless
         // given so), in one launch
         const bool gather = c.variant != 6 && !(opts & BATCH_VALUES_BOUND) && P.terms > 0;
@@ -2093,12 +2110,11 @@ void product_numeric_batched(const ProductPlan &P, const double *Lval, const dou
         if (ngather + ntrans > 0)
         {
             hipLaunchKernelGGL(k_prep_batch, dim3((unsigned) (ngather + ntrans)), dim3(256), 0, s,
-                               ngather, P.terms, P.tA.p, P.tB.p, Lval, Ax,
-                               const_cast<double *>(P.tv.p), (int) P.A.Am, K, D, ldd, P.dT.p,
-                               ldt);
+                               ngather, P.terms, P.tA.p, P.tB.p, Lval, Ax, tvp, (int) P.A.Am, K,
+                               D, ldd, dTp, ldt);
             SDL_HIP(hipGetLastError());
         }
-        const double *DT = transpose ? P.dT.p : D;
+        const double *DT = transpose ? dTp : D;
         const int ldk = transpose ? ldt : (int) ldd;
         if (ev_start) SDL_HIP(hipEventRecord((hipEvent_t) ev_start, s));
         if (c.variant == 6)
@@ -2127,7 +2143,7 @@ void product_numeric_batched(const ProductPlan &P, const double *Lval, const dou
     if (rc == RC_ && ept == EPT_ && vec == VEC_)                                              \
     {                                                                                         \
         hipLaunchKernelGGL((k_num_pull_interleaved_v<RC_, EPT_, VEC_>), dim3((unsigned) blocks), \
-                           dim3(256), 0, s, P.nnz, ntiles, nchunks, P.ep.p, P.tK.p, P.tv.p,   \
+                           dim3(256), 0, s, P.nnz, ntiles, nchunks, P.ep.p, P.tK.p, tvp,      \
                            DT, ldk, K, Cx, ldc);                                          \
         SDL_HIP(hipGetLastError());                                                           \
         if (ev_stop) SDL_HIP(hipEventRecord((hipEvent_t) ev_stop, s));                       \
@@ -2147,12 +2163,11 @@ void product_numeric_batched(const ProductPlan &P, const double *Lval, const dou
 #undef SDL_IV
         if (rc == 8)
             hipLaunchKernelGGL(k_num_pull_interleaved<8>, dim3((unsigned) blocks), dim3(256), 0, s,
-                               P.nnz, ntiles, nchunks, P.ep.p, P.tK.p, P.tv.p, DT, ldk, K, Cx,
-                               ldc);
+                               P.nnz, ntiles, nchunks, P.ep.p, P.tK.p, tvp, DT, ldk, K, Cx, ldc);
         else
             hipLaunchKernelGGL(k_num_pull_interleaved<16>, dim3((unsigned) blocks), dim3(256), 0,
-                               s, P.nnz, ntiles, nchunks, P.ep.p, P.tK.p, P.tv.p, DT, ldk, K,
-                               Cx, ldc);
+                               s, P.nnz, ntiles, nchunks, P.ep.p, P.tK.p, tvp, DT, ldk, K, Cx,
+                               ldc);
         SDL_HIP(hipGetLastError());
         if (ev_stop) SDL_HIP(hipEventRecord((hipEvent_t) ev_stop, s));
         return;
@@ -2204,6 +2219,12 @@ RefillCfg default_refill_cfg(const ProductPlan &P, int K)
     c.variant = 2;
     c.rchunk = K >= 16 ? 8 : 4;
     return c;
+}
+
+int64_t batch_scratch_doubles(const ProductPlan &P, int K)
+{
+    const int64_t ld = ((int64_t) K + 15) / 16 * 16;
+    return 2 * std::max<int64_t>(P.terms, 1) + (K > 1 ? std::max<int64_t>(P.A.Am, 1) * ld : 0);
 }
 
 void product_reserve_batch(ProductPlan &P, int Kmax)

@@ -22,6 +22,41 @@ __global__ void k_stream_copy(const u32x4 *__restrict__ src, u32x4 *__restrict__
     for (; i < n; i += st) dst[i] = src[i];
 }
 
+// Streaming writes (the write-ceiling calibration): constant values over n
+// 16-byte (or 8-byte) words, grid-stride, non-temporal or ordinary stores.
+template <bool NT>
+__global__ void k_stream_write16(u32x4 *__restrict__ dst, size_t n)
+{
+    size_t i = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
+    size_t st = (size_t) gridDim.x * blockDim.x;
+    u32x4 v = {0u, 0u, 0u, 0u};
+    for (; i < n; i += st)
+    {
+        if (NT) __builtin_nontemporal_store(v, dst + i);
+        else dst[i] = v;
+    }
+}
+
+__global__ void k_stream_write8(double *__restrict__ dst, size_t n)
+{
+    size_t i = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
+    size_t st = (size_t) gridDim.x * blockDim.x;
+    for (; i < n; i += st) __builtin_nontemporal_store(0.0, dst + i);
+}
+
+// The store pattern of the grouped batched refill without its reads: thread
+// e of tile b / groups writes rows g*W .. min(K, g*W + W) - 1 of column e
+// (g = b % groups), one non-temporal 8-byte store per row.
+__global__ void __launch_bounds__(256)
+    k_write_rows(double *__restrict__ C, int64_t ldc, int K, int64_t n, int groups, int W)
+{
+    const int g = (int) (blockIdx.x % (unsigned) groups);
+    const int64_t e = (int64_t) (blockIdx.x / (unsigned) groups) * 256 + threadIdx.x;
+    if (e >= n) return;
+    const int r1 = min(K, g * W + W);
+    for (int r = g * W; r < r1; r++) __builtin_nontemporal_store((double) r, C + (int64_t) r * ldc + e);
+}
+
 std::once_flag g_once;
 hipStream_t g_stream = nullptr;
 std::atomic<long long> g_dev_allocs{0};
@@ -195,6 +230,44 @@ int sdl_copy_kernel(void *dst, const void *src, size_t bytes, void *stream)
     hipLaunchKernelGGL(sdl::k_stream_copy, dim3(8192), dim3(256), 0,
                        stream ? (hipStream_t) stream : sdl::stream(), (const sdl::u32x4 *) src,
                        (sdl::u32x4 *) dst, n);
+    SDL_HIP(hipGetLastError());
+    return 0;
+}
+
+int sdl_write_kernel(void *dst, size_t bytes, int mode, void *stream)
+{
+    const size_t word = mode == 2 ? 8 : 16;
+    if (mode < 0 || mode > 2 || bytes % word != 0 || ((uintptr_t) dst % word))
+    {
+        sdl::set_error("sdl_write_kernel: mode 0..2 with word-aligned size and pointer only");
+        return 1;
+    }
+    hipStream_t s = stream ? (hipStream_t) stream : sdl::stream();
+    if (mode == 2)
+        hipLaunchKernelGGL(sdl::k_stream_write8, dim3(8192), dim3(256), 0, s, (double *) dst,
+                           bytes / 8);
+    else if (mode == 0)
+        hipLaunchKernelGGL(sdl::k_stream_write16<true>, dim3(8192), dim3(256), 0, s,
+                           (sdl::u32x4 *) dst, bytes / 16);
+    else
+        hipLaunchKernelGGL(sdl::k_stream_write16<false>, dim3(8192), dim3(256), 0, s,
+                           (sdl::u32x4 *) dst, bytes / 16);
+    SDL_HIP(hipGetLastError());
+    return 0;
+}
+
+int sdl_write_rows_kernel(double *dC, long long ldc, int K, long long n, int groups, void *stream)
+{
+    if (!dC || K < 1 || n < 1 || ldc < n || groups < 1 || groups > 64)
+    {
+        sdl::set_error("sdl_write_rows_kernel: invalid arguments");
+        return 1;
+    }
+    const int W = (K + groups - 1) / groups;
+    const long long blocks = (n + 255) / 256 * groups;
+    hipLaunchKernelGGL(sdl::k_write_rows, dim3((unsigned) blocks), dim3(256), 0,
+                       stream ? (hipStream_t) stream : sdl::stream(), dC, (int64_t) ldc, K,
+                       (int64_t) n, groups, W);
     SDL_HIP(hipGetLastError());
     return 0;
 }

@@ -95,6 +95,18 @@ struct ProductPlan
     int64_t batch_ld = 0;
 };
 
+// Workspace of the interleaved batched refill given by the caller instead of
+// the plan's reservation: operand pairs (2 * terms doubles) and d transposed
+// to k-major (m * ld doubles, ld = K rounded up to 16).
+struct BatchScratch
+{
+    double *tv = nullptr;
+    double *dT = nullptr;
+    int64_t ld = 0;
+};
+// Doubles of workspace the interleaved refill of K refills needs.
+int64_t batch_scratch_doubles(const ProductPlan &P, int K);
+
 // Reserve the interleaved batched-refill workspace for up to Kmax refills.
 void product_reserve_batch(ProductPlan &P, int Kmax);
 
@@ -143,7 +155,8 @@ enum : unsigned
 void product_numeric_batched(const ProductPlan &P, const double *Lval, const double *D,
                              int64_t ldd, int K, const double *Ax, double *Cx, int64_t ldc,
                              hipStream_t s, const RefillCfg *cfg = nullptr, unsigned opts = 0,
-                             void *ev_start = nullptr, void *ev_stop = nullptr);
+                             void *ev_start = nullptr, void *ev_stop = nullptr,
+                             const BatchScratch *ws = nullptr);
 // Gather the term operand pairs for the interleaved refill (needs the reserved
 // workspace; no-op otherwise).
 void product_bind_values(const ProductPlan &P, const double *Lval, const double *Ax,

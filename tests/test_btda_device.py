@@ -154,17 +154,18 @@ def test_btda_20000x5000_bit_exact_single_and_batched():
         P.close()
 
 
-def test_every_refill_configuration_bit_exact():
+@pytest.mark.parametrize("K", [16, 13])
+def test_every_refill_configuration_bit_exact(K):
     """Every candidate launch configuration of the batched refill (including
     the interleaved kernels that need the reservation) gives the oracle's
-    values for every refill of a K = 16 batch."""
+    values for every refill of a K = 16 batch, and of a K = 13 batch whose
+    last chunk is partial."""
     from sparsediff_linalg_amd._lib import sdl_refill_cfg
     lib = load()
     rng = np.random.default_rng(21)
     m, nA, nB = 3000, 700, 500
     Ap, Ai, Ax = random_csr(rng, m, nA, 0.01)
     Bp, Bi, Bx = random_csr(rng, m, nB, 0.012)
-    K = 16
     D = np.stack([workloads.uniform(300 + r, m, 0.1, 10.0) for r in range(K)])
     ref = [O.btda(m, nA, nB, Ap, Ai, Ax, Bp, Bi, Bx, D[r])[2] for r in range(K)]
     P = DevPlan(m, nA, nB, Ap, Ai, Bp, Bi)

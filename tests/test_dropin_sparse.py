@@ -58,6 +58,51 @@ def test_raw_btda_literal_and_random():
     L.free_CSC_matrix(Bc)
 
 
+@pytest.mark.parametrize("nA,ratio_pull", [(400, True), (12, False)])
+def test_raw_btda_unsorted_csc_rows(nA, ratio_pull):
+    """A caller-made CSC B whose row indices are shuffled within each column
+    (one column also repeats a row index): BTA_alloc / BTDA_fill_values fold
+    every entry's terms in B's storage order, as the reference does
+    (gustavson.c:127-135), through the pull schedule (terms <= 3 nnz) and the
+    row accumulator (terms > 3 nnz), bit for bit against the oracle walking the
+    same storage order.  Entries with three or more terms make the order
+    visible in the last bits."""
+    L = D.lib()
+    rng = np.random.default_rng(31 + nA)
+    m, nB = 600, 40
+    Ap, Ai, Ax = random_csr(rng, m, nA, min(0.5, 6.0 / nA))
+    Bp, Bi, Bx = random_csr(rng, m, nB, 0.12)
+    Bcp, Bci, Bcx = (np.array(v) for v in O.transpose(m, nB, Bp, Bi, Bx))
+    for c in range(nB):  # shuffle the row indices of every column
+        s0, s1 = Bcp[c], Bcp[c + 1]
+        perm = rng.permutation(s1 - s0) + s0
+        Bci[s0:s1], Bcx[s0:s1] = Bci[perm].copy(), Bcx[perm].copy()
+    c = int(np.argmax(np.diff(Bcp)))  # a repeated row index in the longest column
+    Bci[Bcp[c] + 1] = Bci[Bcp[c]]
+    Bcx = Bcx * rng.uniform(0.5, 2.0, len(Bcx))
+    d = rng.uniform(0.1, 10, m)
+    Cp, Ci, Cx, terms = O.spgemm(nB, Bcp, Bci, Bcx, 1, d, m, nA, Ap, Ai, Ax)
+    assert (terms <= 3 * len(Ci)) == ratio_pull
+    assert np.max(np.diff(Cp)) > 0
+    Ac, Bc = D.new_csr(Ap, Ai, Ax, m, nA), D.new_csc(Bcp, Bci, Bcx, m, nB)
+    Cc = L.BTA_alloc(Ac, Bc)
+    L.BTDA_fill_values(Ac, Bc, D.dptr(d), None, Cc)
+    p, i, x = D.read_csr(Cc)
+    assert list(p) == list(Cp) and list(i) == list(Ci) and bits_equal(x, Cx)
+    # the order is visible: folding each entry in ascending k (columns sorted)
+    # gives different last bits somewhere
+    Sci, Scx = Bci.copy(), Bcx.copy()
+    for c in range(nB):
+        s0, s1 = Bcp[c], Bcp[c + 1]
+        o = np.argsort(Bci[s0:s1], kind="stable") + s0
+        Sci[s0:s1], Scx[s0:s1] = Bci[o], Bcx[o]
+    _, _, Sx, _ = O.spgemm(nB, Bcp, Sci, Scx, 1, d, m, nA, Ap, Ai, Ax)
+    assert not bits_equal(Sx, Cx)
+    L.free_CSR_matrix(Cc)
+    L.free_CSR_matrix(Ac)
+    L.free_CSC_matrix(Bc)
+
+
 @pytest.mark.parametrize("case", GOLD["ata"], ids=lambda c: c["name"])
 def test_raw_ata_literals(case):
     L = D.lib()
