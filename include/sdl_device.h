@@ -88,6 +88,21 @@ int sdl_btda_refill_batched(const sdl_btda_plan *plan, const double *dAx,
                             const double *dBx, const double *dD, long long ldd, int K,
                             double *dC, long long ldc, void *stream);
 
+/* K refills as sdl_btda_refill_batched, allocating nothing and needing no
+   sdl_btda_plan_reserve: the refill's workspace (the term operand pairs and d
+   transposed to k-major) goes to a fixed 32 MiB device array of the library
+   (sdl_static_scratch_bytes()), so 2 * terms + m * (K rounded up to 16)
+   doubles must fit it; larger calls return an error and launch nothing.  No
+   counterpart in the reference: for callers that cannot allocate (code under
+   HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+   Every call on one GPU shares the array and leaves its workspace there, so
+   calls that overlap on one GPU (different streams, or host threads) race;
+   serialise them.  Results are bit-identical to sdl_btda_refill_batched. */
+int sdl_btda_refill_batched_static(const sdl_btda_plan *plan, const double *dAx,
+                                   const double *dBx, const double *dD, long long ldd, int K,
+                                   double *dC, long long ldc, void *stream);
+long long sdl_static_scratch_bytes(void);
+
 /* Refill of the entries [e0, e1) of C only (plans with a pull schedule):
    dCx is the whole value array, entries outside the range are untouched.
    One launch: the per-call path for one instance of a block-diagonal plan. */

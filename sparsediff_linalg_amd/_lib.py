@@ -116,6 +116,9 @@ SIGNATURES = {
     "sdl_btda_refill_batched_ex": (C.c_int, [vp, vp, vp, vp, C.c_longlong, C.c_int, vp,
                                              C.c_longlong, C.c_uint, vp, vp, vp]),
     "sdl_btda_refill_range": (C.c_int, [vp, vp, vp, vp, vp, C.c_longlong, C.c_longlong, vp]),
+    "sdl_btda_refill_batched_static": (C.c_int, [vp, vp, vp, vp, C.c_longlong, C.c_int, vp,
+                                                 C.c_longlong, vp]),
+    "sdl_static_scratch_bytes": (C.c_longlong, []),
     "sdl_coo_plan_create": (C.c_int, [C.c_longlong, vp, vp, C.c_int, C.c_int, C.POINTER(vp)]),
     "sdl_coo_plan_nnz": (C.c_longlong, [vp]),
     "sdl_coo_plan_rowptr": (vp, [vp]),
