@@ -30,6 +30,9 @@ _SIGS = {
     "orc_spgemm_row": (C.c_int, [C.c_int, C.c_int, ip, ip, dp, C.c_int, dp, C.c_int, C.c_int,
                                  ip, ip, dp, ip, dp, ip, dp]),
     "orc_csr_transpose": (None, [C.c_int, C.c_int, ip, ip, dp, ip, ip, dp, ip]),
+    "orc_spgemm_range": (C.c_longlong, [C.c_int, C.c_int, C.c_int, ip, ip, dp, C.c_int, dp,
+                                        C.c_int, C.c_int, ip, ip, dp, ip, dp, C.c_longlong, lp,
+                                        ip, dp]),
     "orc_row_reduce_alloc": (C.c_int, [C.c_int, C.c_int, ip, ip, ip, C.c_int, ip, ip, ip]),
     "orc_row_reduce_fill": (None, [C.c_int, dp, ip, C.c_int, dp]),
     "orc_coo_coalesce": (C.c_longlong, [C.c_longlong, ip, ip, dp, C.c_int, C.c_int, lp, ip,
@@ -148,6 +151,26 @@ def spgemm_rows(rows, Lm, Lp, Li, Lx, blocks, d, Am, An, Ap, Ai, Ax):
                                Am, An, I(Ap), I(Ai), D(Ax), I(mark), D(acc), I(cols), D(vals))
         out.append((cols[:cnt].copy(), vals[:cnt].copy()))
     return out
+
+
+def spgemm_range(r0, r1, Lm, Lp, Li, Lx, blocks, d, Am, An, Ap, Ai, Ax, cap):
+    """Output rows [r0, r1) of C = kron(I_blocks, L) diag(d) A (the oracle's
+    row routine over a range; arrays must already be contiguous int32 /
+    float64): (Cp int64 from 0, Ci, Cx), or None if nnz would exceed cap.
+    Releases the GIL during the C call, so threads may check blocks in
+    parallel."""
+    o = lib()
+    mark = np.full(max(An, 1), -1, np.int32)
+    acc = np.zeros(max(An, 1), np.float64)
+    Cp = np.zeros(r1 - r0 + 1, np.int64)
+    Ci = np.zeros(max(cap, 1), np.int32)
+    Cx = np.zeros(max(cap, 1), np.float64)
+    nnz = o.orc_spgemm_range(r0, r1, Lm, I(Lp), I(Li), D(Lx), blocks,
+                             D(d) if d is not None else None, Am, An, I(Ap), I(Ai), D(Ax),
+                             I(mark), D(acc), cap, L(Cp), I(Ci), D(Cx))
+    if nnz < 0:
+        return None
+    return Cp, Ci[:nnz], Cx[:nnz]
 
 
 def coo_coalesce(rows, cols, x, m, n):

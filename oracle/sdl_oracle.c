@@ -198,6 +198,34 @@ int orc_spgemm_row(int row, int Lm, const int *Lp, const int *Li, const double *
     return cnt;
 }
 
+int64_t orc_spgemm_range(int r0, int r1, int Lm, const int *Lp, const int *Li,
+                         const double *Lx, int blocks, const double *d, int Am, int An,
+                         const int *Ap, const int *Ai, const double *Ax, int *mark,
+                         double *acc, int64_t cap, int64_t *Cp, int *Ci, double *Cx)
+{
+    int nL = Am / blocks;
+    int64_t off = 0;
+    Cp[0] = 0;
+    for (int row = r0; row < r1; row++)
+    {
+        /* the row's term count bounds its nnz: never write past cap */
+        int b = row / Lm, r = row % Lm;
+        int64_t bound = 0;
+        for (int q = Lp[r]; q < Lp[r + 1]; q++)
+        {
+            int k = Li[q] + b * nL;
+            bound += Ap[k + 1] - Ap[k];
+        }
+        if (bound > An) bound = An;
+        if (off + bound > cap) return -1;
+        int cnt = orc_spgemm_row(row, Lm, Lp, Li, Lx, blocks, d, Am, An, Ap, Ai, Ax, mark, acc,
+                                 Ci + off, Cx + off);
+        off += cnt;
+        Cp[row - r0 + 1] = off;
+    }
+    return off;
+}
+
 /* CSC_matrix.c:62-123 / CSR_matrix.c:99-142: column counts, exclusive scan,
    then a scatter in row order, so each output row lists sources ascending. */
 void orc_csr_transpose(int m, int n, const int *Ap, const int *Ai, const double *Ax,

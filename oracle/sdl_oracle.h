@@ -14,6 +14,8 @@
  *   orc_spgemm_*          pinned: reference literals + bit-exact vs the
  *                         reference's own gustavson.c built in oracle/_ref +
  *                         brute-force dense boolean/FP64 products on tiny inputs
+ *   orc_spgemm_range      a loop of orc_spgemm_row; equal to orc_spgemm_* on
+ *                         random products (tests/test_oracle.py)
  *   orc_csr_transpose     pinned: literals + oracle/_ref + double-transpose identity
  *   orc_row_reduce_*      pinned: literals (test_row_reduce.h) + oracle/_ref vtable
  *   orc_coo_coalesce      pinned: equals orc_row_reduce on one-entry rows +
@@ -66,6 +68,15 @@ int orc_spgemm_row(int row, int Lm, const int *Lp, const int *Li, const double *
                    int blocks, const double *d, int Am, int An, const int *Ap,
                    const int *Ai, const double *Ax, int *mark, double *acc,
                    int *cols, double *vals);
+
+/* The output rows [r0, r1) one after another through orc_spgemm_row (for
+   block-wise checks of large products): Cp gets r1 - r0 + 1 offsets from 0,
+   Ci / Cx the rows back to back.  Returns the range's nnz, or -1 if it would
+   exceed cap (nothing beyond cap is written). */
+int64_t orc_spgemm_range(int r0, int r1, int Lm, const int *Lp, const int *Li,
+                         const double *Lx, int blocks, const double *d, int Am, int An,
+                         const int *Ap, const int *Ai, const double *Ax, int *mark,
+                         double *acc, int64_t cap, int64_t *Cp, int *Ci, double *Cx);
 
 /* ---- transposition (src/sparse/CSC_matrix.c:62-189, CSR_matrix.c:99-205) ---- */
 

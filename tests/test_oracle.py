@@ -434,3 +434,29 @@ def test_faithfulness_record():
     assert 0.8 <= rec["ratio_alloc"] <= 1.2 and 0.8 <= rec["ratio_fill"] <= 1.2
     assert rec["port"]["fill_runs"] >= 5 and rec["reference"]["fill_runs"] >= 5
     assert rec["host"]["nproc"] >= 1 and rec["host"]["cpu_model"]
+
+
+@pytest.mark.parametrize("blocks", [1, 3])
+def test_spgemm_range_equals_whole_product(blocks):
+    """orc_spgemm_range (row blocks, for checks of products too large to
+    form whole) gives the whole product's rows, bit for bit, and refuses a
+    capacity below the range's term bound."""
+    rng = np.random.default_rng(40 + blocks)
+    Lm, nL, An = 37, 29, 41
+    Lp, Li, Lx = random_csr(rng, Lm, nL, 0.2, empty_row=4)
+    Ap, Ai, Ax = random_csr(rng, nL * blocks, An, 0.15, dense_col=7)
+    d = rng.uniform(0.1, 10, nL * blocks)
+    Cp, Ci, Cx, terms = O.spgemm(Lm, Lp, Li, Lx, blocks, d, nL * blocks, An, Ap, Ai, Ax)
+    args = [np.ascontiguousarray(v, t) for v, t in ((Lp, np.int32), (Li, np.int32),
+                                                    (Lx, np.float64))]
+    Ac = [np.ascontiguousarray(v, t) for v, t in ((Ap, np.int32), (Ai, np.int32),
+                                                  (Ax, np.float64))]
+    cuts = [0, 5, 6, 20, Lm * blocks]
+    for r0, r1 in zip(cuts, cuts[1:]):
+        got = O.spgemm_range(r0, r1, Lm, *args, blocks, d, nL * blocks, An, *Ac, cap=terms)
+        gp, gi, gx = got
+        a, b = int(Cp[r0]), int(Cp[r1])
+        assert np.array_equal(gp, np.asarray(Cp[r0:r1 + 1], np.int64) - a)
+        assert np.array_equal(gi, Ci[a:b]) and bits_equal(gx, Cx[a:b])
+    assert O.spgemm_range(0, Lm * blocks, Lm, *args, blocks, d, nL * blocks, An, *Ac,
+                          cap=max(len(Ci) // 2, 1)) is None
