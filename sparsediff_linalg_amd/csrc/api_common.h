@@ -42,9 +42,12 @@ template <typename T> T *attached_as(const void *owner, const char *fn)
     return t;
 }
 
-// Host array allocation used for ABI structs (freed by the free_* functions).
+// Host array allocation used for ABI structs (freed by the free_* functions),
+// counted in the tracked_alloc.h counters (g_allocated_bytes / g_peak_bytes).
 int *host_ints(int64_t n, bool zero);
 double *host_doubles(int64_t n);
+void *host_malloc(size_t bytes);
+void host_free(void *p);
 
 
 } // namespace sdl

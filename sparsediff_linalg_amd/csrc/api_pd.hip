@@ -181,7 +181,7 @@ matrix *pd_view(PDM *A)
     std::vector<PDM *> one{A};
     std::vector<int64_t> off{0};
     blocks_csr_pattern(one, off, A->base.m, c->p, c->i, nullptr);
-    std::free(c->x);
+    host_free(c->x);
     c->x = A->X;  // values alias X (row-major block = CSR order)
     SparseM *S = as_sparse(make_sparse(c));
     S->ext_values = pd_view_vals;
@@ -540,7 +540,7 @@ SPDM *new_spdm(int m, int n, const std::vector<PDM *> &blocks, const std::vector
         S->boff.push_back(off);
         int bn = b->base.nnz;
         if (bn > 0) std::memcpy(S->x + off, b->X, (size_t) bn * sizeof(double));
-        if (!b->parent) std::free(b->X);
+        if (!b->parent) host_free(b->X);
         b->X = S->x + off;
         b->base.x = b->X;
         b->parent = S;
@@ -559,7 +559,7 @@ void pd_kind_free(matrix *m)
         PDM *P = as_pd(m);
         sync();
         free_view(P->csrv, true);
-        if (!P->parent) std::free(P->X);
+        if (!P->parent) host_free(P->X);
         delete P;
         return;
     }
@@ -571,7 +571,7 @@ void pd_kind_free(matrix *m)
         delete b;
     }
     free_view(S->csrv, false);
-    std::free(S->x);
+    host_free(S->x);
     delete S;
 }
 

@@ -8,9 +8,17 @@
 
 #include <climits>
 #include <cstring>
+#include <malloc.h>
 #include <mutex>
 #include <unordered_map>
 #include <vector>
+
+extern "C" {
+#pragma GCC visibility push(default)
+size_t g_allocated_bytes = 0;
+size_t g_peak_bytes = 0;
+#pragma GCC visibility pop
+}
 
 namespace sdl {
 
@@ -62,15 +70,46 @@ void detach(const void *owner)
     sync();
 }
 
+// The host-byte counters of tracked_alloc.h (tracked_alloc.h:40-52 of the
+// reference): every ABI array the library allocates or frees is counted.
+std::mutex g_track_mu;
+
+void count_alloc(void *p)
+{
+    if (!p) return;
+    std::lock_guard<std::mutex> g(g_track_mu);
+    g_allocated_bytes += malloc_usable_size(p);
+    if (g_allocated_bytes > g_peak_bytes) g_peak_bytes = g_allocated_bytes;
+}
+
+void *host_malloc(size_t bytes)
+{
+    void *p = std::malloc(bytes > 0 ? bytes : 1);
+    count_alloc(p);
+    return p;
+}
+
+void host_free(void *p)
+{
+    if (!p) return;
+    {
+        std::lock_guard<std::mutex> g(g_track_mu);
+        g_allocated_bytes -= malloc_usable_size(p);
+    }
+    std::free(p);
+}
+
 int *host_ints(int64_t n, bool zero)
 {
     size_t bytes = (size_t) (n > 0 ? n : 1) * sizeof(int);
-    return (int *) (zero ? std::calloc(1, bytes) : std::malloc(bytes));
+    void *p = zero ? std::calloc(1, bytes) : std::malloc(bytes);
+    count_alloc(p);
+    return (int *) p;
 }
 
 double *host_doubles(int64_t n)
 {
-    return (double *) std::malloc((size_t) (n > 0 ? n : 1) * sizeof(double));
+    return (double *) host_malloc((size_t) (n > 0 ? n : 1) * sizeof(double));
 }
 
 namespace {
@@ -107,7 +146,7 @@ struct LowerCOOState : Attached
 
 CSR_matrix *alloc_csr_struct(int m, int n, int nnz)
 {
-    CSR_matrix *C = (CSR_matrix *) std::malloc(sizeof(CSR_matrix));
+    CSR_matrix *C = (CSR_matrix *) host_malloc(sizeof(CSR_matrix));
     C->p = host_ints((int64_t) m + 1, true);
     C->i = host_ints(nnz, true);
     C->x = host_doubles(nnz);
@@ -133,7 +172,7 @@ Out *transpose_alloc_impl(const char *fn, int m, int n, int nnz, const int *p, c
     st->xin.alloc(nnz);
     st->xout.alloc(nnz);
     Out *C = (Out *) alloc_csr_struct(0, 0, nnz);
-    std::free(C->p);
+    host_free(C->p);
     C->p = host_ints((int64_t) n + 1, false);
     C->m = m;
     C->n = n;  // fixed up by the caller for the CSR (n x m) case
@@ -214,6 +253,20 @@ using namespace sdl;
 extern "C" {
 #pragma GCC visibility push(default)
 
+// ------------------------------------------------- tracked_alloc.h ----
+void sp_memory_stats(size_t *live_bytes, size_t *peak_bytes)
+{
+    std::lock_guard<std::mutex> g(g_track_mu);
+    if (live_bytes) *live_bytes = g_allocated_bytes;
+    if (peak_bytes) *peak_bytes = g_peak_bytes;
+}
+
+void sp_memory_reset_peak(void)
+{
+    std::lock_guard<std::mutex> g(g_track_mu);
+    g_peak_bytes = g_allocated_bytes;
+}
+
 // ------------------------------------------------------------- CSR ----
 CSR_matrix *new_CSR_matrix(int m, int n, int nnz) { return alloc_csr_struct(m, n, nnz); }
 
@@ -230,10 +283,10 @@ void free_CSR_matrix(CSR_matrix *matrix)
 {
     if (!matrix) return;
     detach(matrix);
-    std::free(matrix->p);
-    std::free(matrix->i);
-    std::free(matrix->x);
-    std::free(matrix);
+    host_free(matrix->p);
+    host_free(matrix->i);
+    host_free(matrix->x);
+    host_free(matrix);
 }
 
 void copy_CSR_matrix(const CSR_matrix *A, CSR_matrix *C)
@@ -280,7 +333,7 @@ CSR_matrix *transpose(const CSR_matrix *A, int *iwork)
 // ------------------------------------------------------------- CSC ----
 CSC_matrix *new_CSC_matrix(int m, int n, int nnz)
 {
-    CSC_matrix *C = (CSC_matrix *) std::malloc(sizeof(CSC_matrix));
+    CSC_matrix *C = (CSC_matrix *) host_malloc(sizeof(CSC_matrix));
     C->p = host_ints((int64_t) n + 1, false);
     C->i = host_ints(nnz, false);
     C->x = host_doubles(nnz);
@@ -294,10 +347,10 @@ void free_CSC_matrix(CSC_matrix *matrix)
 {
     if (!matrix) return;
     detach(matrix);
-    std::free(matrix->p);
-    std::free(matrix->i);
-    std::free(matrix->x);
-    std::free(matrix);
+    host_free(matrix->p);
+    host_free(matrix->i);
+    host_free(matrix->x);
+    host_free(matrix);
 }
 
 CSC_matrix *csr_to_csc_alloc(const CSR_matrix *A, int *iwork)
@@ -365,7 +418,7 @@ void ATDA_fill_values(const CSC_matrix *A_csc, const double *d, const CSR_matrix
 // ------------------------------------------------------------- COO ----
 COO_matrix *new_COO_matrix(const CSR_matrix *A)
 {
-    COO_matrix *coo = (COO_matrix *) std::malloc(sizeof(COO_matrix));
+    COO_matrix *coo = (COO_matrix *) host_malloc(sizeof(COO_matrix));
     coo->m = A->m;
     coo->n = A->n;
     coo->nnz = A->nnz;
@@ -386,11 +439,11 @@ void free_COO_matrix(COO_matrix *matrix)
 {
     if (!matrix) return;
     detach(matrix);
-    std::free(matrix->rows);
-    std::free(matrix->cols);
-    std::free(matrix->x);
-    std::free(matrix->value_map);
-    std::free(matrix);
+    host_free(matrix->rows);
+    host_free(matrix->cols);
+    host_free(matrix->x);
+    host_free(matrix->value_map);
+    host_free(matrix);
 }
 
 #pragma GCC visibility pop
@@ -411,7 +464,7 @@ COO_matrix *new_COO_matrix_lower_triangular(const CSR_matrix *A)
     auto st = std::make_unique<LowerCOOState>();
     int count = 0;
     lower_coo_structure(A->m, A->nnz, dp.p, di.p, rows, cols, st->vmap, count);
-    COO_matrix *coo = (COO_matrix *) std::malloc(sizeof(COO_matrix));
+    COO_matrix *coo = (COO_matrix *) host_malloc(sizeof(COO_matrix));
     coo->m = A->m;
     coo->n = A->n;
     coo->nnz = count;

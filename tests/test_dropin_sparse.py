@@ -393,3 +393,84 @@ def test_matrix_kron_sparse():
     L.free_matrix(Cm)
     L.free_matrix(J)
     L.free_matrix(A)
+
+
+# ------------------------------------------------- sizes and overflow ----
+@pytest.mark.parametrize("shape", ["20000x5000", "2000000x500000"])
+def test_transposes_at_size(shape):
+    """The public transposes through the C ABI at the benchmark operands'
+    sizes (CSC_matrix.c:62-189, CSR_matrix.c:99-205): csr_to_csc_alloc /
+    _fill, csc_to_csr_alloc / _fill, transpose, AT_alloc / AT_fill_values,
+    bit for bit against the oracle's stable counting-sort transpose, the
+    fill repeated after the values change."""
+    from sparsediff_linalg_amd import workloads as W
+    L = D.lib()
+    if shape == "20000x5000":
+        w = W.btda_20000x5000(1)
+        m, n, (p, i, x) = w["m"], w["nB"], (w["Bp"], w["Bi"], w["Bx"])
+    else:
+        m, n = 2_000_000, 500_000
+        p, i, x = W.zipf_csr(2, m, n)
+    Tp, Ti, Tx = O.transpose(m, n, p, i, x)
+    A = D.new_csr(p, i, x, m, n)
+    Cc = L.csr_to_csc_alloc(A, None)
+    L.csr_to_csc_fill_values(A, Cc, None)
+    cp, ci, cx = D.read_csc(Cc)
+    assert np.array_equal(cp, Tp) and np.array_equal(ci, Ti) and bits_equal(cx, Tx)
+    R = L.csc_to_csr_alloc(Cc, None)
+    L.csc_to_csr_fill_values(Cc, R, None)
+    rp, ri, rx = D.read_csr(R)
+    assert np.array_equal(rp, p) and np.array_equal(ri, i) and bits_equal(rx, x)
+    T = L.transpose(A, None)
+    tp, ti, tx = D.read_csr(T)
+    assert np.array_equal(tp, Tp) and np.array_equal(ti, Ti) and bits_equal(tx, Tx)
+    AT = L.AT_alloc(A, None)
+    x2 = -2.0 * np.asarray(x)
+    C.memmove(A.contents.x, x2.ctypes.data, x2.nbytes)
+    L.AT_fill_values(A, AT, None)
+    L.csr_to_csc_fill_values(A, Cc, None)
+    _, _, Tx2 = O.transpose(m, n, p, i, x2)
+    assert bits_equal(D.read_csr(AT)[2], Tx2) and bits_equal(D.read_csc(Cc)[2], Tx2)
+    for S in (R, T, AT, A):
+        L.free_CSR_matrix(S)
+    L.free_CSC_matrix(Cc)
+
+
+def test_row_gather_alloc_no_int_overflow():
+    """row_gather_alloc where n_idxs * n = 25000 * 100000 overflows int32
+    (test_alloc_overflow.h:31-80): a valid CSR with the gathered nnz."""
+    L = D.lib()
+    n_idxs, n_cols = 25000, 100000
+    p = np.arange(n_idxs + 1, dtype=np.int32)
+    i = np.arange(n_idxs, dtype=np.int32)
+    M = D.sparse(p, i, np.ones(n_idxs), n_idxs, n_cols)
+    idx = np.arange(n_idxs, dtype=np.int32)
+    out = M.contents.row_gather_alloc(M, D.iptr(idx), n_idxs)
+    M.contents.row_gather_fill_values(M, out)
+    cp, ci, cx = D.mat_csr(out)
+    assert out.contents.nnz == n_idxs and len(ci) == n_idxs
+    assert np.array_equal(cp, p) and np.array_equal(ci, i) and np.all(cx == 1.0)
+    L.free_matrix(out)
+    L.free_matrix(M)
+
+
+def test_ata_alloc_beyond_int_max_exits():
+    """ATA_matrices_alloc of the 2,000,000 x 500,000 rank-Zipf A: nnz(A^T A)
+    = 6.7e9 does not fit the int32 CSR of the ABI, so the call prints the
+    error and exits(1), as the reference does on int overflow
+    (sparse_matrix.c:92-97); run in a child process."""
+    import subprocess
+    import sys
+    code = (
+        "import sys; sys.path.insert(0, %r)\n"
+        "from sparsediff_linalg_amd import workloads as W\n"
+        "from tests import dropin as D\n"
+        "p, i, x = W.zipf_csr(2, 2_000_000, 500_000)\n"
+        "M = D.sparse(p, i, x, 2_000_000, 500_000)\n"
+        "print('calling', flush=True)\n"
+        "D.lib().ATA_matrices_alloc(M)\n"
+        "print('returned', flush=True)\n") % (os.path.dirname(os.path.dirname(__file__)),)
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=240)
+    assert r.returncode == 1, (r.returncode, r.stdout[-500:], r.stderr[-500:])
+    assert "calling" in r.stdout and "returned" not in r.stdout
+    assert "ATA_matrices_alloc" in r.stderr and "INT_MAX" in r.stderr, r.stderr[-500:]

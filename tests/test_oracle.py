@@ -460,3 +460,59 @@ def test_spgemm_range_equals_whole_product(blocks):
         assert np.array_equal(gi, Ci[a:b]) and bits_equal(gx, Cx[a:b])
     assert O.spgemm_range(0, Lm * blocks, Lm, *args, blocks, d, nL * blocks, An, *Ac,
                           cap=max(len(Ci) // 2, 1)) is None
+
+
+def test_tracked_alloc_header_and_counters(tmp_path):
+    """include/sparsediff_linalg/tracked_alloc.h as a C caller uses it
+    (reference tracked_alloc.h:40-136): with -DSP_TRACK_MEMORY the inline
+    sp_* wrappers and the library's own host allocations (a CSR struct's
+    arrays) move the live / peak counters the library exports, freeing brings
+    the live count back, sp_memory_reset_peak resets the mark; without the
+    define the wrappers are libc's.  Host allocations only: no GPU call."""
+    import subprocess
+    from sparsediff_linalg_amd import LIB_PATH
+    root = os.path.dirname(os.path.dirname(__file__))
+    src = tmp_path / "t.c"
+    src.write_text(r'''
+#include <stdio.h>
+#include "sparsediff_linalg/tracked_alloc.h"
+#include "sparsediff_linalg/CSR_matrix.h"
+int main(void)
+{
+#ifdef SP_TRACK_MEMORY
+    size_t l0, p0, l1, p1, l2, p2, l3, p3;
+    sp_memory_stats(&l0, &p0);
+    char *a = (char *) sp_malloc(1000);
+    sp_memory_stats(&l1, &p1);
+    CSR_matrix *C = new_CSR_matrix(100, 50, 400);
+    sp_memory_stats(&l2, &p2);
+    free_CSR_matrix(C);
+    a = (char *) sp_realloc(a, 5000);
+    sp_free(a);
+    sp_memory_stats(&l3, &p3);
+    sp_memory_reset_peak();
+    size_t l4, p4;
+    sp_memory_stats(&l4, &p4);
+    int ok = l1 >= l0 + 1000 && p1 >= l1 && l2 >= l1 + 101 * 4 + 400 * 12 && p2 >= l2
+             && l3 == l0 && p3 >= l2 && p4 == l4 && l4 == l0 && sp_calloc(0, 0) != (void *) 1;
+    printf("%s %zu %zu %zu %zu\n", ok ? "OK" : "BAD", l0, l1, l2, l3);
+    return ok ? 0 : 1;
+#else
+    void *a = sp_calloc(10, 10);
+    a = sp_realloc(a, 500);
+    sp_free(a);
+    printf("OK\n");
+    return 0;
+#endif
+}
+''')
+    libdir = os.path.dirname(LIB_PATH)
+    for define in (["-DSP_TRACK_MEMORY"], []):
+        exe = tmp_path / ("t_track" if define else "t_plain")
+        subprocess.run(["gcc", "-std=c99", "-O1", *define, "-I", os.path.join(root, "include"),
+                        str(src), "-o", str(exe), "-L", libdir, "-l:libsdl_amd.so",
+                        f"-Wl,-rpath,{libdir}"], check=True)
+        r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=60)
+        assert r.returncode == 0 and r.stdout.startswith("OK"), (define, r.stdout, r.stderr)
+    lib = C.CDLL(LIB_PATH, mode=os.RTLD_LOCAL)
+    assert hasattr(lib, "g_allocated_bytes") and hasattr(lib, "g_peak_bytes")
