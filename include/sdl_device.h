@@ -74,9 +74,11 @@ const long long *sdl_btda_plan_rowptr(const sdl_btda_plan *plan);
 const int *sdl_btda_plan_colind(const sdl_btda_plan *plan);
 /* Copy C's structure to host int32 arrays (fails if nnz > INT_MAX). */
 int sdl_btda_plan_structure(const sdl_btda_plan *plan, int *hp, int *hi);
-/* Symbolic-pass statistics: rows, terms, nnz, hash/bitmap/global rows,
-   overflow and predicted rows (8 values). */
-int sdl_btda_plan_stats(const sdl_btda_plan *plan, long long *out8);
+/* Symbolic-pass statistics (11 values): rows, terms, nnz, the fill pass's
+   hash / LDS-bitmap / global-bitmap rows, overflow and predicted rows, then
+   the first count pass's hash / LDS-bitmap / global-bitmap rows (the count
+   pass sizes each row from min(terms, column span), or from its prediction). */
+int sdl_btda_plan_stats(const sdl_btda_plan *plan, long long *out11);
 
 /* One refill: dCx[nnz] = B^T diag(dd) A with dAx/dBx the CSR values of A and
    B (dd may be NULL = identity). */
@@ -299,9 +301,13 @@ int sdl_btda_plan_features(const sdl_btda_plan *plan, float *dF);
 int sdl_btda_plan_row_bins(const sdl_btda_plan *plan, int *dY);
 /* Predicted plans, over non-empty rows: counted within the predicted bound,
    counted again from the term bound (the bound was exceeded), predicted bin
-   different from the bin of the exact nnz, bound not lowered (term bound
-   kept). */
-int sdl_btda_plan_pred_stats(const sdl_btda_plan *plan, long long *out4);
+   (tau rule) different from the bin of the exact nnz among the rows the
+   network evaluated, bound not lowered (term bound kept), and the rows the
+   network evaluated (those whose count a prediction could make at least 20 %
+   cheaper; the others keep their term bound unexamined).  The bound of an
+   evaluated row is lowered to the bin of least expected count cost, the
+   chance of a second count weighed by the network's probabilities. */
+int sdl_btda_plan_pred_stats(const sdl_btda_plan *plan, long long *out5);
 
 /* ---- device refills of permuted_dense / stacked_pd product outputs ---- */
 struct matrix;

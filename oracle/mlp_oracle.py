@@ -123,10 +123,10 @@ def row_features(Lp, Li, Ap, Ai, n):
         f[2] = _lg(maxlen)
         f[3] = _lg(minlen)
         f[4] = _lg(mean)
-        f[5] = _lg(n)
-        f[6] = _lg(n * (1 - np.exp(-terms / n)) if n > 0 else 0.0)
-        f[7] = _lg(span)
-        f[8] = _lg(span * (1 - np.exp(-terms / span)) if span > 0 else 0.0)
+        f[5] = _lg(span)
+        f[6] = _lg(span * (1 - np.exp(-terms / span)) if span > 0 else 0.0)
+        f[7] = _lg(sspan)
+        f[8] = _lg(min(terms, sspan))
         f[9] = _lg(min(terms, span))
         f[10] = _lg(kspan)
         f[11] = deg / kspan if kspan > 0 else 0.0

@@ -129,6 +129,9 @@ int sdl_btda_plan_stats(const sdl_btda_plan *plan, long long *o)
     o[5] = s.global_rows;
     o[6] = s.overflow_rows;
     o[7] = s.predicted_rows;
+    o[8] = s.count_hash_rows;
+    o[9] = s.count_bitmap_rows;
+    o[10] = s.count_global_rows;
     return 0;
 }
 

@@ -220,6 +220,7 @@ int sdl_btda_plan_pred_stats(const sdl_btda_plan *plan, long long *o)
     o[1] = s.overflow_rows;
     o[2] = s.misbin_rows;
     o[3] = s.exact_rows;
+    o[4] = s.candidate_rows;
     return 0;
 }
 

@@ -300,20 +300,71 @@ __device__ __forceinline__ RowCtx row_ctx(int row, int Lm, const int *Lp, int nL
 }
 
 // ------------------------------------------------------------ kernels ----
-__global__ void k_row_terms(int64_t m_out, int Lm, const int *__restrict__ Lp,
-                            const int *__restrict__ Li, int nL, const int *__restrict__ Ap,
-                            int64_t *__restrict__ terms)
+// Smallest and largest column of every row of A (one wavefront per row;
+// rows need not be sorted).  Empty rows get (INT_MAX, -1).
+__global__ void __launch_bounds__(256)
+    k_arow_range(int Am, const int *__restrict__ Ap, const int *__restrict__ Ai,
+                 int2 *__restrict__ range)
 {
-    int64_t row = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
-    if (row >= m_out) return;
-    RowCtx c = row_ctx((int) row, Lm, Lp, nL);
-    int64_t t = 0;
-    for (int q = c.q0; q < c.q1; q++)
+    const int64_t k = (int64_t) blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (k >= Am) return;
+    int lo = INT_MAX, hi = -1;
+    for (int t = Ap[k] + lane; t < Ap[k + 1]; t += 64)
     {
-        int k = Li[q] + c.koff;
-        t += Ap[k + 1] - Ap[k];
+        const int col = Ai[t];
+        lo = min(lo, col);
+        hi = max(hi, col);
     }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+    {
+        lo = min(lo, __shfl_xor(lo, off, 64));
+        hi = max(hi, __shfl_xor(hi, off, 64));
+    }
+    if (lane == 0) range[k] = make_int2(lo, hi);
+}
+
+// Per output row: its term count, and (bound non-null) its nnz bound
+// min(terms, column span): a row's columns all lie between the smallest and
+// the largest column of its A rows (arange, from k_arow_range), so the span
+// bounds the distinct count exactly.  16 lanes per row, four rows per
+// wavefront.
+__global__ void __launch_bounds__(256)
+    k_row_terms(int64_t m_out, int Lm, const int *__restrict__ Lp, const int *__restrict__ Li,
+                int nL, const int *__restrict__ Ap, const int2 *__restrict__ arange,
+                int64_t *__restrict__ terms, int64_t *__restrict__ bound)
+{
+    const int64_t row = (int64_t) blockIdx.x * 16 + (threadIdx.x >> 4);
+    const int sub = threadIdx.x & 15;
+    const bool live = row < m_out;
+    int64_t t = 0;
+    int cmin = INT_MAX, cmax = -1;
+    if (live)
+    {
+        RowCtx c = row_ctx((int) row, Lm, Lp, nL);
+        for (int q = c.q0 + sub; q < c.q1; q += 16)
+        {
+            const int k = Li[q] + c.koff;
+            t += Ap[k + 1] - Ap[k];
+            if (bound)
+            {
+                const int2 r = arange[k];
+                cmin = min(cmin, r.x);
+                cmax = max(cmax, r.y);
+            }
+        }
+    }
+#pragma unroll
+    for (int off = 8; off > 0; off >>= 1)
+    {
+        t += __shfl_xor(t, off, 16);
+        cmin = min(cmin, __shfl_xor(cmin, off, 16));
+        cmax = max(cmax, __shfl_xor(cmax, off, 16));
+    }
+    if (!live || sub != 0) return;
     terms[row] = t;
+    if (bound) bound[row] = t > 0 ? min(t, (int64_t) cmax - cmin + 1) : 0;
 }
 
 __device__ __forceinline__ int classify(int64_t cap, int nwords, int lds_words)
@@ -509,13 +560,15 @@ __device__ __forceinline__ float lg(double x) { return (float) log2(1.0 + x); }
 // columns), then wavefront reductions.  The sums are of integers below 2^53,
 // so they are exact in any order.
 __global__ void __launch_bounds__(256)
-    k_row_features(int64_t m_out, int Lm, const int *__restrict__ Lp,
+    k_row_features(int64_t nrows, const int *__restrict__ list, int Lm, const int *__restrict__ Lp,
                    const int *__restrict__ Li, int nL, const int *__restrict__ Ap,
                    const int *__restrict__ Ai, int An, float *__restrict__ F)
 {
-    const int64_t row = (int64_t) blockIdx.x * 4 + (threadIdx.x >> 6);
+    // feature row i describes output row list[i] (list null: row i)
+    const int64_t i = (int64_t) blockIdx.x * 4 + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
-    if (row >= m_out) return;
+    if (i >= nrows) return;
+    const int64_t row = list ? list[i] : i;
     RowCtx c = row_ctx((int) row, Lm, Lp, nL);
     const int deg = c.q1 - c.q0;
     double terms = 0, maxlen = 0, minlen = 1e300, sumsq = 0, sspan = 0;
@@ -554,20 +607,20 @@ __global__ void __launch_bounds__(256)
     }
     if (lane != 0) return;
     if (terms == 0) minlen = 0;
-    const double n = An;
+    (void) An;  // widths are the row's column span, never n
     const double span = terms > 0 ? (double) (cmax - cmin + 1) : 0.0;
     const double kspan = deg > 0 ? (double) (kmax - kmin + 1) : 0.0;
     const double mean = deg > 0 ? terms / deg : 0.0;
-    float *f = F + row * 16;
+    float *f = F + i * 16;
     f[0] = lg(terms);
     f[1] = lg(deg);
     f[2] = lg(maxlen);
     f[3] = lg(minlen);
     f[4] = lg(mean);
-    f[5] = lg(n);
-    f[6] = lg(n > 0 ? n * (1.0 - exp(-terms / n)) : 0.0);
-    f[7] = lg(span);
-    f[8] = lg(span > 0 ? span * (1.0 - exp(-terms / span)) : 0.0);
+    f[5] = lg(span);
+    f[6] = lg(span > 0 ? span * (1.0 - exp(-terms / span)) : 0.0);
+    f[7] = lg(sspan);
+    f[8] = lg(fmin(terms, sspan));
     f[9] = lg(fmin(terms, span));
     f[10] = lg(kspan);
     f[11] = kspan > 0 ? (float) (deg / kspan) : 0.0f;
@@ -577,63 +630,148 @@ __global__ void __launch_bounds__(256)
     f[15] = lg(span > 0 ? terms / span : 0.0);
 }
 
-// Count-pass size bound of every row: the term bound, lowered to the largest
-// nnz of the row's predicted bin when that bin is bounded and the term bound
-// would need a column bitmap at least four times the predicted hash table (in
-// LDS) or one in global memory.  The count pass sizes each row's accumulator
-// from this bound, so such a row gets a small LDS hash table; rows whose
-// term-bound accumulator is already small keep it (a wrong prediction there
-// would cost a second count for little gain).
-__global__ void k_pred_cap(int64_t m, const int64_t *__restrict__ terms,
-                           const unsigned char *__restrict__ pbin, int nwords, int lds_words,
-                           int64_t *__restrict__ cap)
+// pbin of a row the network did not evaluate (no prediction could help it)
+constexpr unsigned char PBIN_NONE = 255;
+
+// Relative cost (in word operations) of counting a row of t terms with the
+// accumulator a size bound of `cap` gets: a hash table of S slots is cleared
+// and scanned (2S) and takes t probing inserts (2t); an LDS column bitmap of
+// nwords words costs 2 nwords + t; a bitmap in global memory the same through
+// L2 atomics, eight times slower.
+__device__ __forceinline__ double count_cost(int64_t cap, int64_t t, int nwords, int lds_words)
 {
-    int64_t row = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
-    if (row >= m) return;
-    const int64_t t = terms[row];
-    const int b = pbin[row];
-    int64_t c = t;
-    if (b < MLP_NBIN_BOUNDED && t > mlp_bin_max(b))
-    {
-        const int ct = classify(t, nwords, lds_words);
-        int64_t slots = 64;
-        while (slots < 2 * (int64_t) mlp_bin_max(b)) slots <<= 1;
-        if (ct == CLS_GLOBAL || (ct == CLS_BITMAP && (int64_t) nwords >= 4 * slots))
-            c = mlp_bin_max(b);
-    }
-    cap[row] = c;
+    const int c = classify(cap, nwords, lds_words);
+    if (c == 0) return 0.0;
+    // a fixed part per row: the workgroup's start and the row's setup
+    if (c < CLS_BITMAP) return 512.0 + 2.0 * (double) (64 << (c - 1)) + 2.0 * (double) t;
+    if (c == CLS_BITMAP) return 512.0 + 2.0 * nwords + (double) t;
+    return 8.0 * (512.0 + 2.0 * nwords + (double) t);
 }
 
-// Rows to count again from the term bound: those whose count exceeded their
-// predicted bound.  A hash table has at least twice its row's bound in slots
+// Rows a prediction could help: those where even a certain prediction of the
+// smallest bin below the term bound would cost under 0.8 of the count from
+// the term bound.  Only they get features and a network evaluation; one
+// atomic per wavefront appends them to list.
+__global__ void __launch_bounds__(256)
+    k_pred_candidates(int64_t m, const int64_t *__restrict__ terms,
+                      const int64_t *__restrict__ bound, int nwords, int lds_words,
+                      int *__restrict__ list, int *__restrict__ count)
+{
+    const int64_t row = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    bool cand = false;
+    if (row < m)
+    {
+        const int64_t t = terms[row], u = bound[row];
+        if (u > mlp_bin_max(0))
+            cand = count_cost(mlp_bin_max(0), t, nwords, lds_words) <
+                   0.8 * count_cost(u, t, nwords, lds_words);
+    }
+    const unsigned long long v = __ballot(cand);
+    const int lane = threadIdx.x & 63;
+    int base = 0;
+    if (lane == 0 && v) base = atomicAdd(count, (int) __popcll(v));
+    base = __shfl(base, 0, 64);
+    if (cand) list[base + __popcll(v & ((1ull << lane) - 1ull))] = (int) row;
+}
+
+// Count-pass size bound of every row from the network's bin probabilities
+// (logits Z[row][6]): lowering the row's bound to the largest nnz of bin b
+// costs the count with that bound's accumulator, plus, with probability
+// P(nnz > bound_b) = 1 - (p_0 + ... + p_b), a second count from the term
+// bound.  The bound is lowered to the bin of least expected cost when that
+// cost is below 0.8 of the count from the term bound; otherwise the term
+// bound stays.  Uncertain rows (spread probabilities) and rows whose inserts
+// dominate (hub rows) keep their term bound.  pbin gets the tau-rule bin (the
+// smallest bin whose cumulative probability reaches tau) for the statistics.
+__global__ void k_pred_cap(int64_t n, const int *__restrict__ list,
+                           const int64_t *__restrict__ terms, const int64_t *__restrict__ ubound,
+                           const float *__restrict__ F,
+                           const float *__restrict__ Z, float tau, int nwords, int lds_words,
+                           int64_t *__restrict__ cap, unsigned char *__restrict__ pbin)
+{
+    // F[i] / Z[i] hold the features / logits of output row list[i]
+    const int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int64_t row = list[i];
+    const int64_t t = terms[row], u = ubound[row];
+    // the longest A row among the row's terms puts that many distinct columns
+    // in it (feature 2 is log2(1 + longest)): bins below it are impossible
+    const double longest = rint(exp2((double) F[i * MLP_IN + 2]) - 1.0);
+    const float *z = Z + i * MLP_OUT;
+    float p[MLP_OUT];
+    float mx = z[0];
+#pragma unroll
+    for (int c = 1; c < MLP_OUT; c++) mx = fmaxf(mx, z[c]);
+    float s = 0.0f;
+#pragma unroll
+    for (int c = 0; c < MLP_OUT; c++)
+    {
+        p[c] = expf(z[c] - mx);
+        s = s + p[c];
+    }
+    int tb = MLP_OUT - 1;
+    float cum = 0.0f;
+    for (int c = 0; c < MLP_OUT; c++)
+    {
+        p[c] = p[c] / s;
+        cum = cum + p[c];
+        if (cum >= tau && tb == MLP_OUT - 1 && c < MLP_OUT - 1) tb = c;
+    }
+    pbin[row] = (unsigned char) tb;
+    int64_t c_out = u;
+    if (u > 0)
+    {
+        const double e0 = count_cost(u, t, nwords, lds_words);
+        double best = 0.8 * e0;
+        double below = 0.0;  // P(nnz <= bound_b)
+        for (int b = 0; b < MLP_NBIN_BOUNDED; b++)
+        {
+            below += p[b];
+            const int64_t bound = mlp_bin_max(b);
+            if (bound >= u) break;  // the exact bound is already at most this bin's
+            if ((double) bound < longest) continue;
+            const double eb = count_cost(bound, t, nwords, lds_words) +
+                              fmax(0.0, 1.0 - below) * e0;
+            if (eb < best)
+            {
+                best = eb;
+                c_out = bound;
+            }
+        }
+    }
+    cap[row] = c_out;
+}
+
+// Rows to count again from the exact bound (min(terms, span)): those whose
+// count exceeded their predicted bound.  A hash table has at least twice its row's bound in slots
 // and a failed insert leaves it full, so a count within the bound is exact.
 __global__ void k_pred_recount(int64_t m, const int *__restrict__ rowcnt,
                                const int64_t *__restrict__ cap,
-                               const int64_t *__restrict__ terms, int64_t *__restrict__ recount)
+                               const int64_t *__restrict__ bound, int64_t *__restrict__ recount)
 {
     int64_t row = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
     if (row >= m) return;
-    const int64_t t = terms[row];
-    recount[row] = (cap[row] < t && (int64_t) rowcnt[row] > cap[row]) ? t : 0;
+    const int64_t u = bound[row];
+    recount[row] = (cap[row] < u && (int64_t) rowcnt[row] > cap[row]) ? u : 0;
 }
 
 // Over non-empty rows: [0] counted within the predicted bound, [1] counted
 // again (the predicted bound was exceeded), [2] predicted bin differs from
-// the bin of the exact nnz, [3] predicted bound not below the term bound.
+// the bin of the exact nnz, [3] predicted bound not below the exact bound.
 __global__ void __launch_bounds__(256)
     k_pred_stats(int64_t m, const int64_t *__restrict__ cap,
                  const int64_t *__restrict__ recount, const unsigned char *__restrict__ pbin,
-                 const int64_t *__restrict__ terms, const int64_t *__restrict__ Cp,
+                 const int64_t *__restrict__ bound, const int64_t *__restrict__ Cp,
                  unsigned long long *__restrict__ out)
 {
     int64_t row = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
     bool f[4] = {false, false, false, false};
-    if (row < m && terms[row] > 0)
+    if (row < m && bound[row] > 0)
     {
-        const bool lowered = cap[row] < terms[row];
+        const bool lowered = cap[row] < bound[row];
         f[0] = lowered && recount[row] == 0;
         f[1] = lowered && recount[row] != 0;
-        f[2] = pbin[row] != mlp_bin_of(Cp[row + 1] - Cp[row]);
+        f[2] = pbin[row] != PBIN_NONE && pbin[row] != mlp_bin_of(Cp[row + 1] - Cp[row]);
         f[3] = !lowered;
     }
     const int lane = threadIdx.x & 63;
@@ -1755,11 +1893,18 @@ void run_symbolic_class(int c, const Bins &B, const ProductPlan &P, int nwords, 
 
 // Count pass: each row's accumulator sized from cap[row] (rows with cap 0 are
 // skipped); rowcnt[row] = the row's distinct columns.
-void symbolic_count(const ProductPlan &P, const int64_t *cap, int *rowcnt, int nwords)
+void symbolic_count(ProductPlan &P, const int64_t *cap, int *rowcnt, int nwords, bool first)
 {
     Bins B;
     bin_rows<int64_t>(B, cap, P.m_out, nwords);
     for (int c = 1; c < NCLASS; c++) run_symbolic_class<0>(c, B, P, nwords, rowcnt, nullptr, nullptr);
+    if (first)
+    {
+        P.stats.count_hash_rows = 0;
+        for (int c = 1; c < CLS_BITMAP; c++) P.stats.count_hash_rows += B.count[c];
+        P.stats.count_bitmap_rows = B.count[CLS_BITMAP];
+        P.stats.count_global_rows = B.count[CLS_GLOBAL];
+    }
     sync();
 }
 
@@ -1784,9 +1929,11 @@ void symbolic_fill(ProductPlan &P, const int *rowcnt, int nwords)
 }
 
 // Count pass with accumulators sized from the term bound, scan, fill pass.
-void symbolic_exact(ProductPlan &P, const int64_t *rterms, int *rowcnt, int nwords)
+// Count pass sized from each row's exact nnz bound min(terms, span), then
+// the scan and the fill pass.
+void symbolic_exact(ProductPlan &P, const int64_t *rbound, int *rowcnt, int nwords)
 {
-    symbolic_count(P, rterms, rowcnt, nwords);
+    symbolic_count(P, rbound, rowcnt, nwords, true);
     symbolic_fill(P, rowcnt, nwords);
 }
 
@@ -1795,36 +1942,50 @@ void symbolic_exact(ProductPlan &P, const int64_t *rterms, int *rowcnt, int nwor
 // exceeded the predicted bound, then the scan and fill pass of the exact path.
 // With no predictor every bin is unbounded and this is the exact path.
 void symbolic_predicted(ProductPlan &P, const RowPredictor *pred, const int64_t *rterms,
-                        int *rowcnt, int nwords)
+                        const int64_t *rbound, int *rowcnt, int nwords)
 {
     const int64_t m_out = P.m_out;
     const unsigned grid = (unsigned) ((m_out + 255) / 256);
     DBuf<unsigned char> pbin(m_out);
-    DBuf<float> F;
+    DBuf<int64_t> cap(m_out), recount(m_out);
+    // every row starts from its term bound (the exact path's sizes); rows a
+    // prediction could help get features, the network and a cost decision
+    SDL_HIP(hipMemsetAsync(pbin.p, PBIN_NONE, (size_t) m_out, stream()));
+    d2d(cap.p, rbound, (size_t) m_out * sizeof(int64_t));
     if (pred)
     {
-        F.alloc((size_t) m_out * MLP_IN);
-        product_row_features(P.L, P.A, F.p, stream());
-        mlp_predict(*pred, F.p, m_out, pbin.p, stream());
+        DBuf<int> list(m_out), ncand(1);
+        dzero(ncand.p, sizeof(int));
+        hipLaunchKernelGGL(k_pred_candidates, dim3(grid), dim3(256), 0, stream(), m_out, rterms,
+                           rbound, nwords, lds_bitmap_words(), list.p, ncand.p);
+        SDL_HIP(hipGetLastError());
+        int nc = 0;
+        d2h(&nc, ncand.p, sizeof(int));
+        P.stats.candidate_rows = nc;
+        if (nc > 0)
+        {
+            DBuf<float> F((size_t) nc * MLP_IN), Z((size_t) nc * MLP_OUT);
+            hipLaunchKernelGGL(k_row_features, dim3((unsigned) ((nc + 3) / 4)), dim3(256), 0,
+                               stream(), (int64_t) nc, list.p, P.L.Lm, P.L.Lp, P.L.Li,
+                               P.A.Am / P.L.blocks, P.A.Ap, P.A.Ai, P.A.An, F.p);
+            SDL_HIP(hipGetLastError());
+            mlp_forward(*pred, F.p, nc, Z.p, stream());
+            hipLaunchKernelGGL(k_pred_cap, dim3((unsigned) ((nc + 255) / 256)), dim3(256), 0,
+                               stream(), (int64_t) nc, list.p, rterms, rbound, F.p, Z.p,
+                               pred->tau, nwords, lds_bitmap_words(), cap.p, pbin.p);
+            SDL_HIP(hipGetLastError());
+        }
     }
-    else
-    {
-        SDL_HIP(hipMemsetAsync(pbin.p, MLP_NBIN_BOUNDED, (size_t) m_out, stream()));
-    }
-    DBuf<int64_t> cap(m_out), recount(m_out);
-    hipLaunchKernelGGL(k_pred_cap, dim3(grid), dim3(256), 0, stream(), m_out, rterms, pbin.p,
-                       nwords, lds_bitmap_words(), cap.p);
-    SDL_HIP(hipGetLastError());
-    symbolic_count(P, cap.p, rowcnt, nwords);
+    symbolic_count(P, cap.p, rowcnt, nwords, true);
     hipLaunchKernelGGL(k_pred_recount, dim3(grid), dim3(256), 0, stream(), m_out, rowcnt, cap.p,
-                       rterms, recount.p);
+                       rbound, recount.p);
     SDL_HIP(hipGetLastError());
-    symbolic_count(P, recount.p, rowcnt, nwords);
+    symbolic_count(P, recount.p, rowcnt, nwords, false);
     symbolic_fill(P, rowcnt, nwords);
     DBuf<unsigned long long> st(4);
     dzero(st.p, st.bytes());
     hipLaunchKernelGGL(k_pred_stats, dim3(grid), dim3(256), 0, stream(), m_out, cap.p, recount.p,
-                       pbin.p, rterms, P.Cp.p, st.p);
+                       pbin.p, rbound, P.Cp.p, st.p);
     SDL_HIP(hipGetLastError());
     unsigned long long h[4];
     d2h(h, st.p, sizeof(h));
@@ -1847,8 +2008,9 @@ int64_t product_count_terms(const LView &L, const AView &A, int64_t *row_terms_o
         tmp.alloc(m_out);
         terms = tmp.p;
     }
-    hipLaunchKernelGGL(k_row_terms, dim3((unsigned) ((m_out + 255) / 256)), dim3(256), 0,
-                       stream(), m_out, L.Lm, L.Lp, L.Li, A.Am / L.blocks, A.Ap, terms);
+    hipLaunchKernelGGL(k_row_terms, dim3((unsigned) ((m_out + 15) / 16)), dim3(256), 0,
+                       stream(), m_out, L.Lm, L.Lp, L.Li, A.Am / L.blocks, A.Ap,
+                       (const int2 *) nullptr, terms, (int64_t *) nullptr);
     SDL_HIP(hipGetLastError());
     DBuf<int64_t> off(m_out + 1);
     return scan_i64(terms, off.p, m_out, true);
@@ -1859,7 +2021,8 @@ void product_row_features(const LView &L, const AView &A, float *F, hipStream_t 
     const int64_t m_out = (int64_t) L.Lm * L.blocks;
     if (m_out == 0) return;
     hipLaunchKernelGGL(k_row_features, dim3((unsigned) ((m_out + 3) / 4)), dim3(256), 0, s,
-                       m_out, L.Lm, L.Lp, L.Li, A.Am / L.blocks, A.Ap, A.Ai, A.An, F);
+                       m_out, (const int *) nullptr, L.Lm, L.Lp, L.Li, A.Am / L.blocks, A.Ap,
+                       A.Ai, A.An, F);
     SDL_HIP(hipGetLastError());
 }
 
@@ -1888,8 +2051,17 @@ void product_plan(ProductPlan &P, const LView &L, const AView &A, SymMode mode,
 
     // per-row term bound and its scan
     DBuf<int64_t> rterms(m_out);
-    hipLaunchKernelGGL(k_row_terms, dim3((unsigned) ((m_out + 255) / 256)), dim3(256), 0,
-                       stream(), m_out, L.Lm, L.Lp, L.Li, nL, A.Ap, rterms.p);
+    DBuf<int64_t> rbound(m_out);
+    {
+        DBuf<int2> arange(std::max(A.Am, 1));
+        if (A.Am > 0)
+            hipLaunchKernelGGL(k_arow_range, dim3((unsigned) ((A.Am + 3) / 4)), dim3(256), 0,
+                               stream(), A.Am, A.Ap, A.Ai, arange.p);
+        hipLaunchKernelGGL(k_row_terms, dim3((unsigned) ((m_out + 15) / 16)), dim3(256), 0,
+                           stream(), m_out, L.Lm, L.Lp, L.Li, nL, A.Ap, arange.p, rterms.p,
+                           rbound.p);
+        SDL_HIP(hipGetLastError());
+    }
     SDL_HIP(hipGetLastError());
     DBuf<int64_t> toff(m_out + 1);
     P.terms = scan_i64(rterms.p, toff.p, m_out, true);
@@ -1898,8 +2070,9 @@ void product_plan(ProductPlan &P, const LView &L, const AView &A, SymMode mode,
     // structure: count pass sized from the term bound or the predicted bins, fill pass
     DBuf<int> rowcnt(m_out);
     dzero(rowcnt.p, rowcnt.bytes());
-    if (mode == SymMode::Predicted) symbolic_predicted(P, pred, rterms.p, rowcnt.p, nwords);
-    else symbolic_exact(P, rterms.p, rowcnt.p, nwords);
+    if (mode == SymMode::Predicted)
+        symbolic_predicted(P, pred, rterms.p, rbound.p, rowcnt.p, nwords);
+    else symbolic_exact(P, rbound.p, rowcnt.p, nwords);
 
     // refill schedule (reference policy: a plan iff terms <= 3 nnz, gustavson.h:45-52)
     P.pull = allow_pull && P.terms > 0 && (force_pull || P.terms <= 3 * P.nnz) &&

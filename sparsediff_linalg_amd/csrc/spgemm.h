@@ -44,6 +44,8 @@ struct SymStats
     long long predicted_rows = 0;      // predicted path: counted within the predicted bound
     long long misbin_rows = 0;         // predicted path: predicted bin != bin of the exact nnz
     long long exact_rows = 0;          // predicted path: term bound kept (not lowered)
+    long long candidate_rows = 0;      // predicted path: rows given features and the network
+    long long count_hash_rows = 0, count_bitmap_rows = 0, count_global_rows = 0;  // first count pass
 };
 
 // Launch configuration of the batched pull refill (picked by the autotuner):
@@ -130,11 +132,13 @@ void product_plan(ProductPlan &P, const LView &L, const AView &A, SymMode mode,
 
 // The 16 structural features of every output row (F[row * 16 + f], FP32):
 // log2(1 + x) of the term bound, the degree, the longest / shortest A row,
-// the mean A row, n, the uniform-collision estimate n (1 - e^(-terms/n)),
-// the column span of the row's A rows, the uniform estimate within the span,
-// min(terms, span), the A-row index span, degree / index span, the root of
-// the sum of squared A-row lengths, the mean single-row column span, the
-// length skew and the term density within the span.
+// the mean A row, the column span of the row's A rows, the uniform-collision
+// estimate within that span, span (1 - e^(-terms/span)), the summed
+// single-row column spans, min(terms, summed spans), min(terms, span), the
+// A-row index span, degree / index span, the root of the sum of squared
+// A-row lengths, the mean single-row column span, the length skew and the
+// term density within the span.  Widths are spans, never n: a row's columns
+// all lie in its span, and for uniformly spread columns the span is n.
 void product_row_features(const LView &L, const AView &A, float *F, hipStream_t s);
 
 // One refill: Cx = values of kron(I, L) D A.  Lval / Ax / d are device arrays

@@ -78,9 +78,9 @@ class Plan:
         return Y
 
     def pred_stats(self):
-        o = (C.c_longlong * 4)()
+        o = (C.c_longlong * 5)()
         check(self.lib.sdl_btda_plan_pred_stats(self.h, o), "pred stats")
-        return dict(predicted=o[0], overflow=o[1], misbin=o[2], exact=o[3])
+        return dict(predicted=o[0], overflow=o[1], misbin=o[2], exact=o[3], candidates=o[4])
 
     def close(self):
         if self.h:

@@ -37,12 +37,15 @@ def test_features_hand_worked():
     Li = [0, 1, 3, 2]
     F = MO.row_features(Lp, Li, Ap, Ai, 10)
     l2 = math.log2
-    e = 10 * (1 - math.exp(-0.6))
-    want0 = [l2(7), 2, 2, 1, l2(3), l2(11), l2(1 + e), l2(11), l2(1 + e), l2(7), l2(5), 0.75,
+    e = 10 * (1 - math.exp(-0.6))  # 6 terms uniform over the span of 10 columns
+    # span 10 (columns 0..9), summed single-row spans 3 + 1 + 10 = 14
+    want0 = [l2(7), 2, 2, 1, l2(3), l2(11), l2(1 + e), l2(15), l2(7), l2(7), l2(5), 0.75,
              l2(1 + math.sqrt(14)), l2(1 + 14 / 3), 2 - l2(3), l2(1.6)]
-    want1 = [0, 1, 0, 0, 0, l2(11), 0, 0, 0, 0, 1, 1.0, 0, 0, 0, 0]
+    want1 = [0, 1, 0, 0, 0, 0, 0, 0, 0, 0, 1, 1.0, 0, 0, 0, 0]
     assert np.allclose(F[0], want0, rtol=0, atol=1e-12)
     assert np.allclose(F[1], want1, rtol=0, atol=1e-12)
+    # widths are the rows' column spans: the features do not depend on n
+    assert np.array_equal(MO.row_features(Lp, Li, Ap, Ai, 1000), F)
 
 
 def test_bins_and_decision_rule():

@@ -3,8 +3,10 @@ kernels against the CPU FP64 oracle (oracle/mlp_oracle.py, itself pinned by
 finite differences); training on the GPU lowers the loss; predicted plans give
 the exact plan's structure bit for bit (trained, absent and adversarial
 predictors, overflow fallback); parameter files round-trip and reject
-corruption; the drop-in allocs use a bound predictor; and the trained
-network's gain over the term-bound sizing on a wide strided-band A^T A."""
+corruption; the drop-in allocs use a bound predictor; the exact path sizes
+its count pass from min(terms, column span), robust to unsorted A rows; and
+what the network trained on configs 1-3 gains over that exact path on a
+held-out product of config 3's law."""
 import ctypes as C
 import os
 import time
@@ -31,7 +33,7 @@ def csc_of(p, i, m, n):
 
 @pytest.fixture(scope="module")
 def trained():
-    X, Y, n = training_set(training_products(small=True, structured=True))
+    X, Y, n = training_set(training_products(small=True))  # configs 1-3 only
     mlp = Predictor(seed=1)
     mlp.train(X, Y, n, epochs=100, batch=2048, lr=3e-3, seed=7)
     yield mlp
@@ -78,7 +80,7 @@ def test_forward_and_gradients_match_oracle():
 
 
 def test_training_lowers_loss_and_predicts_bins(trained):
-    X, Y, n = training_set(training_products(small=True, structured=True))
+    X, Y, n = training_set(training_products(small=True))  # configs 1-3 only
     fresh = Predictor(seed=1)
     G = DeviceArray(MO.NPARAM, np.float32)
     l0, l1 = C.c_float(), C.c_float()
@@ -114,9 +116,12 @@ def cases():
     wb = W.stacked_pd_blocks(5, nblocks=12, lo=16, hi=64, ncols=512)
     p, i, _ = W.blocks_as_csr(wb)
     yield "blocks", (p, i), None, 512, 512
-    # wide: the term bound asks for 128 KB column bitmaps, so predictions apply
+    # wide: the term bound asks for 128 KB column bitmaps (the span bound does not)
     p, i, _ = W.strided_band_csr(1 << 12, 1 << 20, 129, 4)
     yield "strided", (p, i), None, 1 << 20, 1 << 20
+    # uniform columns in a wide space: rows a prediction may size (candidates)
+    p, i, _ = W.csr_uniform(15, 1 << 18, 1 << 15, 8, 8, values=False)
+    yield "uniform-wide", (p, i), None, 1 << 15, 1 << 15
 
 
 @pytest.mark.parametrize("which", ["trained", "none", "adversarial"])
@@ -191,35 +196,110 @@ def test_dropin_alloc_uses_bound_predictor(trained):
         L.free_matrix(M)
 
 
-def test_trained_predictor_gain_on_wide_band(trained):
-    """A^T A of a strided band (2^19 rows of 129 consecutive columns starting
-    4 apart, 2^21 columns; the network saw widths 65 / 129 with strides 2 / 8
-    on 2^14 rows):
-    every output row has ~4128 terms but only ~257 distinct columns.  From the
-    term bound the count pass needs a column bitmap of 2^21 bits per row, too
-    large for LDS, so it runs in global memory; from the predicted bin it is a
-    1024-slot LDS hash table.  Reports both symbolic times; requires the exact
-    structure, few recounts, and the trained network faster."""
-    m, n = 1 << 19, 1 << 21
-    p, i, _ = W.strided_band_csr(m, n, 129, 4)
+def plan_stats(pl):
+    o = (C.c_longlong * 11)()
+    check(load().sdl_btda_plan_stats(pl.h, o), "plan stats")
+    return list(o)
+
+
+def oracle_ata(p, i, n, d):
+    from oracle import oracle as O
+    m = len(p) - 1
+    x = np.linspace(-1.0, 1.0, int(p[-1]))
+    Lp, Li, Lx = O.transpose(m, n, p, i, x)
+    return x, O.spgemm(n, Lp, Li, Lx, 1, d, m, n, p, i, x)
+
+
+def refill_matches(pl, x, d, Cx):
+    dx, dd = DeviceArray.from_numpy(x), DeviceArray.from_numpy(d)
+    out = DeviceArray(max(len(Cx), 1), np.float64)
+    check(load().sdl_btda_refill(pl.h, dx.ptr, dx.ptr, dd.ptr, out.ptr, None))
+    load().sdl_synchronize()
+    return np.array_equal(out.numpy()[:len(Cx)].view(np.int64), Cx.view(np.int64))
+
+
+def test_span_bound_sizes_the_count_pass():
+    """A^T A of a strided band (4096 rows of 129 consecutive columns starting
+    4 apart, in 2^20 columns): each output row has ~4160 terms, so the term
+    bound would count it in a 2^20-bit column bitmap, but its columns span
+    ~253, so the exact path counts every row in an LDS hash table.  Structure
+    and values against the oracle."""
+    n = 1 << 20
+    p, i, _ = W.strided_band_csr(1 << 12, n, 129, 4)
+    pl = Plan((p, i), None, n, n)
+    st = plan_stats(pl)
+    cp, ci = pl.structure()
+    nonempty = int(np.count_nonzero(np.diff(cp)))
+    assert st[8] == nonempty and st[9] == 0 and st[10] == 0, st
+    d = np.linspace(0.5, 3.0, len(p) - 1)
+    x, (Cp, Ci, Cx, _) = oracle_ata(p, i, n, d)
+    assert np.array_equal(cp, Cp.astype(np.int32)) and np.array_equal(ci, Ci)
+    assert refill_matches(pl, x, d, Cx)
+    pl.close()
+
+
+@pytest.mark.parametrize("with_net", [False, True])
+def test_unsorted_a_rows_exact(with_net, trained):
+    """The span bound takes each A row's smallest and largest column, not its
+    first and last: A rows stored with shuffled columns (the reference's
+    Gustavson accepts them) give the oracle's structure and values, exact and
+    predicted."""
+    n = 1 << 20
+    p, i, _ = W.strided_band_csr(1 << 12, n, 129, 4)
+    rng = np.random.default_rng(9)
+    i = i.copy()
+    for r in range(len(p) - 1):
+        i[p[r]:p[r + 1]] = rng.permutation(i[p[r]:p[r + 1]])
+    pl = Plan((p, i), None, n, n, mode=1 if with_net else 0, mlp=trained if with_net else None)
+    d = np.linspace(0.5, 3.0, len(p) - 1)
+    x, (Cp, Ci, Cx, _) = oracle_ata(p, i, n, d)
+    cp, ci = pl.structure()
+    assert np.array_equal(cp, Cp.astype(np.int32)) and np.array_equal(ci, Ci)
+    assert refill_matches(pl, x, d, Cx)
+    pl.close()
+
+
+def test_trained_predictor_on_held_out_blocks(trained):
+    """What the network trained on configs 1-3 gains over the exact path
+    (count pass sized from min(terms, span)) on a held-out product of
+    config 3's law: 256 permuted-dense blocks (m0, n0 log-uniform in
+    [64, 512]) read as CSR, with column permutations drawn from 2^20 columns
+    instead of 4096, A^T A.  Most output rows then come from one block: ~215
+    columns against ~4.6e4 terms and a span near 2^20, so only a prediction
+    can size them small.  Prints both symbolic times and the statistics;
+    requires the exact structure, recounts of at most 5 % of the lowered rows,
+    and no loss beyond 25 % (the timing is noisy; the gain is reported)."""
+    wb = W.stacked_pd_blocks(13, nblocks=256, ncols=1 << 20)
+    p, i, _ = W.blocks_as_csr(wb)
+    n = wb["n"]
     lib = load()
 
     def timed(**kw):
         Plan((p, i), None, n, n, **kw).close()  # warm-up (kernels, allocator)
-        lib.sdl_synchronize()
-        t0 = time.perf_counter()
-        pl = Plan((p, i), None, n, n, **kw)
-        lib.sdl_synchronize()
-        return pl, time.perf_counter() - t0
+        best = None
+        for _ in range(3):
+            lib.sdl_synchronize()
+            t0 = time.perf_counter()
+            pl = Plan((p, i), None, n, n, **kw)
+            lib.sdl_synchronize()
+            dt = time.perf_counter() - t0
+            if best is None or dt < best[1]:
+                if best:
+                    best[0].close()
+                best = (pl, dt)
+            else:
+                pl.close()
+        return best
 
     ex, t_exact = timed()
     pr, t_pred = timed(mode=1, mlp=trained)
     assert structures_equal(ex, pr)
     st = pr.pred_stats()
-    print(f"\nstrided-band A^T A symbolic: term-bound sizing {t_exact * 1e3:.1f} ms, predicted "
-          f"{t_pred * 1e3:.1f} ms; {st}")
-    assert st["overflow"] <= 0.05 * n
-    assert st["predicted"] >= 0.9 * n
-    assert t_pred < t_exact
+    lowered = st["predicted"] + st["overflow"]
+    print(f"\nheld-out blocks A^T A symbolic: exact {t_exact * 1e3:.1f} ms, predicted "
+          f"{t_pred * 1e3:.1f} ms (gain {t_exact / t_pred:.2f}); {st}")
+    assert st["candidates"] > 0
+    assert st["overflow"] <= 0.05 * max(lowered, 1)
+    assert t_pred <= 1.25 * t_exact
     for pl in (ex, pr):
         pl.close()
