@@ -227,6 +227,39 @@ const long long *sdl_lower_coo_value_map(const sdl_lower_coo *coo);
 int sdl_lower_coo_refresh(const sdl_lower_coo *coo, const double *dCx, double *dx, void *stream);
 void sdl_lower_coo_destroy(sdl_lower_coo *coo);
 
+/* ---- Hessian assembly across expression nodes (no reference counterpart) ----
+   The caller-side loop of a Newton solver's Lagrangian Hessian, one
+   evaluation per step: node k contributes C_k = A_k^T diag(d_k) A_k (the
+   chain-rule product of BTDA_matrices_fill_values, matmul_dispatchers.c:73-92,
+   for A_k: m[k] x n CSR, device arrays dAp[k] / dAi[k]); H = C_0 + ... +
+   C_{nnodes-1} with sum_n semantics (matrix_sum.c:112-180: the union
+   structure, each entry summed from +0.0 in node order); and H's lower
+   triangle in the solver's COO layout (refresh_lower_triangular_coo,
+   COO_matrix.c:92-98).  1 to 32 nodes.  The structures are built once at
+   create; each evaluation runs every node's refill, the sum fill and the
+   export as device launches, eagerly (sdl_hessian_eval) or replayed from one
+   HIP graph captured over fixed buffers (sdl_hessian_capture, then
+   sdl_hessian_replay per step: the caller rewrites the d_k / A_k value
+   buffers in place between replays).  Neither allocates.  dAx[k]: A_k's
+   values (device), dd[k]: m[k] doubles (device) or dd NULL (identity); dHx:
+   H's values (sdl_hessian_nnz), dLx: the lower export's values
+   (sdl_hessian_lower_nnz; may be NULL to skip the export). */
+typedef struct sdl_hessian sdl_hessian;
+int sdl_hessian_create(int nnodes, int n, const int *m, const int *const *dAp,
+                       const int *const *dAi, sdl_hessian **out);
+void sdl_hessian_destroy(sdl_hessian *h);
+long long sdl_hessian_nnz(const sdl_hessian *h);
+const long long *sdl_hessian_rowptr(const sdl_hessian *h);
+const int *sdl_hessian_colind(const sdl_hessian *h);
+long long sdl_hessian_lower_nnz(const sdl_hessian *h);
+const int *sdl_hessian_lower_rows(const sdl_hessian *h);
+const int *sdl_hessian_lower_cols(const sdl_hessian *h);
+int sdl_hessian_eval(sdl_hessian *h, const double *const *dAx, const double *const *dd,
+                     double *dHx, double *dLx, void *stream);
+int sdl_hessian_capture(sdl_hessian *h, const double *const *dAx, const double *const *dd,
+                        double *dHx, double *dLx, void *stream);
+int sdl_hessian_replay(const sdl_hessian *h, void *stream);
+
 /* ---- Dense Kronecker-vector products on device arrays ----
    Device forms of the reference's Y_kron_I_vec / I_kron_XT_vec
    (src/support/mini_numpy.c:74-105): same index conventions, every output a

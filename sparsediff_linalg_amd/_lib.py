@@ -119,6 +119,18 @@ SIGNATURES = {
     "sdl_btda_refill_batched_static": (C.c_int, [vp, vp, vp, vp, C.c_longlong, C.c_int, vp,
                                                  C.c_longlong, vp]),
     "sdl_static_scratch_bytes": (C.c_longlong, []),
+    "sdl_hessian_create": (C.c_int, [C.c_int, C.c_int, c_int_p, C.POINTER(vp), C.POINTER(vp),
+                                     C.POINTER(vp)]),
+    "sdl_hessian_destroy": (None, [vp]),
+    "sdl_hessian_nnz": (C.c_longlong, [vp]),
+    "sdl_hessian_rowptr": (vp, [vp]),
+    "sdl_hessian_colind": (vp, [vp]),
+    "sdl_hessian_lower_nnz": (C.c_longlong, [vp]),
+    "sdl_hessian_lower_rows": (vp, [vp]),
+    "sdl_hessian_lower_cols": (vp, [vp]),
+    "sdl_hessian_eval": (C.c_int, [vp, C.POINTER(vp), C.POINTER(vp), vp, vp, vp]),
+    "sdl_hessian_capture": (C.c_int, [vp, C.POINTER(vp), C.POINTER(vp), vp, vp, vp]),
+    "sdl_hessian_replay": (C.c_int, [vp, vp]),
     "sdl_coo_plan_create": (C.c_int, [C.c_longlong, vp, vp, C.c_int, C.c_int, C.POINTER(vp)]),
     "sdl_coo_plan_nnz": (C.c_longlong, [vp]),
     "sdl_coo_plan_rowptr": (vp, [vp]),
@@ -248,6 +260,15 @@ class DeviceArray:
 
     def offset(self, elems):
         return self.ptr + int(elems) * self.dtype.itemsize
+
+    def copy_from(self, a):
+        """Overwrite the whole allocation with host array a (same size), in
+        place: device pointers held elsewhere (a captured graph) stay valid."""
+        a = np.ascontiguousarray(a, self.dtype)
+        if a.nbytes != self.nbytes:
+            raise ValueError(f"copy_from: {a.nbytes} bytes into {self.nbytes}")
+        if a.nbytes:
+            check(self.lib.sdl_memcpy_htod(self.ptr, a.ctypes.data, a.nbytes), "htod")
 
     def free(self):
         if self.ptr:

@@ -273,3 +273,23 @@ def sum_operands(seed: int = 6, m: int = 1_000_000, n: int = 1_000_000, nops: in
         i = (keys % n).astype(np.int32)
         ops.append((p.astype(np.int32), i, normal(seed * 100 + k, len(i))))
     return ops
+
+
+def lagrangian_nodes(seed: int = 7, n: int = 200_000, nnodes: int = 12, m_lo: int = 20_000,
+                     m_hi: int = 120_000, nnz_lo: int = 3, nnz_hi: int = 9,
+                     window: int = 50_000):
+    """Chain-rule nodes of a synthetic Lagrangian Hessian over n variables:
+    node k is an m_k x n Jacobian-like CSR (m_k log-uniform in [m_lo, m_hi],
+    row lengths uniform in [nnz_lo, nnz_hi], distinct columns uniform in a
+    window of `window` variables starting at a node-specific offset, so the
+    nodes' Hessian blocks overlap), FP64 N(0,1) values.  Returns a list of
+    (p, i, x) and n."""
+    g = np.random.Generator(np.random.PCG64(seed))
+    nodes = []
+    for k in range(nnodes):
+        m = int(np.rint(np.exp(g.uniform(np.log(m_lo), np.log(m_hi)))))
+        w = min(window, n)
+        c0 = int(g.integers(0, n - w + 1))
+        p, i, x = csr_uniform(seed * 1000 + k, m, w, nnz_lo, min(nnz_hi, w))
+        nodes.append((p, (i + c0).astype(np.int32), x))
+    return nodes, n
