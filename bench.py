@@ -57,7 +57,7 @@ def parse():
     ap.add_argument("--workload", default="btda",
                     choices=["btda", "blocks-atda", "blocks-btda", "multi", "ata-large", "coo",
                              "learned", "sum", "dense-kron", "lower-coo", "write-ceiling",
-                             "hessian"],
+                             "hessian", "tune"],
                     help="btda: the sparse refill (default line); blocks-*: the stacked "
                          "permuted-dense products (A^T D A, or B^T D A of two stacks)")
     ap.add_argument("--rmat-scale", type=int, default=20,
@@ -125,6 +125,8 @@ def main():
         return main_lower_coo(args)
     if args.workload == "write-ceiling":
         return main_write_ceiling(args)
+    if args.workload == "tune":
+        return main_tune(args)
     if args.workload == "hessian":
         return main_hessian(args)
     if args.workload != "btda":
@@ -446,6 +448,113 @@ def main_hessian(args):
                         "eager_ms_per_step": ms_eager_s, "graph_ms_per_step": ms_graph_s,
                         "graph_vs_eager": ms_eager_s / ms_graph_s, "graph_equals_eager": same}
     Hs.close()
+    print(json.dumps(out), flush=True)
+
+
+def main_tune(args):
+    """Tuned vs built-in plan thresholds (the symbolic phase's largest LDS hash
+    table and bitmap density factor, the row-accumulator refill's row cuts) on
+    the A^T A of the rank-Zipf 2,000,000 x 500,000 matrix, of the banded 2^20
+    matrix (bandwidth 64) and of the R-MAT 2^20 graph.  Per product the tuner
+    runs once (sdl_btda_tune: plan creation over 20 symbolic candidates, then
+    refills over 9 row-cut candidates where values are given), then plans are
+    created afresh with the built-in and the recorded thresholds, interleaved
+    twice, and timed: plan creation (host clock around device syncs) plus one
+    refill for a fresh d (device events).  The R-MAT product (9.7e9 output
+    entries) is tuned and timed on plan creation alone."""
+    import sparsediff_linalg_amd as sdl
+    from sparsediff_linalg_amd import DeviceArray, check, tuning, workloads
+    from sparsediff_linalg_amd.predictor import Plan
+
+    lib = sdl.load()
+    stream = lib.sdl_library_stream()
+    e0, e1 = lib.sdl_event_create(), lib.sdl_event_create()
+
+    def measure(p, i, n, t, dAx, dd):
+        lib.sdl_synchronize()
+        t0 = time.perf_counter()
+        pl = Plan((p, i), None, n, n, tune=t)
+        lib.sdl_synchronize()
+        plan_ms = (time.perf_counter() - t0) * 1e3
+        nnz = pl.nnz()
+        rowptr = np.empty(n + 1, np.int64)
+        lib.sdl_memcpy_dtoh(rowptr.ctypes.data, lib.sdl_btda_plan_rowptr(pl.h), rowptr.nbytes)
+        ref_ms = 0.0
+        if dAx is not None:
+            Cx = DeviceArray(max(nnz, 1), np.float64)
+            check(lib.sdl_btda_refill(pl.h, dAx.ptr, dAx.ptr, dd.ptr, Cx.ptr, stream), "refill")
+            lib.sdl_event_record(e0, stream)
+            check(lib.sdl_btda_refill(pl.h, dAx.ptr, dAx.ptr, dd.ptr, Cx.ptr, stream), "refill")
+            lib.sdl_event_record(e1, stream)
+            lib.sdl_synchronize()
+            ref_ms = lib.sdl_event_elapsed_ms(e0, e1)
+            del Cx
+        schedule = "pull" if lib.sdl_btda_plan_is_pull(pl.h) else "row accumulators"
+        pl.close()
+        return plan_ms, ref_ms, nnz, hash(rowptr.tobytes()), schedule
+
+    def zipf():
+        p, i, x = workloads.zipf_csr(2, 2_000_000, 500_000)
+        return p, i, x, 500_000
+
+    def banded():
+        p, i, x = workloads.banded_csr(1 << 20, 64)
+        return p, i, x, 1 << 20
+
+    def rmat():
+        p, i, x = workloads.rmat_csr(args.rmat_scale)
+        return p, i, None, 1 << args.rmat_scale
+
+    products = [("A^T A, A 2,000,000 x 500,000 rank-Zipf(1.2) rows, mean 16", zipf),
+                ("A^T A, banded 2^20 x 2^20, bandwidth 64", banded),
+                (f"A^T A, R-MAT 2^{args.rmat_scale} nodes, 16 edges/node", rmat)]
+    results = []
+    tot_def = tot_tuned = 0.0
+    for label, make in products:
+        p, i, x, n = make()
+        m = len(p) - 1
+        dAx = DeviceArray.from_numpy(x) if x is not None else None
+        dd = DeviceArray.from_numpy(workloads.uniform(77, m, 0.1, 10.0))
+        tuning.clear()
+        Plan((p, i), None, n, n).close()  # warm-up: kernels, allocator
+        t0 = time.perf_counter()
+        best, t_best, t_def = tuning.tune((p, i), None, n, n,
+                                          values=(dAx, dAx) if dAx is not None else None,
+                                          reps=3)
+        tune_s = time.perf_counter() - t0
+        print(f"[tune] {label}: tuner {tune_s:.1f} s -> {best}", file=sys.stderr, flush=True)
+        default = tuning.default()
+        runs = {"built-in": [], "tuned": []}
+        for _ in range(2):
+            for key, t in (("built-in", default), ("tuned", best)):
+                runs[key].append(measure(p, i, n, t, dAx, dd))
+        d_best = min(runs["built-in"], key=lambda r: r[0] + r[1])
+        t_best_run = min(runs["tuned"], key=lambda r: r[0] + r[1])
+        assert d_best[2] == t_best_run[2] and d_best[3] == t_best_run[3], label
+        td, tt = d_best[0] + d_best[1], t_best_run[0] + t_best_run[1]
+        rec = {"product": label, "nnz_out": d_best[2], "schedule": d_best[4],
+               "tuned": best, "tuner_s": tune_s,
+               "tuner_estimate_ms": {"tuned": t_best, "built_in": t_def},
+               "built_in_ms": {"plan": d_best[0], "refill": d_best[1], "total": td},
+               "tuned_ms": {"plan": t_best_run[0], "refill": t_best_run[1], "total": tt},
+               "gain": td / tt, "structure_same": True}
+        tot_def += td
+        tot_tuned += tt
+        results.append(rec)
+        print(f"[tune] {json.dumps(rec)}", file=sys.stderr, flush=True)
+        del dAx, dd
+    tuning.clear()
+    out = {
+        "metric": "plan + refill time, built-in / tuned thresholds (tuning table)",
+        "value": tot_def / tot_tuned, "unit": "x", "n_gpus": 1, "steps": 2, "warmup": 1,
+        "ms_per_step": tot_tuned, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "int32+f64", "data": "synthetic",
+        "config": {"workload": ("plan creation + one refill of three A^T A products "
+                                "(rank-Zipf 2M x 500k, banded 2^20 bw 64, R-MAT plan only), "
+                                "tuned vs built-in thresholds, best of 2 interleaved"),
+                   "products": results},
+        "roofline": None, "cpu_baseline": None,
+    }
     print(json.dumps(out), flush=True)
 
 

@@ -80,7 +80,12 @@ int sdl_btda_plan_structure(const sdl_btda_plan *plan, int *hp, int *hi);
    pass sizes each row from min(terms, column span), or from its prediction). */
 int sdl_btda_plan_stats(const sdl_btda_plan *plan, long long *out11);
 
-/* One refill: dCx[nnz] = B^T diag(dd) A with dAx/dBx the CSR values of A and
+/* Refills of one plan share the plan's workspace (operand pairs, transposed
+   d, dense accumulators, the rank kernel's spill list): issue all refills of
+   a plan on one stream, or serialise them; two refills of the same plan in
+   flight on different streams race.  Different plans are independent.
+
+   One refill: dCx[nnz] = B^T diag(dd) A with dAx/dBx the CSR values of A and
    B (dd may be NULL = identity). */
 int sdl_btda_refill(const sdl_btda_plan *plan, const double *dAx, const double *dBx,
                     const double *dd, double *dCx, void *stream);
@@ -154,12 +159,62 @@ int sdl_btda_refill_batched_cfg(const sdl_btda_plan *plan, const double *dAx,
 int sdl_refill_candidates(sdl_refill_cfg *out, int max);
 /* Autotuner: times every candidate on this plan and these buffers (reps timed
    launches each, after 2 warm-ups), keeps the fastest in the plan for later
-   sdl_btda_refill_batched calls, and reports it. */
+   sdl_btda_refill_batched calls, records it in the tuning table under the
+   plan's shape signature (below), and reports it. */
 int sdl_btda_autotune(sdl_btda_plan *plan, const double *dAx, const double *dBx,
                       const double *dD, long long ldd, int K, double *dC, long long ldc,
                       int reps, sdl_refill_cfg *best, float *best_us);
 /* Pin a configuration (NULL: back to the default heuristic). */
 int sdl_btda_set_refill_cfg(sdl_btda_plan *plan, const sdl_refill_cfg *cfg);
+
+/* ---- Tuning table (no counterpart in the reference, whose symbolic and
+   numeric phases are one fixed CPU algorithm, src/sparse/gustavson.c:20-150)
+   Every plan creation looks up (this GPU's architecture, the product's shape
+   signature) in a per-process table; an entry supplies the thresholds below
+   and, when recorded, the batched refill's launch shape.  The signature is
+   floor(log2) of the output rows, the columns and the mean terms per output
+   row.  The thresholds only choose between kernels: structures and values
+   are identical under every valid setting. */
+typedef struct sdl_sym_tune
+{
+    int hash_max;   /* largest LDS hash table, slots: 1024, 2048, 4096 or 8192 */
+    int dens;       /* a row of bound u takes the LDS bitmap when u*dens >= its
+                       words (1 .. 64) */
+    int rowacc_cut; /* longest row of the LDS row-accumulator refill (64 .. 1024) */
+    int rank_cut;   /* longest row of the by-rank refill (rowacc_cut < . <= 65536) */
+} sdl_sym_tune;
+/* The built-in thresholds (8192, 8, 1024, 65536). */
+int sdl_sym_tune_default(sdl_sym_tune *out);
+/* Plan creation with forced thresholds (NULL: as sdl_btda_plan_create, i.e.
+   the table's entry or the built-in values); out-of-range values are an error. */
+int sdl_btda_plan_create_tuned(int m, int nA, int nB, const int *dAp, const int *dAi,
+                               const int *dBp, const int *dBi, int mode,
+                               const sdl_sym_tune *tune, sdl_btda_plan **out);
+/* The thresholds a plan was created with. */
+int sdl_btda_plan_tune(const sdl_btda_plan *plan, sdl_sym_tune *out);
+/* Tunes the thresholds on this product: times plan creation (median of reps)
+   over hash_max x dens, then -- with dAx and dBx given (both or neither) --
+   one refill per (rowacc_cut, rank_cut) on a row-accumulator plan (reps
+   refills each); keeps the built-in values unless a candidate's plan time
+   plus refill time is lower.  Records the winner in the table under the
+   product's signature (keeping a recorded refill shape), reports it and the
+   two totals, in ms.  Fails if a candidate's output structure differs. */
+int sdl_btda_tune(int m, int nA, int nB, const int *dAp, const int *dAi, const int *dBp,
+                  const int *dBi, int mode, const double *dAx, const double *dBx, int reps,
+                  sdl_sym_tune *best, float *best_ms, float *default_ms);
+/* Records a batched refill launch shape (one of sdl_refill_candidates) for
+   the plan's signature. */
+int sdl_btda_tune_record_refill(const sdl_btda_plan *plan, const sdl_refill_cfg *cfg);
+/* Entries in the table; drop them all. */
+int sdl_tune_table_size(void);
+void sdl_tune_table_clear(void);
+/* Save the table as text closed by an FNV-1a-64 checksum line; load merges a
+   saved table over the current one (the file's entries win).  A file with a
+   wrong header or checksum, or any out-of-range entry, is rejected whole and
+   leaves the table unchanged.  Plans created before a load keep their
+   thresholds. */
+int sdl_tune_table_save(const char *path);
+int sdl_tune_table_load(const char *path);
 
 /* ---- COO coalescing on device-resident triplets ----
    The reference has no direct COO -> CSR entry; its coalescing semantics are

@@ -51,6 +51,14 @@ class sdl_refill_cfg(C.Structure):
     _fields_ = [("variant", C.c_int), ("rchunk", C.c_int)]
 
 
+class sdl_sym_tune(C.Structure):
+    _fields_ = [("hash_max", C.c_int), ("dens", C.c_int), ("rowacc_cut", C.c_int),
+                ("rank_cut", C.c_int)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 _to_csr_fn = C.CFUNCTYPE(C.POINTER(CSR_matrix), C.POINTER(matrix))
 _gather_alloc_fn = C.CFUNCTYPE(C.POINTER(matrix), C.POINTER(matrix), c_int_p, C.c_int)
 _gather_fill_fn = C.CFUNCTYPE(None, C.POINTER(matrix), C.POINTER(matrix))
@@ -111,6 +119,18 @@ SIGNATURES = {
     "sdl_btda_autotune": (C.c_int, [vp, vp, vp, vp, C.c_longlong, C.c_int, vp, C.c_longlong,
                                     C.c_int, C.POINTER(sdl_refill_cfg), C.POINTER(C.c_float)]),
     "sdl_btda_set_refill_cfg": (C.c_int, [vp, C.POINTER(sdl_refill_cfg)]),
+    "sdl_sym_tune_default": (C.c_int, [C.POINTER(sdl_sym_tune)]),
+    "sdl_btda_plan_create_tuned": (C.c_int, [C.c_int, C.c_int, C.c_int, vp, vp, vp, vp, C.c_int,
+                                             C.POINTER(sdl_sym_tune), C.POINTER(vp)]),
+    "sdl_btda_plan_tune": (C.c_int, [vp, C.POINTER(sdl_sym_tune)]),
+    "sdl_btda_tune": (C.c_int, [C.c_int, C.c_int, C.c_int, vp, vp, vp, vp, C.c_int, vp, vp,
+                                C.c_int, C.POINTER(sdl_sym_tune), C.POINTER(C.c_float),
+                                C.POINTER(C.c_float)]),
+    "sdl_btda_tune_record_refill": (C.c_int, [vp, C.POINTER(sdl_refill_cfg)]),
+    "sdl_tune_table_size": (C.c_int, []),
+    "sdl_tune_table_clear": (None, []),
+    "sdl_tune_table_save": (C.c_int, [C.c_char_p]),
+    "sdl_tune_table_load": (C.c_int, [C.c_char_p]),
     "sdl_btda_plan_reserve": (C.c_int, [vp, C.c_int]),
     "sdl_btda_bind_values": (C.c_int, [vp, vp, vp, vp]),
     "sdl_btda_refill_batched_ex": (C.c_int, [vp, vp, vp, vp, C.c_longlong, C.c_int, vp,

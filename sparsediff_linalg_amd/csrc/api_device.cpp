@@ -236,6 +236,11 @@ int sdl_btda_autotune(sdl_btda_plan *plan, const double *dAx, const double *dBx,
     SDL_HIP(hipEventDestroy(e0));
     SDL_HIP(hipEventDestroy(e1));
     plan->P.tuned = best_c;
+    {
+        sdl::SymTune keep = plan->P.tune;
+        sdl::tune_lookup(plan->P.sig, &keep, nullptr);
+        sdl::tune_record(plan->P.sig, &keep, &best_c);
+    }
     if (best)
     {
         best->variant = best_c.variant;

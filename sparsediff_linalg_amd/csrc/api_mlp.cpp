@@ -15,18 +15,6 @@ struct sdl_mlp
 };
 
 namespace {
-// FNV-1a over the parameter bytes: the checksum of a saved predictor
-uint64_t fnv1a(const void *p, size_t n)
-{
-    const unsigned char *b = (const unsigned char *) p;
-    uint64_t h = 1469598103934665603ull;
-    for (size_t i = 0; i < n; i++)
-    {
-        h ^= b[i];
-        h *= 1099511628211ull;
-    }
-    return h;
-}
 const char MAGIC[8] = {'S', 'D', 'L', 'M', 'L', 'P', '1', '\0'};
 } // namespace
 
@@ -136,7 +124,7 @@ int sdl_mlp_save(const sdl_mlp *p, const char *path)
     std::vector<float> h(sdl::MLP_NPARAM);
     sdl::sync();
     sdl::d2h(h.data(), p->R.params.p, h.size() * sizeof(float));
-    uint64_t sum = fnv1a(h.data(), h.size() * sizeof(float));
+    uint64_t sum = sdl::fnv1a64(h.data(), h.size() * sizeof(float));
     FILE *f = std::fopen(path, "wb");
     if (!f)
     {
@@ -171,7 +159,7 @@ int sdl_mlp_load(sdl_mlp *p, const char *path)
               std::fread(&tau, 4, 1, f) == 1 && std::fread(h.data(), 4, h.size(), f) == h.size() &&
               std::fread(&sum, 8, 1, f) == 1;
     std::fclose(f);
-    if (!ok || sum != fnv1a(h.data(), h.size() * sizeof(float)) || !(tau > 0.0f && tau <= 1.0f))
+    if (!ok || sum != sdl::fnv1a64(h.data(), h.size() * sizeof(float)) || !(tau > 0.0f && tau <= 1.0f))
     {
         sdl::set_error("sdl_mlp_load: %s is not a valid predictor file (format or checksum)", path);
         return 1;

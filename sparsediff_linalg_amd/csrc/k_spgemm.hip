@@ -19,6 +19,7 @@
 #include "mlp.h"
 
 #include <algorithm>
+#include <cmath>
 #include <climits>
 #include <vector>
 
@@ -367,12 +368,18 @@ __global__ void __launch_bounds__(256)
     if (bound) bound[row] = t > 0 ? min(t, (int64_t) cmax - cmin + 1) : 0;
 }
 
-__device__ __forceinline__ int classify(int64_t cap, int nwords, int lds_words)
+// Accumulator class of a row whose distinct count is at most cap: a hash
+// table of the smallest power of two >= 2 cap slots (64 .. hash_max), or the
+// LDS column bitmap when the row is dense enough (cap * dens >= nwords) or its
+// table would pass hash_max, or a bitmap in global memory when the bitmap
+// does not fit LDS either.  hash_max and dens are tuning parameters (SymTune).
+__device__ __forceinline__ int classify(int64_t cap, int nwords, int lds_words,
+                                        int hash_max = HASH_MAX, int dens = 8)
 {
     if (cap <= 0) return 0;
     bool bm_ok = nwords <= lds_words;
-    if (bm_ok && (cap * 8 >= nwords || 2 * cap > HASH_MAX)) return CLS_BITMAP;
-    if (2 * cap <= HASH_MAX)
+    if (bm_ok && (cap * dens >= nwords || 2 * cap > hash_max)) return CLS_BITMAP;
+    if (2 * cap <= hash_max)
     {
         int64_t s = 64;
         int c = 1;
@@ -386,12 +393,12 @@ __device__ __forceinline__ int classify(int64_t cap, int nwords, int lds_words)
     return CLS_GLOBAL;
 }
 
-// mode 0: symbolic accumulator classes; mode 1: 0 empty, 1 <= t1, 2 <= t2, 3 larger;
-// mode 2: cap already holds the class
+// mode 0: symbolic accumulator classes (classify with hash_max / dens);
+// mode 1: 0 empty, 1 <= t1, 2 <= t2, 3 larger
 template <typename T>
 __global__ void k_classify(int64_t m, const T *__restrict__ cap, int mode, int nwords,
-                           int lds_words, int t1, int t2, unsigned char *__restrict__ cls,
-                           int *__restrict__ counts)
+                           int lds_words, int hash_max, int dens, int t1, int t2,
+                           unsigned char *__restrict__ cls, int *__restrict__ counts)
 {
     __shared__ int hist[NCLASS];
     if (threadIdx.x < NCLASS) hist[threadIdx.x] = 0;
@@ -400,9 +407,8 @@ __global__ void k_classify(int64_t m, const T *__restrict__ cap, int mode, int n
     if (row < m)
     {
         int64_t v = (int64_t) cap[row];
-        int c = mode == 0 ? classify(v, nwords, lds_words)
-                : mode == 1 ? (v <= 0 ? 0 : (v <= t1 ? 1 : (v <= t2 ? 2 : 3)))
-                            : (int) v;  // mode 2: the value is the class
+        int c = mode == 0 ? classify(v, nwords, lds_words, hash_max, dens)
+                          : (v <= 0 ? 0 : (v <= t1 ? 1 : (v <= t2 ? 2 : 3)));
         cls[row] = (unsigned char) c;
         atomicAdd(&hist[c], 1);
     }
@@ -638,9 +644,10 @@ constexpr unsigned char PBIN_NONE = 255;
 // and scanned (2S) and takes t probing inserts (2t); an LDS column bitmap of
 // nwords words costs 2 nwords + t; a bitmap in global memory the same through
 // L2 atomics, eight times slower.
-__device__ __forceinline__ double count_cost(int64_t cap, int64_t t, int nwords, int lds_words)
+__device__ __forceinline__ double count_cost(int64_t cap, int64_t t, int nwords, int lds_words,
+                                             int hash_max, int dens)
 {
-    const int c = classify(cap, nwords, lds_words);
+    const int c = classify(cap, nwords, lds_words, hash_max, dens);
     if (c == 0) return 0.0;
     // a fixed part per row: the workgroup's start and the row's setup
     if (c < CLS_BITMAP) return 512.0 + 2.0 * (double) (64 << (c - 1)) + 2.0 * (double) t;
@@ -655,7 +662,7 @@ __device__ __forceinline__ double count_cost(int64_t cap, int64_t t, int nwords,
 __global__ void __launch_bounds__(256)
     k_pred_candidates(int64_t m, const int64_t *__restrict__ terms,
                       const int64_t *__restrict__ bound, int nwords, int lds_words,
-                      int *__restrict__ list, int *__restrict__ count)
+                      int hash_max, int dens, int *__restrict__ list, int *__restrict__ count)
 {
     const int64_t row = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
     bool cand = false;
@@ -663,8 +670,8 @@ __global__ void __launch_bounds__(256)
     {
         const int64_t t = terms[row], u = bound[row];
         if (u > mlp_bin_max(0))
-            cand = count_cost(mlp_bin_max(0), t, nwords, lds_words) <
-                   0.8 * count_cost(u, t, nwords, lds_words);
+            cand = count_cost(mlp_bin_max(0), t, nwords, lds_words, hash_max, dens) <
+                   0.8 * count_cost(u, t, nwords, lds_words, hash_max, dens);
     }
     const unsigned long long v = __ballot(cand);
     const int lane = threadIdx.x & 63;
@@ -687,7 +694,8 @@ __global__ void k_pred_cap(int64_t n, const int *__restrict__ list,
                            const int64_t *__restrict__ terms, const int64_t *__restrict__ ubound,
                            const float *__restrict__ F,
                            const float *__restrict__ Z, float tau, int nwords, int lds_words,
-                           int64_t *__restrict__ cap, unsigned char *__restrict__ pbin)
+                           int hash_max, int dens, int64_t *__restrict__ cap,
+                           unsigned char *__restrict__ pbin)
 {
     // F[i] / Z[i] hold the features / logits of output row list[i]
     const int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
@@ -721,7 +729,7 @@ __global__ void k_pred_cap(int64_t n, const int *__restrict__ list,
     int64_t c_out = u;
     if (u > 0)
     {
-        const double e0 = count_cost(u, t, nwords, lds_words);
+        const double e0 = count_cost(u, t, nwords, lds_words, hash_max, dens);
         double best = 0.8 * e0;
         double below = 0.0;  // P(nnz <= bound_b)
         for (int b = 0; b < MLP_NBIN_BOUNDED; b++)
@@ -730,7 +738,7 @@ __global__ void k_pred_cap(int64_t n, const int *__restrict__ list,
             const int64_t bound = mlp_bin_max(b);
             if (bound >= u) break;  // the exact bound is already at most this bin's
             if ((double) bound < longest) continue;
-            const double eb = count_cost(bound, t, nwords, lds_words) +
+            const double eb = count_cost(bound, t, nwords, lds_words, hash_max, dens) +
                               fmax(0.0, 1.0 - below) * e0;
             if (eb < best)
             {
@@ -1811,7 +1819,8 @@ struct Bins
 };
 
 template <typename T>
-void bin_rows(Bins &B, const T *cap, int64_t m, int nwords, int mode = 0, int t1 = 0, int t2 = 0)
+void bin_rows(Bins &B, const T *cap, int64_t m, int nwords, const SymTune &tune, int mode = 0,
+              int t1 = 0, int t2 = 0)
 {
     DBuf<unsigned char> cls(m);
     DBuf<int> counts(NCLASS);
@@ -1819,7 +1828,8 @@ void bin_rows(Bins &B, const T *cap, int64_t m, int nwords, int mode = 0, int t1
     unsigned grid = (unsigned) ((m + 255) / 256);
     if (m > 0)
         hipLaunchKernelGGL(k_classify<T>, dim3(grid), dim3(256), 0, stream(), m, cap, mode,
-                           nwords, lds_bitmap_words(), t1, t2, cls.p, counts.p);
+                           nwords, lds_bitmap_words(), tune.hash_max, tune.dens, t1, t2, cls.p,
+                           counts.p);
     SDL_HIP(hipGetLastError());
     d2h(B.count.data(), counts.p, NCLASS * sizeof(int));
     B.off[0] = 0;
@@ -1896,7 +1906,7 @@ void run_symbolic_class(int c, const Bins &B, const ProductPlan &P, int nwords, 
 void symbolic_count(ProductPlan &P, const int64_t *cap, int *rowcnt, int nwords, bool first)
 {
     Bins B;
-    bin_rows<int64_t>(B, cap, P.m_out, nwords);
+    bin_rows<int64_t>(B, cap, P.m_out, nwords, P.tune);
     for (int c = 1; c < NCLASS; c++) run_symbolic_class<0>(c, B, P, nwords, rowcnt, nullptr, nullptr);
     if (first)
     {
@@ -1917,7 +1927,7 @@ void symbolic_fill(ProductPlan &P, const int *rowcnt, int nwords)
     P.stats.nnz = P.nnz;
     P.Ci.alloc(P.nnz);
     Bins B2;
-    bin_rows<int>(B2, rowcnt, m_out, nwords);
+    bin_rows<int>(B2, rowcnt, m_out, nwords, P.tune);
     for (int c = 1; c < NCLASS; c++)
     {
         run_symbolic_class<1>(c, B2, P, nwords, nullptr, P.Cp.p, P.Ci.p);
@@ -1957,7 +1967,8 @@ void symbolic_predicted(ProductPlan &P, const RowPredictor *pred, const int64_t 
         DBuf<int> list(m_out), ncand(1);
         dzero(ncand.p, sizeof(int));
         hipLaunchKernelGGL(k_pred_candidates, dim3(grid), dim3(256), 0, stream(), m_out, rterms,
-                           rbound, nwords, lds_bitmap_words(), list.p, ncand.p);
+                           rbound, nwords, lds_bitmap_words(), P.tune.hash_max, P.tune.dens,
+                           list.p, ncand.p);
         SDL_HIP(hipGetLastError());
         int nc = 0;
         d2h(&nc, ncand.p, sizeof(int));
@@ -1972,7 +1983,8 @@ void symbolic_predicted(ProductPlan &P, const RowPredictor *pred, const int64_t 
             mlp_forward(*pred, F.p, nc, Z.p, stream());
             hipLaunchKernelGGL(k_pred_cap, dim3((unsigned) ((nc + 255) / 256)), dim3(256), 0,
                                stream(), (int64_t) nc, list.p, rterms, rbound, F.p, Z.p,
-                               pred->tau, nwords, lds_bitmap_words(), cap.p, pbin.p);
+                               pred->tau, nwords, lds_bitmap_words(), P.tune.hash_max,
+                               P.tune.dens, cap.p, pbin.p);
             SDL_HIP(hipGetLastError());
         }
     }
@@ -2067,6 +2079,23 @@ void product_plan(ProductPlan &P, const LView &L, const AView &A, SymMode mode,
     P.terms = scan_i64(rterms.p, toff.p, m_out, true);
     P.stats.terms = P.terms;
 
+    // thresholds: forced (the tuner), the tuning table's for this shape, or
+    // the defaults
+    {
+        auto lg = [](double v) { return v >= 1.0 ? (int) std::floor(std::log2(v)) : 0; };
+        P.sig.lg_rows = lg((double) m_out);
+        P.sig.lg_cols = lg((double) A.An);
+        P.sig.lg_tpr = lg((double) P.terms / (double) m_out);
+        RefillCfg rc;
+        SymTune t;
+        if (g_force_tune) P.tune = *g_force_tune;
+        else if (tune_lookup(P.sig, &t, &rc))
+        {
+            P.tune = t;
+            if (rc.variant >= 0) P.tuned = rc;
+        }
+    }
+
     // structure: count pass sized from the term bound or the predicted bins, fill pass
     DBuf<int> rowcnt(m_out);
     dzero(rowcnt.p, rowcnt.bytes());
@@ -2087,7 +2116,7 @@ void product_plan(ProductPlan &P, const LView &L, const AView &A, SymMode mode,
         DBuf<int> err(1);
         dzero(err.p, err.bytes());
         Bins BT;
-        bin_rows<int>(BT, rowcnt.p, m_out, nwords, 1, 128, 2048);
+        bin_rows<int>(BT, rowcnt.p, m_out, nwords, P.tune, 1, 128, 2048);
         DBuf<int> gscratch;
         for (int c = 1; c <= 3; c++)
         {
@@ -2120,11 +2149,13 @@ void product_plan(ProductPlan &P, const LView &L, const AView &A, SymMode mode,
     }
     else
     {
-        // row-accumulator bins: rows with <= 1024 entries accumulate in LDS; rows
-        // up to RANK_MAX_ENTRIES entries by rank when the column bitmap fits
-        // LDS; longer rows in dense accumulators
+        // row-accumulator bins: rows with <= rowacc_cut (<= 1024) entries
+        // accumulate in LDS; rows up to rank_cut (<= RANK_MAX_ENTRIES) entries
+        // by rank when the column bitmap fits LDS; longer rows in dense
+        // accumulators
         Bins BR;
-        bin_rows<int>(BR, rowcnt.p, m_out, nwords, 1, 1024, RANK_MAX_ENTRIES);
+        bin_rows<int>(BR, rowcnt.p, m_out, nwords, P.tune, 1, P.tune.rowacc_cut,
+                      std::min(P.tune.rank_cut, RANK_MAX_ENTRIES));
         const int small = BR.count[1], mid = BR.count[2], large = BR.count[3];
         P.rbins.alloc(std::max(1, small + mid + large));
         P.rbin_off[0] = 0;

@@ -30,6 +30,20 @@ void set_error(const char *fmt, ...);
         if (!(cond)) ::sdl::fatal(msg, __FILE__, __LINE__);                        \
     } while (0)
 
+// FNV-1a, 64-bit (offset basis 0xcbf29ce484222325, prime 0x100000001b3): the
+// checksum closing the library's saved files (predictor, tuning table).
+inline uint64_t fnv1a64(const void *p, size_t n)
+{
+    const unsigned char *b = (const unsigned char *) p;
+    uint64_t h = 0xcbf29ce484222325ull;
+    for (size_t i = 0; i < n; i++)
+    {
+        h ^= b[i];
+        h *= 0x100000001b3ull;
+    }
+    return h;
+}
+
 // The library's own stream (default priority, non-blocking w.r.t. stream 0).
 hipStream_t stream();
 void sync();

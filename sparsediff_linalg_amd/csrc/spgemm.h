@@ -63,6 +63,38 @@ struct RefillCfg
     RefillCfg(int v, int r) : variant(v), rchunk(r) {}
 };
 
+// Tuning parameters of the symbolic phase and the row-accumulator refill:
+// the largest LDS hash table (slots; 1024 .. 8192), the density factor of
+// the hash / bitmap choice (a row of bound u takes the LDS column bitmap when
+// u * dens >= the bitmap's words), the longest row (entries) refilled by the
+// LDS row accumulator before the by-rank kernel takes over (<= 1024), and the
+// longest row refilled by rank before the dense accumulator takes over
+// (<= 65536).
+struct SymTune
+{
+    int hash_max = 8192;
+    int dens = 8;
+    int rowacc_cut = 1024;
+    int rank_cut = 65536;
+};
+// true when every field is in the range above (hash_max a power of two)
+bool tune_valid(const SymTune &t);
+
+// Shape signature of a product for the tuning table: floor(log2) of the
+// output rows, the columns and the mean terms per output row.
+struct ShapeSig
+{
+    int lg_rows = 0, lg_cols = 0, lg_tpr = 0;
+};
+// The tuning table (per process; saved to / loaded from a checksummed file
+// by the sdl_tune_table_* entry points): thresholds and refill launch shape
+// recorded for (GPU architecture, shape signature).  lookup returns false
+// when there is no entry (the defaults apply).
+bool tune_lookup(const ShapeSig &sig, SymTune *tune, RefillCfg *refill);
+void tune_record(const ShapeSig &sig, const SymTune *tune, const RefillCfg *refill);
+// Thresholds forced on the plans this thread creates (the tuner's candidates).
+extern thread_local const SymTune *g_force_tune;
+
 struct ProductPlan
 {
     LView L;
@@ -90,6 +122,8 @@ struct ProductPlan
     DBuf<double> acc;
     SymStats stats;
     RefillCfg tuned;   // variant < 0: not tuned, use the default heuristic
+    SymTune tune;      // thresholds the symbolic phase and the refill bins used
+    ShapeSig sig;
     // Batched-refill workspace (reserved in the alloc phase): term operand pairs
     // in term order, and D transposed to k-major with row stride batch_ld.
     DBuf<double> tv, dT;

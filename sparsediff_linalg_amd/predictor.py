@@ -31,7 +31,7 @@ NIN = 16
 class Plan:
     """A device B^T diag(d) A plan (A^T A when B is A)."""
 
-    def __init__(self, A, B=None, nA=None, nB=None, mode=0, mlp=None):
+    def __init__(self, A, B=None, nA=None, nB=None, mode=0, mlp=None, tune=None):
         lib = load()
         Ap, Ai = A[0], A[1]
         m = len(Ap) - 1
@@ -42,7 +42,12 @@ class Plan:
                        for a in (Ap, Ai, Bp, Bi)]
         dAp, dAi, dBp, dBi = (a.ptr for a in self.arrays)
         h = C.c_void_p()
-        if mlp is not None or mode == 1:
+        if tune is not None:
+            from .tuning import sym_tune
+            check(lib.sdl_btda_plan_create_tuned(m, self.nA, self.nB, dAp, dAi, dBp, dBi, mode,
+                                                 C.byref(sym_tune(tune)), C.byref(h)),
+                  "tuned plan")
+        elif mlp is not None or mode == 1:
             check(lib.sdl_btda_plan_create_predicted(m, self.nA, self.nB, dAp, dAi, dBp, dBi,
                                                      mlp.handle if mlp else None, C.byref(h)),
                   "predicted plan")
@@ -76,6 +81,13 @@ class Plan:
         Y = DeviceArray(self.nB, np.int32)
         check(self.lib.sdl_btda_plan_row_bins(self.h, Y.ptr), "bins")
         return Y
+
+    def tune(self):
+        """The symbolic / refill thresholds this plan was created with."""
+        from ._lib import sdl_sym_tune
+        t = sdl_sym_tune()
+        check(self.lib.sdl_btda_plan_tune(self.h, C.byref(t)), "plan tune")
+        return t.as_dict()
 
     def pred_stats(self):
         o = (C.c_longlong * 5)()
