@@ -58,7 +58,13 @@ enum
     /* or-ed into mode: build the pull schedule (per-entry term lists) whatever
        the terms / nnz ratio, as the batched, captured and range refills need
        it (the reference builds it iff terms <= 3 nnz; values are the same) */
-    SDL_PLAN_PULL = 256
+    SDL_PLAN_PULL = 256,
+    /* or-ed into mode: the by-rank rows (above the row-accumulator cut, 1024
+       entries by default) find each term's entry by ranking the row's columns
+       at every refill, instead of the plan recording every term's position
+       (2 bytes per term; recorded when these rows' terms are at most 1.25x
+       their entries).  Values are the same. */
+    SDL_PLAN_RANK_REFILL = 512
 };
 
 /* A: m x nA CSR (dAp[m+1], dAi[nnzA]); B: m x nB CSR.  The plan copies the
@@ -79,6 +85,11 @@ int sdl_btda_plan_structure(const sdl_btda_plan *plan, int *hp, int *hi);
    the first count pass's hash / LDS-bitmap / global-bitmap rows (the count
    pass sizes each row from min(terms, column span), or from its prediction). */
 int sdl_btda_plan_stats(const sdl_btda_plan *plan, long long *out11);
+/* Refill schedule (5 values): pull schedule (0/1), by-rank rows (row-
+   accumulator plans), recorded positions for them (0/1), then with recorded
+   positions the listed terms of entries of several terms and the rows left
+   to the dense accumulator. */
+int sdl_btda_plan_schedule(const sdl_btda_plan *plan, long long *out5);
 
 /* Refills of one plan share the plan's workspace (operand pairs, transposed
    d, dense accumulators, the rank kernel's spill list): issue all refills of
@@ -192,13 +203,14 @@ int sdl_btda_plan_create_tuned(int m, int nA, int nB, const int *dAp, const int 
                                const sdl_sym_tune *tune, sdl_btda_plan **out);
 /* The thresholds a plan was created with. */
 int sdl_btda_plan_tune(const sdl_btda_plan *plan, sdl_sym_tune *out);
-/* Tunes the thresholds on this product: times plan creation (median of reps)
-   over hash_max x dens, then -- with dAx and dBx given (both or neither) --
-   one refill per (rowacc_cut, rank_cut) on a row-accumulator plan (reps
-   refills each); keeps the built-in values unless a candidate's plan time
-   plus refill time is lower.  Records the winner in the table under the
-   product's signature (keeping a recorded refill shape), reports it and the
-   two totals, in ms.  Fails if a candidate's output structure differs. */
+/* Tunes the thresholds on this product.  A setting's cost is plan creation
+   plus, with dAx and dBx given (both or neither), one refill; median of reps.
+   Stage 1 sweeps hash_max x dens, stage 2 (row-accumulator plans, values
+   given) rowacc_cut x rank_cut from stage 1's winner; a candidate replaces
+   the incumbent (first the built-in values) only when at least 3 % cheaper.
+   Records the winner in the table under the product's signature (keeping a
+   recorded refill shape), reports it and the two costs, in ms.  Fails if a
+   candidate's output structure differs. */
 int sdl_btda_tune(int m, int nA, int nB, const int *dAp, const int *dAi, const int *dBp,
                   const int *dBi, int mode, const double *dAx, const double *dBx, int reps,
                   sdl_sym_tune *best, float *best_ms, float *default_ms);

@@ -110,6 +110,7 @@ SIGNATURES = {
     "sdl_btda_plan_colind": (vp, [vp]),
     "sdl_btda_plan_structure": (C.c_int, [vp, c_int_p, c_int_p]),
     "sdl_btda_plan_stats": (C.c_int, [vp, c_ll_p]),
+    "sdl_btda_plan_schedule": (C.c_int, [vp, c_ll_p]),
     "sdl_btda_refill": (C.c_int, [vp, vp, vp, vp, vp, vp]),
     "sdl_btda_refill_batched": (C.c_int, [vp, vp, vp, vp, C.c_longlong, C.c_int, vp,
                                           C.c_longlong, vp]),

@@ -32,7 +32,8 @@ int sdl_btda_plan_create(int m, int nA, int nB, const int *dAp, const int *dAi,
     return sdl_btda_plan_create_mode(m, nA, nB, dAp, dAi, dBp, dBi,
                                      sym == SDL_SYM_PREDICTED ? sdl::SymMode::Predicted
                                                               : sdl::SymMode::Exact,
-                                     sdl::g_row_predictor, out, (mode & SDL_PLAN_PULL) != 0);
+                                     sdl::g_row_predictor, out, (mode & SDL_PLAN_PULL) != 0,
+                                     (mode & SDL_PLAN_RANK_REFILL) != 0);
 }
 #pragma GCC visibility pop
 } // extern "C"
@@ -40,7 +41,7 @@ int sdl_btda_plan_create(int m, int nA, int nB, const int *dAp, const int *dAi,
 int sdl_btda_plan_create_mode(int m, int nA, int nB, const int *dAp, const int *dAi,
                               const int *dBp, const int *dBi, sdl::SymMode mode,
                               const sdl::RowPredictor *pred, sdl_btda_plan **out,
-                              bool force_pull)
+                              bool force_pull, bool rank_refill)
 {
     if (!out || m < 0 || nA < 0 || nB < 0 || !dAp || !dBp)
     {
@@ -76,6 +77,7 @@ int sdl_btda_plan_create_mode(int m, int nA, int nB, const int *dAp, const int *
     A.An = nA;
     A.Ap = pl->Ap.p;
     A.Ai = pl->Ai.p;
+    pl->P.rank_refill = rank_refill;
     sdl::product_plan(pl->P, L, A, mode, pred, true, force_pull);
     *out = pl;
     return 0;
@@ -302,6 +304,20 @@ int sdl_btda_plan_reserve(sdl_btda_plan *plan, int Kmax)
     if (!plan || Kmax < 0) return 1;
     sdl::product_reserve_batch(plan->P, Kmax);
     sdl::sync();
+    return 0;
+}
+
+int sdl_btda_plan_schedule(const sdl_btda_plan *plan, long long *out5)
+{
+    if (!plan || !out5) return 1;
+    const sdl::ProductPlan &P = plan->P;
+    out5[0] = P.pull ? 1 : 0;
+    out5[1] = P.pull ? 0 : P.rbin_off[2] - P.rbin_off[1];
+    out5[2] = P.rankpos ? 1 : 0;
+    out5[3] = P.rankpos ? P.nmulti : 0;
+    int ns = 0;
+    if (P.rankpos) sdl::d2h(&ns, P.nspill.p, sizeof ns);
+    out5[4] = ns;
     return 0;
 }
 

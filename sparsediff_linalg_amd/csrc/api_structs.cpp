@@ -118,6 +118,7 @@ namespace {
 struct TransposeState : Attached
 {
     int src_m = 0, src_n = 0, nnz = 0;
+    uint64_t fp = 0;
     DBuf<int> perm;
     DBuf<double> xin, xout;
 };
@@ -125,6 +126,7 @@ struct TransposeState : Attached
 struct ProductState : Attached
 {
     int Am = 0, An = 0, nnzA = 0, Lm = 0, nnzL = 0;
+    uint64_t fpA = 0, fpL = 0;
     DBuf<int> Ap, Ai, Lp, Li;
     ProductPlan P;
     DBuf<double> Ax, Lx, d, Cx;
@@ -133,6 +135,7 @@ struct ProductState : Attached
 struct CoalesceState : Attached
 {
     int64_t nnz_in = 0;
+    uint64_t fp = 0;
     CoalescePlan P;
     DBuf<double> x, Cx;
 };
@@ -143,6 +146,35 @@ struct LowerCOOState : Attached
     DBuf<int> vmap;
     DBuf<double> vals, x;
 };
+
+// Fingerprint of a compressed structure, taken at alloc and checked by every
+// fill: the two dimensions, all np pointers and up to 4096 evenly spaced
+// entries of each index array, mixed as 32-bit words (FNV-1a over words).  A
+// fill handed another structure of the same nnz exits rather than gathering
+// through the alloc's permutation; O(np) per fill, so a change confined to
+// unsampled indices of an unchanged pointer array goes unseen.
+uint64_t fingerprint(int d0, int d1, int np, const int *p, int64_t nnz, const int *i,
+                     const int *j = nullptr)
+{
+    uint64_t h = 0xcbf29ce484222325ull;
+    auto mix = [&h](uint32_t w) {
+        h ^= w;
+        h *= 0x100000001b3ull;
+    };
+    mix((uint32_t) d0);
+    mix((uint32_t) d1);
+    mix((uint32_t) nnz);
+    mix((uint32_t) (nnz >> 32));
+    for (int k = 0; k < np; k++) mix((uint32_t) p[k]);
+    const int64_t S = std::min<int64_t>(nnz, 4096);
+    for (int64_t k = 0; k < S; k++)
+    {
+        const int64_t at = k * nnz / S;
+        mix((uint32_t) i[at]);
+        if (j) mix((uint32_t) j[at]);
+    }
+    return h;
+}
 
 CSR_matrix *alloc_csr_struct(int m, int n, int nnz)
 {
@@ -165,6 +197,7 @@ Out *transpose_alloc_impl(const char *fn, int m, int n, int nnz, const int *p, c
     st->src_m = m;
     st->src_n = n;
     st->nnz = nnz;
+    st->fp = fingerprint(m, n, m + 1, p, nnz, i);
     DBuf<int> dp = upload(p, (int64_t) m + 1), di = upload(i, nnz);
     DBuf<int> tp((size_t) n + 1), ti(nnz);
     st->perm.alloc(nnz);
@@ -183,11 +216,13 @@ Out *transpose_alloc_impl(const char *fn, int m, int n, int nnz, const int *p, c
     return C;
 }
 
-void transpose_fill_impl(const char *fn, const void *C, int nnz, const double *xin,
-                         double *xout)
+void transpose_fill_impl(const char *fn, const void *C, int m, int n, int nnz, const int *p,
+                         const int *i, const double *xin, double *xout)
 {
     TransposeState *st = attached_as<TransposeState>(C, fn);
-    if (st->nnz != nnz) api_exit(fn, "operand does not match the alloc");
+    if (st->nnz != nnz || st->src_m != m || st->src_n != n ||
+        st->fp != fingerprint(m, n, m + 1, p, nnz, i))
+        api_exit(fn, "operand does not match the alloc");
     h2d(st->xin.p, xin, (size_t) nnz * sizeof(double));
     gather_values(nnz, st->perm.p, st->xin.p, st->xout.p, stream());
     d2h(xout, st->xout.p, (size_t) nnz * sizeof(double));
@@ -207,6 +242,8 @@ CSR_matrix *product_alloc_impl(const char *fn, int Lm, const int *Lp, const int 
     st->Ai = upload(Ai, st->nnzA);
     st->Lp = upload(Lp, (int64_t) Lm + 1);
     st->Li = upload(Li, st->nnzL);
+    st->fpA = fingerprint(Am, An, Am + 1, Ap, st->nnzA, Ai);
+    st->fpL = fingerprint(Lm, Am, Lm + 1, Lp, st->nnzL, Li);
     LView L;
     L.Lm = Lm;
     L.Lp = st->Lp.p;
@@ -232,11 +269,14 @@ CSR_matrix *product_alloc_impl(const char *fn, int Lm, const int *Lp, const int 
     return C;
 }
 
-void product_fill_impl(const char *fn, const double *Lx, int nnzL, const double *d,
+void product_fill_impl(const char *fn, const int *Lp, const int *Li, const double *Lx,
+                       int nnzL, const double *d, const int *Ap, const int *Ai,
                        const double *Ax, int nnzA, CSR_matrix *C)
 {
     ProductState *st = attached_as<ProductState>(C, fn);
-    if (st->nnzA != nnzA || st->nnzL != nnzL || C->nnz != st->P.nnz)
+    if (st->nnzA != nnzA || st->nnzL != nnzL || C->nnz != st->P.nnz ||
+        st->fpA != fingerprint(st->Am, st->An, st->Am + 1, Ap, nnzA, Ai) ||
+        st->fpL != fingerprint(st->Lm, st->Am, st->Lm + 1, Lp, nnzL, Li))
         api_exit(fn, "operands do not match the alloc");
     h2d(st->Ax.p, Ax, (size_t) nnzA * sizeof(double));
     h2d(st->Lx.p, Lx, (size_t) nnzL * sizeof(double));
@@ -320,7 +360,7 @@ CSR_matrix *AT_alloc(const CSR_matrix *A, int *iwork)
 void AT_fill_values(const CSR_matrix *A, CSR_matrix *AT, int *iwork)
 {
     (void) iwork;
-    transpose_fill_impl("AT_fill_values", AT, A->nnz, A->x, AT->x);
+    transpose_fill_impl("AT_fill_values", AT, A->m, A->n, A->nnz, A->p, A->i, A->x, AT->x);
 }
 
 CSR_matrix *transpose(const CSR_matrix *A, int *iwork)
@@ -366,7 +406,8 @@ CSC_matrix *csr_to_csc_alloc(const CSR_matrix *A, int *iwork)
 void csr_to_csc_fill_values(const CSR_matrix *A, CSC_matrix *C, int *iwork)
 {
     (void) iwork;
-    transpose_fill_impl("csr_to_csc_fill_values", C, A->nnz, A->x, C->x);
+    transpose_fill_impl("csr_to_csc_fill_values", C, A->m, A->n, A->nnz, A->p, A->i, A->x,
+                        C->x);
 }
 
 CSR_matrix *csc_to_csr_alloc(const CSC_matrix *A, int *iwork)
@@ -383,7 +424,8 @@ CSR_matrix *csc_to_csr_alloc(const CSC_matrix *A, int *iwork)
 void csc_to_csr_fill_values(const CSC_matrix *A, CSR_matrix *C, int *iwork)
 {
     (void) iwork;
-    transpose_fill_impl("csc_to_csr_fill_values", C, A->nnz, A->x, C->x);
+    transpose_fill_impl("csc_to_csr_fill_values", C, A->n, A->m, A->nnz, A->p, A->i, A->x,
+                        C->x);
 }
 
 CSR_matrix *BTA_alloc(const CSR_matrix *A, const CSC_matrix *B)
@@ -398,7 +440,8 @@ void BTDA_fill_values(const CSR_matrix *A, const CSC_matrix *B, const double *d,
                       CSR_matrix *C)
 {
     (void) acc;
-    product_fill_impl("BTDA_fill_values", B->x, B->nnz, d, A->x, A->nnz, C);
+    product_fill_impl("BTDA_fill_values", B->p, B->i, B->x, B->nnz, d, A->p, A->i, A->x,
+                      A->nnz, C);
 }
 
 CSR_matrix *ATA_alloc(const CSC_matrix *A_csc, const CSR_matrix *A)
@@ -412,7 +455,8 @@ void ATDA_fill_values(const CSC_matrix *A_csc, const double *d, const CSR_matrix
                       double *acc, CSR_matrix *C)
 {
     (void) acc;
-    product_fill_impl("ATDA_fill_values", A_csc->x, A_csc->nnz, d, A->x, A->nnz, C);
+    product_fill_impl("ATDA_fill_values", A_csc->p, A_csc->i, A_csc->x, A_csc->nnz, d, A->p,
+                      A->i, A->x, A->nnz, C);
 }
 
 // ------------------------------------------------------------- COO ----
@@ -497,6 +541,7 @@ CSR_matrix *COO_coalesce_to_csr_alloc(const COO_matrix *A)
 {
     auto st = std::make_unique<CoalesceState>();
     st->nnz_in = A->nnz;
+    st->fp = fingerprint(A->m, A->n, 0, nullptr, A->nnz, A->rows, A->cols);
     DBuf<int> r = upload(A->rows, A->nnz), c = upload(A->cols, A->nnz);
     coalesce_plan(st->P, A->nnz, r.p, c.p, A->m, A->n);
     int nnz = checked_nnz("COO_coalesce_to_csr_alloc", st->P.nnz);
@@ -514,7 +559,9 @@ CSR_matrix *COO_coalesce_to_csr_alloc(const COO_matrix *A)
 void COO_coalesce_to_csr_fill_values(const COO_matrix *A, CSR_matrix *C)
 {
     CoalesceState *st = attached_as<CoalesceState>(C, "COO_coalesce_to_csr_fill_values");
-    if (st->nnz_in != A->nnz) api_exit("COO_coalesce_to_csr_fill_values", "COO does not match the alloc");
+    if (st->nnz_in != A->nnz ||
+        st->fp != fingerprint(A->m, A->n, 0, nullptr, A->nnz, A->rows, A->cols))
+        api_exit("COO_coalesce_to_csr_fill_values", "COO does not match the alloc");
     h2d(st->x.p, A->x, (size_t) A->nnz * sizeof(double));
     coalesce_fill(st->P, st->x.p, st->Cx.p, stream());
     d2h(C->x, st->Cx.p, (size_t) C->nnz * sizeof(double));

@@ -15,4 +15,4 @@ struct sdl_btda_plan
 int sdl_btda_plan_create_mode(int m, int nA, int nB, const int *dAp, const int *dAi,
                               const int *dBp, const int *dBi, sdl::SymMode mode,
                               const sdl::RowPredictor *pred, sdl_btda_plan **out,
-                              bool force_pull = false);
+                              bool force_pull = false, bool rank_refill = false);

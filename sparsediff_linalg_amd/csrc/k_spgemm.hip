@@ -1582,7 +1582,30 @@ __device__ __forceinline__ int rank_of(const unsigned *bm, const unsigned short 
     return (int) (b + (unsigned short) (sub[w] - (unsigned short) b) + __popc(below));
 }
 
-template <int G>
+// What the plan-time instance (PLAN = true) records, for rows of at most
+// RANK_POS_MAX entries: every term's entry position within its row at
+// rpos[roff[ri] + term index] (term index: the row's terms in ascending L
+// index, then A entry), or, for a term of an entry of several terms,
+// RANK_ITEM | its index in the row's item list -- the entry positions of
+// those terms sorted by (entry, L index), at ipos[moff[ri] + i], i < mcnt[ri]
+// <= RANK_LIST, in a region reserved by an atomic.  A longer row, or one whose
+// list outgrows RANK_LIST, is marked in rskip and appended to spill.
+constexpr int RANK_POS_MAX = 0x7FFF;
+constexpr int RANK_ITEM = 0x8000;
+struct RankRecord
+{
+    const int64_t *roff;
+    uint16_t *rpos;
+    unsigned char *rskip;
+    int64_t *moff;
+    int *mcnt;
+    uint16_t *ipos;
+    unsigned long long *mcount;
+    int64_t mcap;
+    int *err;
+};
+
+template <int G, bool PLAN = false>
 __global__ void __launch_bounds__(G)
     k_num_rowacc_rank(const int *__restrict__ rows, int nrows, int Lm,
                       const int *__restrict__ Lp, const int *__restrict__ Li,
@@ -1591,11 +1614,13 @@ __global__ void __launch_bounds__(G)
                       const int *__restrict__ Ci, const double *__restrict__ Lval,
                       const double *__restrict__ d, const double *__restrict__ Ax,
                       double *__restrict__ Cx, int nwords, int *__restrict__ spill,
-                      int *__restrict__ nspill)
+                      int *__restrict__ nspill, RankRecord rec = RankRecord())
 {
     extern __shared__ unsigned long long rank_lds[];
     unsigned long long *lkey = rank_lds;                       // RANK_LIST
     double *lval = (double *) (lkey + RANK_LIST);              // RANK_LIST
+    // PLAN: the payload of a listed term, its index in rpos
+    unsigned long long *lpay = (unsigned long long *) lval;
     unsigned *seen1 = (unsigned *) (lval + RANK_LIST);         // RANK_MAX_ENTRIES / 32
     unsigned *seen2 = seen1 + RANK_MAX_ENTRIES / 32;           // RANK_MAX_ENTRIES / 32
     unsigned *bm = seen2 + RANK_MAX_ENTRIES / 32;              // nwords
@@ -1605,6 +1630,7 @@ __global__ void __launch_bounds__(G)
     __shared__ Expand<G> ex;
     __shared__ int wsum[G / 64];
     __shared__ int nlist;
+    __shared__ unsigned long long mbase;
     const int per = (nwords + G - 1) / G;
     for (int ri = blockIdx.x; ri < nrows; ri += gridDim.x)
     {
@@ -1612,6 +1638,16 @@ __global__ void __launch_bounds__(G)
         const RowCtx c = row_ctx(row, Lm, Lp, nL);   // loads issued with Cp's
         const int64_t cbase = Cp[row];
         const int cnt = (int) (Cp[row + 1] - cbase);
+        if (PLAN && cnt > RANK_POS_MAX)
+        {
+            // positions take 15 bits: a longer row spills
+            if (threadIdx.x == 0)
+            {
+                spill[atomicAdd(nspill, 1)] = row;
+                rec.rskip[ri] = 1;
+            }
+            continue;
+        }
         const int swords = (cnt + 31) >> 5;
         for (int i = threadIdx.x; i < nwords; i += G) bm[i] = 0u;
         for (int i = threadIdx.x; i < swords; i += G)
@@ -1686,10 +1722,52 @@ __global__ void __launch_bounds__(G)
             __syncthreads();
         }
         // pass 2: single-term entries written, the others' terms listed
+        // (PLAN: every term's position recorded instead)
+        int64_t tbase = PLAN ? rec.roff[ri] : 0;
         for (int qb = c.q0; qb < c.q1; qb += G)
         {
             const int tot = one_chunk ? tot1 : expand_chunk<G>(ex, qb, c.q1, c.koff, Li, Ap);
             const int nq = min(G, c.q1 - qb);
+            if constexpr (PLAN)
+            {
+                for (int f0 = threadIdx.x; f0 < tot; f0 += RU2 * G)
+                {
+                    int col[RU2], qq[RU2];
+#pragma unroll
+                    for (int u = 0; u < RU2; u++)
+                    {
+                        const int fi = f0 + u * G;
+                        int t = 0, q = 0, k = 0;
+                        expand_term<G>(ex, nq, fi < tot ? fi : tot - 1, t, q, k);
+                        const int a = Ai[t];
+                        col[u] = fi < tot ? a : -1;
+                        qq[u] = q;
+                    }
+#pragma unroll
+                    for (int u = 0; u < RU2; u++)
+                    {
+                        if (col[u] < 0) continue;
+                        const int pos = rank_of(bm, sub, sup, col[u]);
+                        const bool single = !((seen2[pos >> 5] >> (pos & 31)) & 1u);
+                        const int64_t at = tbase + f0 + u * G;
+                        if (single) rec.rpos[at] = (uint16_t) pos;
+                        else
+                        {
+                            // the item index is written once the list is sorted
+                            const int i = atomicAdd(&nlist, 1);
+                            if (i < RANK_LIST)
+                            {
+                                lkey[i] = ((unsigned long long) (unsigned) pos << 32) |
+                                          (unsigned) qq[u];
+                                lpay[i] = (unsigned long long) at;
+                            }
+                        }
+                    }
+                }
+                tbase += tot;
+                __syncthreads();
+                continue;
+            }
             for (int f0 = threadIdx.x; f0 < tot; f0 += RU2 * G)
             {
                 int col[RU2], qq[RU2];
@@ -1749,7 +1827,11 @@ __global__ void __launch_bounds__(G)
         const int nl = nlist;
         if (nl > RANK_LIST)
         {
-            if (threadIdx.x == 0) spill[atomicAdd(nspill, 1)] = row;
+            if (threadIdx.x == 0)
+            {
+                spill[atomicAdd(nspill, 1)] = row;
+                if (PLAN) rec.rskip[ri] = 1;
+            }
             __syncthreads();
             continue;
         }
@@ -1759,7 +1841,7 @@ __global__ void __launch_bounds__(G)
         for (int i = nl + (int) threadIdx.x; i < np; i += G)
         {
             lkey[i] = ~0ull;
-            lval[i] = 0.0;
+            lpay[i] = 0ull;
         }
         __syncthreads();
         for (int kk = 2; kk <= np; kk <<= 1)
@@ -1776,13 +1858,40 @@ __global__ void __launch_bounds__(G)
                     {
                         lkey[i] = b;
                         lkey[ixj] = a;
-                        const double va = lval[i];
-                        lval[i] = lval[ixj];
-                        lval[ixj] = va;
+                        const unsigned long long va = lpay[i];
+                        lpay[i] = lpay[ixj];
+                        lpay[ixj] = va;
                     }
                 }
                 __syncthreads();
             }
+        }
+        if constexpr (PLAN)
+        {
+            // the sorted list becomes this row's region of the item list;
+            // each listed term's rpos slot gets its item index
+            if (threadIdx.x == 0) mbase = atomicAdd(rec.mcount, (unsigned long long) nl);
+            __syncthreads();
+            const unsigned long long mb = mbase;
+            if (mb + (unsigned long long) nl > (unsigned long long) rec.mcap)
+            {
+                if (threadIdx.x == 0) atomicOr(rec.err, 1);
+            }
+            else
+            {
+                if (threadIdx.x == 0)
+                {
+                    rec.moff[ri] = (int64_t) mb;
+                    rec.mcnt[ri] = nl;
+                }
+                for (int i = threadIdx.x; i < nl; i += G)
+                {
+                    rec.ipos[mb + i] = (uint16_t) (lkey[i] >> 32);
+                    rec.rpos[(int64_t) lpay[i]] = (uint16_t) (RANK_ITEM | i);
+                }
+            }
+            __syncthreads();
+            continue;
         }
         // fold each listed entry's terms in ascending q
         for (int i = threadIdx.x; i < nl; i += G)
@@ -1795,6 +1904,217 @@ __global__ void __launch_bounds__(G)
         }
         __syncthreads();
     }
+}
+
+__global__ void k_rank_rows(const int *__restrict__ rows, int n,
+                            const unsigned char *__restrict__ rskip,
+                            const int64_t *__restrict__ roff, const int64_t *__restrict__ moff,
+                            const int *__restrict__ mcnt, int Lm, const int *__restrict__ Lp,
+                            int nL, const int64_t *__restrict__ Cp, RankRow *__restrict__ out)
+{
+    const int i = (int) (blockIdx.x * blockDim.x + threadIdx.x);
+    if (i >= n) return;
+    const int row = rows[i];
+    const RowCtx c = row_ctx(row, Lm, Lp, nL);
+    RankRow r;
+    r.cbase = Cp[row];
+    r.tbase = roff[i];
+    r.skip = rskip[i];
+    r.mb = r.skip ? 0 : moff[i];
+    r.nl = r.skip ? 0 : mcnt[i];
+    r.q0 = c.q0;
+    r.q1 = c.q1;
+    r.koff = c.koff;
+    r.cnt = (int) (Cp[row + 1] - r.cbase);
+    out[i] = r;
+}
+
+// Refill of the rank bin from recorded positions.  A persistent grid, one
+// workgroup per CU (the staging window takes most of the LDS), strides over
+// the rows; a row is staged in LDS in windows of S entries (one window for
+// rows of at most S entries): its terms in the recorded order (ascending L
+// index, then A entry) -- 2-byte positions and A's values by row segment,
+// both coalesced -- each single-term entry staged as 0 + (L value * d_k) * A
+// value, each term of an entry of several terms kept at its item index; the
+// items then fold in their sorted order (ascending L index per entry, so
+// ((0 + x_1) + x_2) + ...), the accumulators' arithmetic; the window is
+// written out with whole-line stores.  The next row's descriptor and its
+// first chunk's L entries are loaded while the current row runs.
+template <int G, int U>
+__global__ void __launch_bounds__(G)
+    k_num_rank_pos(const RankRow *__restrict__ rr, int nrows, const uint16_t *__restrict__ rpos,
+                   const uint16_t *__restrict__ ipos, const int *__restrict__ Li,
+                   const int *__restrict__ Lperm, const int *__restrict__ Ap,
+                   const double *__restrict__ Lval, const double *__restrict__ d,
+                   const double *__restrict__ Ax, double *__restrict__ Cx, int S)
+{
+    extern __shared__ double stage[];  // S
+    __shared__ Expand<G> ex;
+    __shared__ double vq[G];
+    __shared__ double litem[RANK_LIST];
+    __shared__ unsigned short lipos[RANK_LIST];
+    int ri = blockIdx.x;
+    if (ri >= nrows) return;
+    RankRow cur = rr[ri];
+    // the first chunk's L entry of this thread, for the current row: its A
+    // row (k), that row's start and length, and L value * d_k
+    auto slot_k = [&](const RankRow &r, int &kq, int &lq) {
+        const int q = r.q0 + (int) threadIdx.x;
+        kq = -1;
+        lq = 0;
+        if (q < r.q1)
+        {
+            kq = Li[q] + r.koff;
+            lq = Lperm ? Lperm[q] : q;
+        }
+    };
+    auto slot_rest = [&](int kq, int lq, int &st, int &len, double &v) {
+        st = 0;
+        len = 0;
+        v = 0.0;
+        if (kq >= 0)
+        {
+            st = Ap[kq];
+            len = Ap[kq + 1] - st;
+            v = Lval[lq];
+            if (d) v = v * d[kq];
+        }
+    };
+    int pk, pl, pst, plen;
+    double pv;
+    slot_k(cur, pk, pl);
+    slot_rest(pk, pl, pst, plen, pv);
+    for (; ri < nrows; ri += gridDim.x)
+    {
+        const int rn = ri + gridDim.x;
+        RankRow nxt = cur;
+        if (rn < nrows) nxt = rr[rn];
+        int nk = -1, nlq = 0;
+        bool pre_next = false;
+        if (!cur.skip)
+        {
+            for (int i = threadIdx.x; i < cur.nl; i += G) lipos[i] = ipos[cur.mb + i];
+            for (int w0 = 0; w0 < cur.cnt; w0 += S)
+            {
+                const int w1 = min(cur.cnt, w0 + S);
+                int64_t tbase = cur.tbase;
+                for (int qb = cur.q0; qb < cur.q1; qb += G)
+                {
+                    int k = 0, st = 0, len = 0;
+                    double v = 0.0;
+                    if (w0 == 0 && qb == cur.q0)
+                    {
+                        k = pk < 0 ? 0 : pk;
+                        st = pst;
+                        len = plen;
+                        v = pv;
+                    }
+                    else
+                    {
+                        const int q = qb + (int) threadIdx.x;
+                        int kq = -1, lq = 0;
+                        if (q < cur.q1)
+                        {
+                            kq = Li[q] + cur.koff;
+                            lq = Lperm ? Lperm[q] : q;
+                        }
+                        slot_rest(kq, lq, st, len, v);
+                        k = kq < 0 ? 0 : kq;
+                    }
+                    int tot;
+                    const int ex_off = block_excl_scan_int<G>(len, &tot, ex.wsum);
+                    ex.start[threadIdx.x] = st;
+                    ex.k[threadIdx.x] = k;
+                    ex.q[threadIdx.x] = (int) threadIdx.x;  // the slot: vq's index
+                    ex.off[threadIdx.x] = ex_off;
+                    vq[threadIdx.x] = v;
+                    if (threadIdx.x == 0) ex.off[G] = tot;
+                    __syncthreads();
+                    if (!pre_next && rn < nrows)
+                    {
+                        // first level of the next row's first chunk
+                        slot_k(nxt, nk, nlq);
+                        pre_next = true;
+                    }
+                    const int nq = min(G, cur.q1 - qb);
+                    for (int f0 = threadIdx.x; f0 < tot; f0 += U * G)
+                    {
+                        int p[U];
+                        double x[U];
+#pragma unroll
+                        for (int u = 0; u < U; u++)
+                        {
+                            const int fi = f0 + u * G;
+                            const int fc = fi < tot ? fi : tot - 1;
+                            int t = 0, slot = 0, kk = 0;
+                            expand_term<G>(ex, nq, fc, t, slot, kk);
+                            const int pr = rpos[tbase + fc];
+                            x[u] = vq[slot] * Ax[t];
+                            p[u] = fi < tot ? pr : -1;
+                        }
+#pragma unroll
+                        for (int u = 0; u < U; u++)
+                        {
+                            const int pr = p[u];
+                            if (pr < 0) continue;
+                            if (pr & RANK_ITEM)
+                            {
+                                if (w0 == 0) litem[pr & (RANK_ITEM - 1)] = x[u];
+                            }
+                            else if (pr >= w0 && pr < w1) stage[pr - w0] = 0.0 + x[u];
+                        }
+                    }
+                    tbase += tot;
+                    __syncthreads();
+                }
+                // the entries of several terms in this window: the first item
+                // of each folds the entry's items in order
+                for (int i = threadIdx.x; i < cur.nl; i += G)
+                {
+                    const int pos = lipos[i];
+                    if (pos < w0 || pos >= w1) continue;
+                    if (i > 0 && lipos[i - 1] == pos) continue;
+                    double acc = 0.0;
+                    for (int j = i; j < cur.nl && lipos[j] == pos; j++) acc = acc + litem[j];
+                    stage[pos - w0] = acc;
+                }
+                __syncthreads();
+                double *out = Cx + cur.cbase + w0;
+                const int wn = w1 - w0;
+                int j = threadIdx.x;
+                for (; j + 3 * G < wn; j += 4 * G)
+                {
+                    const double a0 = stage[j], a1 = stage[j + G], a2 = stage[j + 2 * G],
+                                 a3 = stage[j + 3 * G];
+                    __builtin_nontemporal_store(a0, out + j);
+                    __builtin_nontemporal_store(a1, out + j + G);
+                    __builtin_nontemporal_store(a2, out + j + 2 * G);
+                    __builtin_nontemporal_store(a3, out + j + 3 * G);
+                }
+                for (; j < wn; j += G) __builtin_nontemporal_store(stage[j], out + j);
+                __syncthreads();
+            }
+        }
+        if (rn >= nrows) break;
+        if (!pre_next) slot_k(nxt, nk, nlq);
+        slot_rest(nk, nlq, pst, plen, pv);
+        pk = nk;
+        cur = nxt;
+    }
+}
+
+// Terms per row of the rank bin (for its offsets), its entry total and its
+// longest row.
+__global__ void k_rank_bin_sums(const int *__restrict__ rows, int n,
+                                const int64_t *__restrict__ rterms, const int *__restrict__ rowcnt,
+                                int64_t *__restrict__ out, unsigned long long *__restrict__ nnz,
+                                unsigned *__restrict__ longest)
+{
+    const int i = (int) (blockIdx.x * blockDim.x + threadIdx.x);
+    if (i >= n) return;
+    out[i] = rterms[rows[i]];
+    atomicAdd(nnz, (unsigned long long) rowcnt[rows[i]]);
+    atomicMax(longest, (unsigned) rowcnt[rows[i]]);
 }
 
 __global__ void k_zero_int(int *p, int n)
@@ -2038,6 +2358,96 @@ void product_row_features(const LView &L, const AView &A, float *F, hipStream_t 
     SDL_HIP(hipGetLastError());
 }
 
+// LDS of k_num_rank_pos<RP_G, RP_U> beside its staging window, and the
+// window: as many entries as the rest of the CU's 160 KB LDS holds, or the
+// bin's longest row when shorter.
+constexpr int RP_G = 512, RP_U = 8;
+constexpr int CU_LDS_BYTES = 160 * 1024;
+
+// Records the rank bin's term positions (ProductPlan::rankpos) when the bin's
+// terms are at most 1.25x its entries -- the entries of several terms, and so
+// the items, stay few -- and the positions take at most 48 GB.  Otherwise the
+// bin keeps the per-refill ranks.
+void record_rank_positions(ProductPlan &P, const LView &L, const AView &A, const int64_t *rterms,
+                           const int *rowcnt, int mid)
+{
+    const int *rows = P.rbins.p + P.rbin_off[1];
+    DBuf<int64_t> rt(mid);
+    DBuf<unsigned long long> nz(1);
+    DBuf<unsigned> longest(1);
+    dzero(nz.p, nz.bytes());
+    dzero(longest.p, longest.bytes());
+    hipLaunchKernelGGL(k_rank_bin_sums, dim3((unsigned) ((mid + 255) / 256)), dim3(256), 0,
+                       stream(), rows, mid, rterms, rowcnt, rt.p, nz.p, longest.p);
+    SDL_HIP(hipGetLastError());
+    DBuf<int64_t> roff((size_t) mid + 1);
+    const int64_t T = scan_i64(rt.p, roff.p, mid, true);
+    unsigned long long N = 0;
+    unsigned lmax = 0;
+    d2h(&N, nz.p, sizeof N);
+    d2h(&lmax, longest.p, sizeof lmax);
+    const int64_t extra = T - (int64_t) N;
+    if (T <= 0 || 4 * extra > (int64_t) N || 2 * T > ((int64_t) 48 << 30)) return;
+    P.roff = std::move(roff);
+    P.rpos.alloc((size_t) T);
+    P.rskip.alloc(mid);
+    P.moff.alloc(mid);
+    P.mcnt.alloc(mid);
+    dzero(P.rskip.p, P.rskip.bytes());
+    dzero(P.mcnt.p, P.mcnt.bytes());
+    // items <= 2 (terms - entries) (an entry of c >= 2 terms lists c of
+    // them), and <= RANK_LIST per row
+    const int64_t mcap =
+        std::max<int64_t>(1, std::min<int64_t>(2 * extra, (int64_t) mid * RANK_LIST));
+    P.ipos.alloc(mcap);
+    DBuf<unsigned long long> mcount(1);
+    DBuf<int> err(1);
+    dzero(mcount.p, mcount.bytes());
+    dzero(err.p, err.bytes());
+    dzero(P.nspill.p, sizeof(int));
+    RankRecord rec;
+    rec.roff = P.roff.p;
+    rec.rpos = P.rpos.p;
+    rec.rskip = P.rskip.p;
+    rec.moff = P.moff.p;
+    rec.mcnt = P.mcnt.p;
+    rec.ipos = P.ipos.p;
+    rec.mcount = mcount.p;
+    rec.mcap = mcap;
+    rec.err = err.p;
+    const int nwords = (int) ((P.n + 31) / 32);
+    SDL_HIP(hipFuncSetAttribute((const void *) k_num_rowacc_rank<1024, true>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int) rank_lds_bytes(RANK_MAX_WORDS)));
+    const int nL = A.Am / L.blocks;
+    hipLaunchKernelGGL((k_num_rowacc_rank<1024, true>), dim3(P.rank_grid), dim3(1024),
+                       rank_lds_bytes(nwords), stream(), rows, mid, L.Lm, L.Lp, L.Li, L.Lperm,
+                       nL, A.Ap, A.Ai, P.Cp.p, P.Ci.p, (const double *) nullptr,
+                       (const double *) nullptr, (const double *) nullptr, (double *) nullptr,
+                       nwords, P.spill.p, P.nspill.p, rec);
+    SDL_HIP(hipGetLastError());
+    int herr = 0;
+    unsigned long long nm = 0;
+    d2h(&herr, err.p, sizeof herr);
+    d2h(&nm, mcount.p, sizeof nm);
+    SDL_REQUIRE(herr == 0 && (int64_t) nm <= mcap, "rank positions: item list overflow");
+    P.nmulti = (int64_t) nm;
+    P.rrows.alloc(mid);
+    hipLaunchKernelGGL(k_rank_rows, dim3((unsigned) ((mid + 255) / 256)), dim3(256), 0, stream(),
+                       rows, mid, P.rskip.p, P.roff.p, P.moff.p, P.mcnt.p, L.Lm, L.Lp, nL,
+                       P.Cp.p, P.rrows.p);
+    SDL_HIP(hipGetLastError());
+    hipFuncAttributes fa;
+    SDL_HIP(hipFuncGetAttributes(&fa, (const void *) k_num_rank_pos<RP_G, RP_U>));
+    const int wmax = (CU_LDS_BYTES - (int) fa.sharedSizeBytes - 2048) / 8 / 256 * 256;
+    P.rank_window = std::min(wmax, (int) std::min<unsigned>(lmax, RANK_POS_MAX) + 255 & ~255);
+    P.rank_window = std::max(P.rank_window, 256);
+    SDL_HIP(hipFuncSetAttribute((const void *) k_num_rank_pos<RP_G, RP_U>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize,
+                                P.rank_window * (int) sizeof(double)));
+    P.rankpos = true;
+}
+
 void product_plan(ProductPlan &P, const LView &L, const AView &A, SymMode mode,
                   const RowPredictor *pred, bool allow_pull, bool force_pull)
 {
@@ -2186,6 +2596,7 @@ void product_plan(ProductPlan &P, const LView &L, const AView &A, SymMode mode,
             P.dense_grid = (int) std::min<int64_t>({(int64_t) dense_rows, fit, 1024});
             P.acc.alloc((size_t) P.dense_grid * std::max<int64_t>(P.n, 1));
         }
+        if (P.rank && !P.rank_refill) record_rank_positions(P, L, A, rterms.p, rowcnt.p, mid);
         sync();
     }
 }
@@ -2223,11 +2634,22 @@ void product_numeric(const ProductPlan &P, const double *Lval, const double *d,
                                             (int) rank_lds_bytes(RANK_MAX_WORDS)));
                 attr_set = true;
             }
-            hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, s, P.nspill.p, 1);
-            hipLaunchKernelGGL((k_num_rowacc_rank<1024>), dim3(P.rank_grid), dim3(1024), lds, s,
-                               long_rows, nmid, P.L.Lm, P.L.Lp, P.L.Li, P.L.Lperm, nL, P.A.Ap,
-                               P.A.Ai, P.Cp.p, P.Ci.p, Lval, d, Ax, Cx, nwords, P.spill.p,
-                               P.nspill.p);
+            if (P.rankpos)
+            {
+                // recorded positions; the spill list is the plan's
+                hipLaunchKernelGGL((k_num_rank_pos<RP_G, RP_U>), dim3(P.rank_grid), dim3(RP_G),
+                                   (size_t) P.rank_window * sizeof(double), s, P.rrows.p, nmid,
+                                   P.rpos.p, P.ipos.p, P.L.Li, P.L.Lperm, P.A.Ap, Lval, d, Ax,
+                                   Cx, P.rank_window);
+            }
+            else
+            {
+                hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, s, P.nspill.p, 1);
+                hipLaunchKernelGGL((k_num_rowacc_rank<1024>), dim3(P.rank_grid), dim3(1024),
+                                   lds, s, long_rows, nmid, P.L.Lm, P.L.Lp, P.L.Li, P.L.Lperm,
+                                   nL, P.A.Ap, P.A.Ai, P.Cp.p, P.Ci.p, Lval, d, Ax, Cx, nwords,
+                                   P.spill.p, P.nspill.p);
+            }
             hipLaunchKernelGGL((k_num_rowacc_dense<256>), dim3(P.dense_grid), dim3(256), 0, s,
                                P.spill.p, 0, P.nspill.p, P.L.Lm, P.L.Lp, P.L.Li, P.L.Lperm, nL,
                                P.A.Ap, P.A.Ai, P.Cp.p, P.Ci.p, Lval, d, Ax, Cx, P.acc.p,

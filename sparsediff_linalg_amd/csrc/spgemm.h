@@ -95,6 +95,15 @@ void tune_record(const ShapeSig &sig, const SymTune *tune, const RefillCfg *refi
 // Thresholds forced on the plans this thread creates (the tuner's candidates).
 extern thread_local const SymTune *g_force_tune;
 
+// One row of the rank bin as its position refill reads it (built by the plan).
+struct RankRow
+{
+    int64_t cbase;   // first entry of the row in C
+    int64_t tbase;   // first term of the row in the recorded positions
+    int64_t mb;      // first item of the row in the item positions
+    int q0, q1, koff, cnt, nl, skip;
+};
+
 struct ProductPlan
 {
     LView L;
@@ -120,6 +129,24 @@ struct ProductPlan
     // large for that (dense_grid == 0).
     int dense_grid = 0;
     DBuf<double> acc;
+    // Rank bin with recorded positions (its terms at most 1.25x its entries;
+    // the plan records them unless rank_refill): every term's entry position
+    // within its row, or the item index of a term of an entry of several
+    // terms, whose items are the entry positions sorted by (entry, L index);
+    // rows of more than 32767 entries or 1024 items are marked in rskip and
+    // listed in spill[0 .. *nspill) once, by the plan.  A refill stages each
+    // row in LDS windows of rank_window entries.
+    bool rank_refill = false;        // set before planning: keep per-refill ranks
+    bool rankpos = false;
+    int rank_window = 0;
+    DBuf<int64_t> roff;              // rank-bin row ri's terms start at roff[ri]
+    DBuf<uint16_t> rpos;
+    DBuf<unsigned char> rskip;
+    int64_t nmulti = 0;              // items
+    DBuf<int64_t> moff;              // row ri's items start at moff[ri] ...
+    DBuf<int> mcnt;                  // ... and number mcnt[ri]
+    DBuf<uint16_t> ipos;
+    DBuf<RankRow> rrows;             // the refill's per-row descriptors
     SymStats stats;
     RefillCfg tuned;   // variant < 0: not tuned, use the default heuristic
     SymTune tune;      // thresholds the symbolic phase and the refill bins used

@@ -82,6 +82,14 @@ class Plan:
         check(self.lib.sdl_btda_plan_row_bins(self.h, Y.ptr), "bins")
         return Y
 
+    def schedule(self):
+        """The refill schedule: pull, by-rank rows, recorded positions, listed
+        multi-term items, rows left to the dense accumulator."""
+        o = (C.c_longlong * 5)()
+        check(self.lib.sdl_btda_plan_schedule(self.h, o), "schedule")
+        return dict(pull=bool(o[0]), rank_rows=o[1], rank_positions=bool(o[2]), items=o[3],
+                    dense_rows=o[4])
+
     def tune(self):
         """The symbolic / refill thresholds this plan was created with."""
         from ._lib import sdl_sym_tune

@@ -474,3 +474,67 @@ def test_ata_alloc_beyond_int_max_exits():
     assert r.returncode == 1, (r.returncode, r.stdout[-500:], r.stderr[-500:])
     assert "calling" in r.stdout and "returned" not in r.stdout
     assert "ATA_matrices_alloc" in r.stderr and "INT_MAX" in r.stderr, r.stderr[-500:]
+
+
+_MISMATCH_CHILD = r'''
+import ctypes as C, sys
+sys.path.insert(0, %(root)r)
+import numpy as np
+from tests import dropin as D
+from sparsediff_linalg_amd._lib import COO_matrix
+L = D.lib()
+case = %(case)r
+m, n = 6, 5
+p = np.array([0, 2, 3, 5, 6, 8, 9], np.int32)
+i1 = np.array([0, 3, 1, 2, 4, 0, 1, 3, 2], np.int32)
+i2 = i1.copy(); i2[3] = 3               # same pointers and nnz, one column moved
+x = np.arange(1.0, 10.0)
+d = np.ones(m)
+A1, A2 = D.new_csr(p, i1, x, m, n), D.new_csr(p, i2, x, m, n)
+if case == "AT":
+    T = L.AT_alloc(A1, None); fill = lambda: L.AT_fill_values(A2, T, None)
+elif case == "csr_to_csc":
+    T = L.csr_to_csc_alloc(A1, None); fill = lambda: L.csr_to_csc_fill_values(A2, T, None)
+elif case == "csc_to_csr":
+    S1, S2 = D.new_csc(p[:6], i1[:8], x[:8], 5, 5), D.new_csc(p[:6], i2[:8], x[:8], 5, 5)
+    T = L.csc_to_csr_alloc(S1, None); fill = lambda: L.csc_to_csr_fill_values(S2, T, None)
+elif case == "BTDA":
+    B1 = L.csr_to_csc_alloc(A1, None); L.csr_to_csc_fill_values(A1, B1, None)
+    B2 = L.csr_to_csc_alloc(A2, None); L.csr_to_csc_fill_values(A2, B2, None)
+    T = L.BTA_alloc(A1, B1)
+    L.BTDA_fill_values(A1, B1, D.dptr(d), None, T)         # the alloc's structure: fine
+    fill = lambda: L.BTDA_fill_values(A1, B2, D.dptr(d), None, T)
+elif case == "ATDA":
+    B1 = L.csr_to_csc_alloc(A1, None); L.csr_to_csc_fill_values(A1, B1, None)
+    T = L.ATA_alloc(B1, A1)
+    L.ATDA_fill_values(B1, D.dptr(d), A1, None, T)
+    fill = lambda: L.ATDA_fill_values(B1, D.dptr(d), A2, None, T)
+else:
+    r = np.array([0, 1, 1, 2, 0], np.int32); c1 = np.array([0, 1, 1, 2, 3], np.int32)
+    c2 = c1.copy(); c2[4] = 4
+    v = np.ones(5)
+    k1 = COO_matrix(D.iptr(r), D.iptr(c1), D.dptr(v), None, 3, 5, 5)
+    k2 = COO_matrix(D.iptr(r), D.iptr(c2), D.dptr(v), None, 3, 5, 5)
+    T = L.COO_coalesce_to_csr_alloc(C.byref(k1))
+    L.COO_coalesce_to_csr_fill_values(C.byref(k1), T)
+    fill = lambda: L.COO_coalesce_to_csr_fill_values(C.byref(k2), T)
+print("calling", flush=True)
+fill()
+print("returned", flush=True)
+'''
+
+
+@pytest.mark.parametrize("case", ["AT", "csr_to_csc", "csc_to_csr", "BTDA", "ATDA", "COO"])
+def test_fill_with_another_structure_exits(case):
+    """A fill handed an operand whose structure differs from the alloc's (same
+    dimensions, pointers and nnz; one column index moved) exits(1) with the
+    alloc-mismatch message instead of gathering through the alloc's
+    permutation (the fills check a structure fingerprint); child process."""
+    import subprocess
+    import sys
+    code = _MISMATCH_CHILD % {"root": os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                              "case": case}
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 1, (r.returncode, r.stdout[-500:], r.stderr[-800:])
+    assert "calling" in r.stdout and "returned" not in r.stdout
+    assert "match the alloc" in r.stderr, r.stderr[-500:]
