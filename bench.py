@@ -66,6 +66,9 @@ def parse():
     ap.add_argument("--gemm-slice", type=int, default=0,
                     help="blocks-*: reduction slice depth of the block GEMM (16 or 32; 0 = "
                          "library default)")
+    ap.add_argument("--rank-refill", action="store_true",
+                    help="ata-large: plan with SDL_PLAN_RANK_REFILL (rows of more than 1024 "
+                         "entries re-ranked at every refill instead of recorded positions)")
     ap.add_argument("--sweep", action="store_true",
                     help="btda: also time every refill launch configuration (stderr table)")
     ap.add_argument("--calib-copy-mb", type=int, default=0,
@@ -635,7 +638,7 @@ def main_ata_large(args):
     Ap, Ai, Ax = workloads.zipf_csr(2, m, n)
     nnzA = int(Ap[-1])
     t0 = time.perf_counter()
-    pl = Plan((Ap, Ai), None, n, n)
+    pl = Plan((Ap, Ai), None, n, n, mode=512 if args.rank_refill else 0)
     lib.sdl_synchronize()
     t_sym = time.perf_counter() - t0
     nnz, terms = pl.nnz(), lib.sdl_btda_plan_terms(pl.h)
@@ -695,13 +698,16 @@ def main_ata_large(args):
             "sampled_max_rel_err_vs_scipy": worst,
         },
         "roofline": {"bound": "hbm",
-                     "kernel": ("k_num_rank_pos<256, 8> (recorded positions) + k_num_multi + "
-                                "k_num_rowacc_dense spills" if sched["rank_positions"] else
+                     "kernel": ("k_num_rank_pos<1024, 512, 8> (recorded positions, rows "
+                                "staged in LDS) + k_num_rowacc_dense spills"
+                                if sched["rank_positions"] else
                                 "k_num_rowacc_rank<1024> (rows by rank) + k_num_rowacc_dense "
                                 "spills"),
                      "achieved": bytes_ / t / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": bytes_ / t / 1e9 / HBM_PEAK_GBS,
-                     "traffic": measured_traffic("k_num_rowacc_rank_ata_large_traffic.json"),
+                     "traffic": measured_traffic("k_num_rank_pos_ata_large_traffic.json"
+                                                 if sched["rank_positions"] else
+                                                 "k_num_rowacc_rank_ata_large_traffic.json"),
                      "alg_bytes_per_launch": bytes_},
         "gflops": (nnzA + 2 * terms) / t / 1e9,
         "symbolic_terms_per_s": terms / t_sym,

@@ -1940,7 +1940,7 @@ __global__ void k_rank_rows(const int *__restrict__ rows, int n,
 // ((0 + x_1) + x_2) + ...), the accumulators' arithmetic; the window is
 // written out with whole-line stores.  The next row's descriptor and its
 // first chunk's L entries are loaded while the current row runs.
-template <int G, int U>
+template <int G, int CH, int U>
 __global__ void __launch_bounds__(G)
     k_num_rank_pos(const RankRow *__restrict__ rr, int nrows, const uint16_t *__restrict__ rpos,
                    const uint16_t *__restrict__ ipos, const int *__restrict__ Li,
@@ -1948,9 +1948,11 @@ __global__ void __launch_bounds__(G)
                    const double *__restrict__ Lval, const double *__restrict__ d,
                    const double *__restrict__ Ax, double *__restrict__ Cx, int S)
 {
+    static_assert(CH <= G && CH % 64 == 0, "chunk of at most G L entries");
     extern __shared__ double stage[];  // S
-    __shared__ Expand<G> ex;
-    __shared__ double vq[G];
+    __shared__ Expand<CH> ex;          // a chunk of CH L entries, the first CH threads'
+    __shared__ int wsum[G / 64];
+    __shared__ double vq[CH];
     __shared__ double litem[RANK_LIST];
     __shared__ unsigned short lipos[RANK_LIST];
     int ri = blockIdx.x;
@@ -1962,7 +1964,7 @@ __global__ void __launch_bounds__(G)
         const int q = r.q0 + (int) threadIdx.x;
         kq = -1;
         lq = 0;
-        if (q < r.q1)
+        if ((int) threadIdx.x < CH && q < r.q1)
         {
             kq = Li[q] + r.koff;
             lq = Lperm ? Lperm[q] : q;
@@ -1998,7 +2000,7 @@ __global__ void __launch_bounds__(G)
             {
                 const int w1 = min(cur.cnt, w0 + S);
                 int64_t tbase = cur.tbase;
-                for (int qb = cur.q0; qb < cur.q1; qb += G)
+                for (int qb = cur.q0; qb < cur.q1; qb += CH)
                 {
                     int k = 0, st = 0, len = 0;
                     double v = 0.0;
@@ -2013,7 +2015,7 @@ __global__ void __launch_bounds__(G)
                     {
                         const int q = qb + (int) threadIdx.x;
                         int kq = -1, lq = 0;
-                        if (q < cur.q1)
+                        if ((int) threadIdx.x < CH && q < cur.q1)
                         {
                             kq = Li[q] + cur.koff;
                             lq = Lperm ? Lperm[q] : q;
@@ -2022,13 +2024,16 @@ __global__ void __launch_bounds__(G)
                         k = kq < 0 ? 0 : kq;
                     }
                     int tot;
-                    const int ex_off = block_excl_scan_int<G>(len, &tot, ex.wsum);
-                    ex.start[threadIdx.x] = st;
-                    ex.k[threadIdx.x] = k;
-                    ex.q[threadIdx.x] = (int) threadIdx.x;  // the slot: vq's index
-                    ex.off[threadIdx.x] = ex_off;
-                    vq[threadIdx.x] = v;
-                    if (threadIdx.x == 0) ex.off[G] = tot;
+                    const int ex_off = block_excl_scan_int<G>(len, &tot, wsum);
+                    if ((int) threadIdx.x < CH)
+                    {
+                        ex.start[threadIdx.x] = st;
+                        ex.k[threadIdx.x] = k;
+                        ex.q[threadIdx.x] = (int) threadIdx.x;  // the slot: vq's index
+                        ex.off[threadIdx.x] = ex_off;
+                        vq[threadIdx.x] = v;
+                    }
+                    if (threadIdx.x == 0) ex.off[CH] = tot;
                     __syncthreads();
                     if (!pre_next && rn < nrows)
                     {
@@ -2036,7 +2041,7 @@ __global__ void __launch_bounds__(G)
                         slot_k(nxt, nk, nlq);
                         pre_next = true;
                     }
-                    const int nq = min(G, cur.q1 - qb);
+                    const int nq = min(CH, cur.q1 - qb);
                     for (int f0 = threadIdx.x; f0 < tot; f0 += U * G)
                     {
                         int p[U];
@@ -2047,7 +2052,7 @@ __global__ void __launch_bounds__(G)
                             const int fi = f0 + u * G;
                             const int fc = fi < tot ? fi : tot - 1;
                             int t = 0, slot = 0, kk = 0;
-                            expand_term<G>(ex, nq, fc, t, slot, kk);
+                            expand_term<CH>(ex, nq, fc, t, slot, kk);
                             const int pr = rpos[tbase + fc];
                             x[u] = vq[slot] * Ax[t];
                             p[u] = fi < tot ? pr : -1;
@@ -2079,19 +2084,23 @@ __global__ void __launch_bounds__(G)
                     stage[pos - w0] = acc;
                 }
                 __syncthreads();
+                // written out in 16-byte stores from the first 16-byte
+                // boundary (an entry before it is stored alone)
                 double *out = Cx + cur.cbase + w0;
                 const int wn = w1 - w0;
-                int j = threadIdx.x;
-                for (; j + 3 * G < wn; j += 4 * G)
+                const int h = (int) (((uintptr_t) out >> 3) & 1);
+                if (h && threadIdx.x == 0) __builtin_nontemporal_store(stage[0], out);
+                const int npair = (wn - h) >> 1;
+                dvec2 *out2 = (dvec2 *) (out + h);
+                for (int j = threadIdx.x; j < npair; j += G)
                 {
-                    const double a0 = stage[j], a1 = stage[j + G], a2 = stage[j + 2 * G],
-                                 a3 = stage[j + 3 * G];
-                    __builtin_nontemporal_store(a0, out + j);
-                    __builtin_nontemporal_store(a1, out + j + G);
-                    __builtin_nontemporal_store(a2, out + j + 2 * G);
-                    __builtin_nontemporal_store(a3, out + j + 3 * G);
+                    dvec2 v2;
+                    v2.x = stage[h + 2 * j];
+                    v2.y = stage[h + 2 * j + 1];
+                    __builtin_nontemporal_store(v2, out2 + j);
                 }
-                for (; j < wn; j += G) __builtin_nontemporal_store(stage[j], out + j);
+                if (((wn - h) & 1) && threadIdx.x == 0)
+                    __builtin_nontemporal_store(stage[wn - 1], out + wn - 1);
                 __syncthreads();
             }
         }
@@ -2358,10 +2367,10 @@ void product_row_features(const LView &L, const AView &A, float *F, hipStream_t 
     SDL_HIP(hipGetLastError());
 }
 
-// LDS of k_num_rank_pos<RP_G, RP_U> beside its staging window, and the
+// LDS of k_num_rank_pos<RP_G, RP_CH, RP_U> beside its staging window, and the
 // window: as many entries as the rest of the CU's 160 KB LDS holds, or the
 // bin's longest row when shorter.
-constexpr int RP_G = 512, RP_U = 8;
+constexpr int RP_G = 1024, RP_CH = 512, RP_U = 8;
 constexpr int CU_LDS_BYTES = 160 * 1024;
 
 // Records the rank bin's term positions (ProductPlan::rankpos) when the bin's
@@ -2438,11 +2447,11 @@ void record_rank_positions(ProductPlan &P, const LView &L, const AView &A, const
                        P.Cp.p, P.rrows.p);
     SDL_HIP(hipGetLastError());
     hipFuncAttributes fa;
-    SDL_HIP(hipFuncGetAttributes(&fa, (const void *) k_num_rank_pos<RP_G, RP_U>));
+    SDL_HIP(hipFuncGetAttributes(&fa, (const void *) k_num_rank_pos<RP_G, RP_CH, RP_U>));
     const int wmax = (CU_LDS_BYTES - (int) fa.sharedSizeBytes - 2048) / 8 / 256 * 256;
     P.rank_window = std::min(wmax, (int) std::min<unsigned>(lmax, RANK_POS_MAX) + 255 & ~255);
     P.rank_window = std::max(P.rank_window, 256);
-    SDL_HIP(hipFuncSetAttribute((const void *) k_num_rank_pos<RP_G, RP_U>,
+    SDL_HIP(hipFuncSetAttribute((const void *) k_num_rank_pos<RP_G, RP_CH, RP_U>,
                                 hipFuncAttributeMaxDynamicSharedMemorySize,
                                 P.rank_window * (int) sizeof(double)));
     P.rankpos = true;
@@ -2637,7 +2646,7 @@ void product_numeric(const ProductPlan &P, const double *Lval, const double *d,
             if (P.rankpos)
             {
                 // recorded positions; the spill list is the plan's
-                hipLaunchKernelGGL((k_num_rank_pos<RP_G, RP_U>), dim3(P.rank_grid), dim3(RP_G),
+                hipLaunchKernelGGL((k_num_rank_pos<RP_G, RP_CH, RP_U>), dim3(P.rank_grid), dim3(RP_G),
                                    (size_t) P.rank_window * sizeof(double), s, P.rrows.p, nmid,
                                    P.rpos.p, P.ipos.p, P.L.Li, P.L.Lperm, P.A.Ap, Lval, d, Ax,
                                    Cx, P.rank_window);
