@@ -45,6 +45,11 @@ int sdl_write_kernel(void *dst, size_t bytes, int mode, void *stream);
    rows (groups = 1: every row), one 8-byte non-temporal store per row. */
 int sdl_write_rows_kernel(double *dC, long long ldc, int K, long long n, int groups,
                           void *stream);
+/* Sustained FP64 matrix-rate calibration: blocks workgroups of 4 wavefronts,
+   each wavefront running nacc (1, 2, 4, 8, 16) independent accumulators
+   through iters rounds of v_mfma_f64_16x16x4_f64 on registers, so
+   blocks * 4 * iters * nacc * 2048 flops; out receives blocks * 256 doubles. */
+int sdl_mfma_kernel(int nacc, int blocks, int iters, double *out, void *stream);
 
 /* ---- C = B^T diag(d) A on device-resident CSR operands (the hot path of
    BTA_matrices_alloc / BTDA_matrices_fill_values for two sparse operands,
@@ -421,6 +426,12 @@ struct matrix;
    the library stream.  Allocates nothing. */
 int sdl_block_refill_device(struct matrix *C, struct matrix *left, struct matrix *right,
                             const double *dd, void *ev_start, void *ev_stop);
+/* As sdl_block_refill_device, with ev_products (may be NULL) recorded after
+   the block products (GEMM and sparse x dense lists) and before the ordered
+   coalesce into C's blocks: the two phases timed apart. */
+int sdl_block_refill_device_split(struct matrix *C, struct matrix *left, struct matrix *right,
+                                  const double *dd, void *ev_start, void *ev_products,
+                                  void *ev_stop);
 /* out[8]: GEMM flops, sparse x dense flops, GEMM tiles, GEMM tasks, raw
    partial cells, output cells, coalesce terms, sparse x dense lists. */
 int sdl_block_stats(const struct matrix *C, double *out);

@@ -98,6 +98,7 @@ SIGNATURES = {
     "sdl_device_name": (C.c_int, [C.c_char_p, C.c_int]),
     "sdl_copy_kernel": (C.c_int, [vp, vp, C.c_size_t, vp]),
     "sdl_write_kernel": (C.c_int, [vp, C.c_size_t, C.c_int, vp]),
+    "sdl_mfma_kernel": (C.c_int, [C.c_int, C.c_int, C.c_int, vp, vp]),
     "sdl_write_rows_kernel": (C.c_int, [vp, C.c_longlong, C.c_int, C.c_longlong, C.c_int, vp]),
     # device-resident B^T D A plans
     "sdl_btda_plan_create": (C.c_int, [C.c_int, C.c_int, C.c_int, vp, vp, vp, vp, C.c_int,
@@ -203,6 +204,7 @@ SIGNATURES = {
     "sdl_btda_plan_pred_stats": (C.c_int, [vp, c_ll_p]),
     # device refills of block-kind product outputs (matrix * passed as pointers)
     "sdl_block_refill_device": (C.c_int, [vp, vp, vp, vp, vp, vp]),
+    "sdl_block_refill_device_split": (C.c_int, [vp, vp, vp, vp, vp, vp, vp]),
     "sdl_block_stats": (C.c_int, [vp, c_double_p]),
     "sdl_block_device_values": (vp, [vp]),
     "sdl_block_publish": (C.c_int, [vp]),

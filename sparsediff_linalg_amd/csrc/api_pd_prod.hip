@@ -425,7 +425,7 @@ ProdB *prod_of(matrix *C, const char *fn)
 // The device part of a fill: every kernel of C's program on the library
 // stream, d already in device memory (or null), C's device values written.
 void launch(ProdB *b, matrix *B, matrix *A, const double *dd, matrix *C, hipEvent_t e0,
-            hipEvent_t e1)
+            hipEvent_t e1, hipEvent_t e_gemm = nullptr)
 {
     const double *dB = mat_vals(b->left_view ? b->left_view : B);
     const double *dA = mat_vals(A);
@@ -438,6 +438,7 @@ void launch(ProdB *b, matrix *B, matrix *A, const double *dd, matrix *C, hipEven
         const double *sv = j->sparse_is_B ? dB : dA;
         sdmm_run(j->plan, sv, dense, dd, target + j->out_off, stream());
     }
+    if (e_gemm) SDL_HIP(hipEventRecord(e_gemm, stream()));
     if (b->to_raw) pull_sum(b->coalesce, b->raw.p, out_buf(C), stream());
     if (e1) SDL_HIP(hipEventRecord(e1, stream()));
 }
@@ -789,8 +790,8 @@ void pd_dense_kron_fill(const matrix *A, int p, matrix *J, matrix *C, bool kron_
 extern "C" {
 #pragma GCC visibility push(default)
 
-int sdl_block_refill_device(matrix *C, matrix *left, matrix *right, const double *dd,
-                            void *ev_start, void *ev_stop)
+int sdl_block_refill_device_split(matrix *C, matrix *left, matrix *right, const double *dd,
+                                  void *ev_start, void *ev_products, void *ev_stop)
 {
     using namespace sdl;
     if (!C || !left || !right || (C->kind != MATRIX_PERMUTED_DENSE && C->kind != MATRIX_STACKED_PD))
@@ -806,8 +807,14 @@ int sdl_block_refill_device(matrix *C, matrix *left, matrix *right, const double
         return 1;
     }
     launch(b, left, right, b->d_len > 0 ? dd : nullptr, C, (hipEvent_t) ev_start,
-           (hipEvent_t) ev_stop);
+           (hipEvent_t) ev_stop, (hipEvent_t) ev_products);
     return 0;
+}
+
+int sdl_block_refill_device(matrix *C, matrix *left, matrix *right, const double *dd,
+                            void *ev_start, void *ev_stop)
+{
+    return sdl_block_refill_device_split(C, left, right, dd, ev_start, nullptr, ev_stop);
 }
 
 int sdl_block_stats(const matrix *C, double *out)

@@ -105,7 +105,10 @@ constexpr int XCD_RUN = 4;  // consecutive tiles kept on one XCD (shared panels)
 
 // One SK x 64 slice of a task's P and Q operands into registers: thread tid
 // owns tile column ii = tid % 64 and slice rows sb, sb + 4, ... (sb = tid / 64),
-// so each wavefront reads 64 consecutive columns of a row.
+// so each wavefront reads 64 consecutive columns of a row.  The slice's row
+// indices come first (direct, or pool lookups issued together), then every
+// value load, each at a clamped valid address with out-of-range values
+// selected to 0 afterwards, so no load waits behind a branch.
 template <int SK>
 __device__ __forceinline__ void load_slice(const GJob &job, const GTask &T, int s0,
                                            const int *__restrict__ pool,
@@ -115,6 +118,7 @@ __device__ __forceinline__ void load_slice(const GJob &job, const GTask &T, int 
                                            double pr[SK / 4], double qr[SK / 4],
                                            double wr[SK / 4])
 {
+    constexpr int U = SK / 4;
     const double *P = (T.p_buf ? buf1 : buf0) + T.p_off;
     const double *Q = (T.q_buf ? buf1 : buf0) + T.q_off;
     const int ii = tid & 63, sb = tid >> 6;
@@ -122,28 +126,53 @@ __device__ __forceinline__ void load_slice(const GJob &job, const GTask &T, int 
     int mi = -1, mj = -1;
     if (i < job.M) mi = T.pmap < 0 ? i : pool[T.pmap + i];
     if (j < job.N) mj = T.qmap < 0 ? j : pool[T.qmap + j];
-#pragma unroll
-    for (int u = 0; u < SK / 4; u++)
+    int rp[U], rq[U];
+    if (T.direct & 1)
     {
-        const int s = s0 + sb + 4 * u;
-        double pv = 0.0, qv = 0.0, wv = 1.0;
-        if (s < T.S)
-        {
-            if (mi >= 0)
-            {
-                const long long r = (T.direct & 1) ? T.p_r0 + s : pool[T.prow + s];
-                pv = T.p_mode == 0 ? P[r * T.p_ld + mi] : P[(long long) mi * T.p_ld + r];
-            }
-            if (mj >= 0)
-            {
-                const long long r = (T.direct & 2) ? T.q_r0 + s : pool[T.qrow + s];
-                qv = T.q_mode == 0 ? Q[r * T.q_ld + mj] : Q[(long long) mj * T.q_ld + r];
-                if (w != nullptr && T.widx >= 0) wv = w[T.widx + s];
-            }
-        }
-        pr[u] = pv;
-        qr[u] = qv;
-        wr[u] = wv;
+#pragma unroll
+        for (int u = 0; u < U; u++) rp[u] = T.p_r0 + min(s0 + sb + 4 * u, T.S - 1);
+    }
+    else
+    {
+#pragma unroll
+        for (int u = 0; u < U; u++) rp[u] = pool[T.prow + min(s0 + sb + 4 * u, T.S - 1)];
+    }
+    if (T.direct & 2)
+    {
+#pragma unroll
+        for (int u = 0; u < U; u++) rq[u] = T.q_r0 + min(s0 + sb + 4 * u, T.S - 1);
+    }
+    else
+    {
+#pragma unroll
+        for (int u = 0; u < U; u++) rq[u] = pool[T.qrow + min(s0 + sb + 4 * u, T.S - 1)];
+    }
+    const long long mic = mi >= 0 ? mi : 0, mjc = mj >= 0 ? mj : 0;
+    double pv[U], qv[U], wv[U];
+#pragma unroll
+    for (int u = 0; u < U; u++)
+    {
+        const long long r = rp[u], q = rq[u];
+        pv[u] = P[T.p_mode == 0 ? r * T.p_ld + mic : mic * T.p_ld + r];
+        qv[u] = Q[T.q_mode == 0 ? q * T.q_ld + mjc : mjc * T.q_ld + q];
+    }
+    if (w != nullptr && T.widx >= 0)
+    {
+#pragma unroll
+        for (int u = 0; u < U; u++) wv[u] = w[T.widx + min(s0 + sb + 4 * u, T.S - 1)];
+    }
+    else
+    {
+#pragma unroll
+        for (int u = 0; u < U; u++) wv[u] = 1.0;
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++)
+    {
+        const bool in = s0 + sb + 4 * u < T.S;
+        pr[u] = (in && mi >= 0) ? pv[u] : 0.0;
+        qr[u] = (in && mj >= 0) ? qv[u] : 0.0;
+        wr[u] = in ? wv[u] : 1.0;
     }
 }
 
@@ -152,17 +181,6 @@ __global__ void k_gather_w(int n, const int *__restrict__ src, const double *__r
 {
     int q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q < n) w[q] = d[src[q]];
-}
-
-// next (task, slice) after (t, s0) in the job, skipping empty tasks
-template <int SK>
-__device__ __forceinline__ void next_slice(const GTask *__restrict__ tasks, int t1, int &t, int &s0,
-                                           int S)
-{
-    s0 += SK;
-    if (s0 < S) return;
-    s0 = 0;
-    for (t = t + 1; t < t1 && tasks[t].S <= 0; t++) {}
 }
 
 // Grid position of job index j (jobs sorted by decreasing work): runs of
@@ -213,7 +231,14 @@ __global__ void __launch_bounds__(256)
     while (t < job.t1 && tasks[t].S <= 0) t++;
     int s0 = 0;
     double pr[SK / 4], qr[SK / 4], wr[SK / 4];
-    if (t < job.t1) load_slice<SK>(job, tasks[t], 0, pool, buf0, buf1, w, tid, pr, qr, wr);
+    // the current task stays in registers; the task table is read again only
+    // when the job moves to its next task
+    GTask T;
+    if (t < job.t1)
+    {
+        T = tasks[t];
+        load_slice<SK>(job, T, 0, pool, buf0, buf1, w, tid, pr, qr, wr);
+    }
     while (t < job.t1)
     {
         // the d factor scales Q's row before the product: ((d Q) entries)
@@ -224,10 +249,15 @@ __global__ void __launch_bounds__(256)
             Qs[sb + 4 * u][ii] = wr[u] * qr[u];
         }
         __syncthreads();
-        int nt = t, ns0 = s0;
-        next_slice<SK>(tasks, job.t1, nt, ns0, tasks[t].S);
+        int nt = t, ns0 = s0 + SK;
+        if (ns0 >= T.S)
+        {
+            ns0 = 0;
+            for (nt = t + 1; nt < job.t1 && tasks[nt].S <= 0; nt++) {}
+            if (nt < job.t1) T = tasks[nt];
+        }
         // the next slice's loads are in flight during this slice's MFMAs
-        if (nt < job.t1) load_slice<SK>(job, tasks[nt], ns0, pool, buf0, buf1, w, tid, pr, qr, wr);
+        if (nt < job.t1) load_slice<SK>(job, T, ns0, pool, buf0, buf1, w, tid, pr, qr, wr);
 #pragma unroll
         for (int kk = 0; kk < SK; kk += 4)
         {

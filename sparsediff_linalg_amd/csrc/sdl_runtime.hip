@@ -152,10 +152,54 @@ void dzero(void *dst, size_t bytes)
     SDL_HIP(hipMemsetAsync(dst, 0, bytes, stream()));
 }
 
+// Sustained FP64 matrix rate: every wavefront runs NACC independent
+// accumulators through iters rounds of v_mfma_f64_16x16x4_f64 on register
+// operands (no memory traffic in the loop); one value per thread is stored.
+typedef double dmf4 __attribute__((ext_vector_type(4)));
+template <int NACC>
+__global__ void __launch_bounds__(256) k_mfma_f64(int iters, double *__restrict__ out)
+{
+    dmf4 acc[NACC];
+#pragma unroll
+    for (int a = 0; a < NACC; a++) acc[a] = dmf4{0.0, 0.0, 0.0, 0.0};
+    const double x = 1.0 + threadIdx.x * 1e-9, y = 1.0 - threadIdx.x * 1e-9;
+    for (int it = 0; it < iters; it++)
+    {
+#pragma unroll
+        for (int a = 0; a < NACC; a++)
+            acc[a] = __builtin_amdgcn_mfma_f64_16x16x4f64(x, y, acc[a], 0, 0, 0);
+    }
+    double sum = 0.0;
+#pragma unroll
+    for (int a = 0; a < NACC; a++) sum = sum + acc[a][0] + acc[a][1] + acc[a][2] + acc[a][3];
+    out[(size_t) blockIdx.x * blockDim.x + threadIdx.x] = sum;
+}
+
 } // namespace sdl
 
 extern "C" {
 #pragma GCC visibility push(default)
+
+int sdl_mfma_kernel(int nacc, int blocks, int iters, double *out, void *stream)
+{
+    if (!out || blocks < 1 || iters < 1 || (nacc != 1 && nacc != 2 && nacc != 4 && nacc != 8 &&
+                                            nacc != 16))
+    {
+        sdl::set_error("sdl_mfma_kernel: nacc 1, 2, 4, 8 or 16, blocks and iters >= 1");
+        return 1;
+    }
+    hipStream_t s = stream ? (hipStream_t) stream : sdl::stream();
+    switch (nacc)
+    {
+    case 1: hipLaunchKernelGGL(sdl::k_mfma_f64<1>, dim3(blocks), dim3(256), 0, s, iters, out); break;
+    case 2: hipLaunchKernelGGL(sdl::k_mfma_f64<2>, dim3(blocks), dim3(256), 0, s, iters, out); break;
+    case 4: hipLaunchKernelGGL(sdl::k_mfma_f64<4>, dim3(blocks), dim3(256), 0, s, iters, out); break;
+    case 8: hipLaunchKernelGGL(sdl::k_mfma_f64<8>, dim3(blocks), dim3(256), 0, s, iters, out); break;
+    default: hipLaunchKernelGGL(sdl::k_mfma_f64<16>, dim3(blocks), dim3(256), 0, s, iters, out);
+    }
+    SDL_HIP(hipGetLastError());
+    return 0;
+}
 
 const char *sdl_last_error(void) { return sdl::g_err; }
 
