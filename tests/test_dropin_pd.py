@@ -446,3 +446,43 @@ def test_spd_disjoint_columns_single_source_outputs():
     assert np.all(np.abs(x - ref[r, i]) <= 1e-13 * mag[r, i] + 1e-300)
     for M in (Cm, Ah):
         L.free_matrix(M)
+
+
+def test_block_refill_split_equals_refill():
+    """sdl_block_refill_device_split (events around the block products and
+    the whole refill) gives the same values as sdl_block_refill_device, and
+    its events are ordered (products end before the refill does)."""
+    import ctypes as C
+    lib = D.lib()
+    rng = np.random.default_rng(8)
+    nb, N = 12, 400
+    sizes = np.exp(rng.uniform(np.log(64), np.log(200), (nb, 2))).astype(int)
+    m = int(sizes[:, 0].sum())
+    rows = rng.permutation(m)
+    bl, start = [], 0
+    for b in range(nb):
+        rp = sorted(rows[start:start + sizes[b, 0]].tolist())
+        start += sizes[b, 0]
+        cp = sorted(rng.choice(N, sizes[b, 1], replace=False).tolist())
+        bl.append(PO.pd(m, N, rp, cp, rng.standard_normal((len(rp), len(cp)))))
+    Ah = make(PO.spd(m, N, bl))
+    Cm = lib.ATA_matrices_alloc(Ah)
+    d = rng.uniform(0.1, 10, m)
+    lib.ATDA_matrices_fill_values(Ah, D.dptr(d), Cm)  # binds the operand mirrors
+    from sparsediff_linalg_amd import DeviceArray
+    dd = DeviceArray.from_numpy(d)
+    nval = int(Cm.contents.nnz)
+    assert lib.sdl_block_refill_device(Cm, Ah, Ah, dd.ptr, None, None) == 0
+    lib.sdl_synchronize()
+    first = np.empty(nval)
+    lib.sdl_memcpy_dtoh(first.ctypes.data, lib.sdl_block_device_values(Cm), first.nbytes)
+    e0, eg, e1 = lib.sdl_event_create(), lib.sdl_event_create(), lib.sdl_event_create()
+    assert lib.sdl_block_refill_device_split(Cm, Ah, Ah, dd.ptr, e0, eg, e1) == 0
+    lib.sdl_synchronize()
+    second = np.empty(nval)
+    lib.sdl_memcpy_dtoh(second.ctypes.data, lib.sdl_block_device_values(Cm), second.nbytes)
+    assert np.array_equal(first.view(np.int64), second.view(np.int64))
+    t_products, t_all = lib.sdl_event_elapsed_ms(e0, eg), lib.sdl_event_elapsed_ms(e0, e1)
+    assert 0.0 <= t_products <= t_all
+    lib.free_matrix(Cm)
+    lib.free_matrix(Ah)
