@@ -209,7 +209,8 @@ int sdl_btda_plan_create_tuned(int m, int nA, int nB, const int *dAp, const int 
 /* The thresholds a plan was created with. */
 int sdl_btda_plan_tune(const sdl_btda_plan *plan, sdl_sym_tune *out);
 /* Tunes the thresholds on this product.  A setting's cost is plan creation
-   plus, with dAx and dBx given (both or neither), one refill; median of reps.
+   plus, with dAx and dBx given (both or neither), one refill; the least of
+   reps (allocation stalls only add time).
    Stage 1 sweeps hash_max x dens, stage 2 (row-accumulator plans, values
    given) rowacc_cut x rank_cut from stage 1's winner; a candidate replaces
    the incumbent (first the built-in values) only when at least 3 % cheaper.

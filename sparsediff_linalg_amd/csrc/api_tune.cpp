@@ -215,8 +215,10 @@ int sdl_btda_tune(int m, int nA, int nB, const int *dAp, const int *dAi, const i
     }
     if (reps < 1) reps = 1;
     const sdl::SymTune def;
-    // cost of a setting: plan creation plus (with values) one refill, median
-    // of reps; the structure must not move (checked before any refill)
+    // cost of a setting: plan creation plus (with values) one refill, the
+    // least of reps (device allocation and release only ever add time, and
+    // by widely varying amounts); the structure must not move (checked
+    // before any refill)
     long long nnz0 = -1;
     bool rowacc = false;
     sdl::DBuf<double> C;
@@ -244,10 +246,16 @@ int sdl_btda_tune(int m, int nA, int nB, const int *dAp, const int *dAi, const i
             sdl_btda_plan_destroy(pl);
             v.push_back(ms);
         }
-        std::sort(v.begin(), v.end());
-        *med = v[v.size() / 2];
+        *med = *std::min_element(v.begin(), v.end());
         return 0;
     };
+    {
+        // untimed warm-up: kernels loaded, allocator primed
+        sdl_btda_plan *pl = nullptr;
+        double ms = 0.0;
+        if (timed_create(m, nA, nB, dAp, dAi, dBp, dBi, mode, def, &pl, &ms)) return 1;
+        sdl_btda_plan_destroy(pl);
+    }
     // a candidate replaces the incumbent only when at least 3 % cheaper, so
     // timing noise does not move the table away from the built-in values
     const double MARGIN = 0.97;

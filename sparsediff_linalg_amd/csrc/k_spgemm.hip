@@ -1945,8 +1945,9 @@ __global__ void __launch_bounds__(G)
     k_num_rank_pos(const RankRow *__restrict__ rr, int nrows, const uint16_t *__restrict__ rpos,
                    const uint16_t *__restrict__ ipos, const int *__restrict__ Li,
                    const int *__restrict__ Lperm, const int *__restrict__ Ap,
-                   const double *__restrict__ Lval, const double *__restrict__ d,
-                   const double *__restrict__ Ax, double *__restrict__ Cx, int S)
+                   const int2 *__restrict__ qrange, const double *__restrict__ Lval,
+                   const double *__restrict__ d, const double *__restrict__ Ax,
+                   double *__restrict__ Cx, int S)
 {
     static_assert(CH <= G && CH % 64 == 0, "chunk of at most G L entries");
     extern __shared__ double stage[];  // S
@@ -1960,38 +1961,52 @@ __global__ void __launch_bounds__(G)
     RankRow cur = rr[ri];
     // the first chunk's L entry of this thread, for the current row: its A
     // row (k), that row's start and length, and L value * d_k
-    auto slot_k = [&](const RankRow &r, int &kq, int &lq) {
+    // (qrange, when given: the A-row range of every L entry, recorded by the
+    // plan, read in L order instead of gathered from Ap)
+    auto slot_k = [&](const RankRow &r, int &kq, int &lq, int2 &qr) {
         const int q = r.q0 + (int) threadIdx.x;
         kq = -1;
         lq = 0;
+        qr = int2{0, 0};
         if ((int) threadIdx.x < CH && q < r.q1)
         {
             kq = Li[q] + r.koff;
             lq = Lperm ? Lperm[q] : q;
+            if (qrange) qr = qrange[q];
         }
     };
-    auto slot_rest = [&](int kq, int lq, int &st, int &len, double &v) {
+    auto slot_rest = [&](int kq, int lq, int2 qr, int &st, int &len, double &v) {
         st = 0;
         len = 0;
         v = 0.0;
         if (kq >= 0)
         {
-            st = Ap[kq];
-            len = Ap[kq + 1] - st;
+            if (qrange)
+            {
+                st = qr.x;
+                len = qr.y;
+            }
+            else
+            {
+                st = Ap[kq];
+                len = Ap[kq + 1] - st;
+            }
             v = Lval[lq];
             if (d) v = v * d[kq];
         }
     };
     int pk, pl, pst, plen;
+    int2 pqr;
     double pv;
-    slot_k(cur, pk, pl);
-    slot_rest(pk, pl, pst, plen, pv);
+    slot_k(cur, pk, pl, pqr);
+    slot_rest(pk, pl, pqr, pst, plen, pv);
     for (; ri < nrows; ri += gridDim.x)
     {
         const int rn = ri + gridDim.x;
         RankRow nxt = cur;
         if (rn < nrows) nxt = rr[rn];
         int nk = -1, nlq = 0;
+        int2 nqr = int2{0, 0};
         bool pre_next = false;
         if (!cur.skip)
         {
@@ -2015,12 +2030,14 @@ __global__ void __launch_bounds__(G)
                     {
                         const int q = qb + (int) threadIdx.x;
                         int kq = -1, lq = 0;
+                        int2 qr = int2{0, 0};
                         if ((int) threadIdx.x < CH && q < cur.q1)
                         {
                             kq = Li[q] + cur.koff;
                             lq = Lperm ? Lperm[q] : q;
+                            if (qrange) qr = qrange[q];
                         }
-                        slot_rest(kq, lq, st, len, v);
+                        slot_rest(kq, lq, qr, st, len, v);
                         k = kq < 0 ? 0 : kq;
                     }
                     int tot;
@@ -2038,7 +2055,7 @@ __global__ void __launch_bounds__(G)
                     if (!pre_next && rn < nrows)
                     {
                         // first level of the next row's first chunk
-                        slot_k(nxt, nk, nlq);
+                        slot_k(nxt, nk, nlq, nqr);
                         pre_next = true;
                     }
                     const int nq = min(CH, cur.q1 - qb);
@@ -2053,7 +2070,9 @@ __global__ void __launch_bounds__(G)
                             const int fc = fi < tot ? fi : tot - 1;
                             int t = 0, slot = 0, kk = 0;
                             expand_term<CH>(ex, nq, fc, t, slot, kk);
-                            const int pr = rpos[tbase + fc];
+                            // positions are read once: kept out of L2's way
+                            // of the re-read A values
+                            const int pr = __builtin_nontemporal_load(rpos + tbase + fc);
                             x[u] = vq[slot] * Ax[t];
                             p[u] = fi < tot ? pr : -1;
                         }
@@ -2105,8 +2124,8 @@ __global__ void __launch_bounds__(G)
             }
         }
         if (rn >= nrows) break;
-        if (!pre_next) slot_k(nxt, nk, nlq);
-        slot_rest(nk, nlq, pst, plen, pv);
+        if (!pre_next) slot_k(nxt, nk, nlq, nqr);
+        slot_rest(nk, nlq, nqr, pst, plen, pv);
         pk = nk;
         cur = nxt;
     }
@@ -2124,6 +2143,17 @@ __global__ void k_rank_bin_sums(const int *__restrict__ rows, int n,
     out[i] = rterms[rows[i]];
     atomicAdd(nnz, (unsigned long long) rowcnt[rows[i]]);
     atomicMax(longest, (unsigned) rowcnt[rows[i]]);
+}
+
+// A-row range (start, length) of every L entry of a one-block L.
+__global__ void k_qrange(int64_t n, const int *__restrict__ Li, const int *__restrict__ Ap,
+                         int2 *__restrict__ out)
+{
+    const int64_t q = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    const int k = Li[q];
+    const int st = Ap[k];
+    out[q] = int2{st, Ap[k + 1] - st};
 }
 
 __global__ void k_zero_int(int *p, int n)
@@ -2370,7 +2400,7 @@ void product_row_features(const LView &L, const AView &A, float *F, hipStream_t 
 // LDS of k_num_rank_pos<RP_G, RP_CH, RP_U> beside its staging window, and the
 // window: as many entries as the rest of the CU's 160 KB LDS holds, or the
 // bin's longest row when shorter.
-constexpr int RP_G = 1024, RP_CH = 512, RP_U = 8;
+constexpr int RP_G = 1024, RP_CH = 256, RP_U = 8;
 constexpr int CU_LDS_BYTES = 160 * 1024;
 
 // Records the rank bin's term positions (ProductPlan::rankpos) when the bin's
@@ -2441,6 +2471,19 @@ void record_rank_positions(ProductPlan &P, const LView &L, const AView &A, const
     d2h(&nm, mcount.p, sizeof nm);
     SDL_REQUIRE(herr == 0 && (int64_t) nm <= mcap, "rank positions: item list overflow");
     P.nmulti = (int64_t) nm;
+    if (L.blocks == 1)
+    {
+        // every L entry's A-row range (one block: an L entry always meets
+        // the same A row)
+        int nle = 0;
+        d2h(&nle, L.Lp + L.Lm, sizeof nle);
+        const int64_t nL_entries = nle;
+        P.qrange.alloc((size_t) std::max<int64_t>(nL_entries, 1));
+        if (nL_entries > 0)
+            hipLaunchKernelGGL(k_qrange, dim3((unsigned) ((nL_entries + 255) / 256)), dim3(256),
+                               0, stream(), nL_entries, L.Li, A.Ap, P.qrange.p);
+        SDL_HIP(hipGetLastError());
+    }
     P.rrows.alloc(mid);
     hipLaunchKernelGGL(k_rank_rows, dim3((unsigned) ((mid + 255) / 256)), dim3(256), 0, stream(),
                        rows, mid, P.rskip.p, P.roff.p, P.moff.p, P.mcnt.p, L.Lm, L.Lp, nL,
@@ -2648,8 +2691,8 @@ void product_numeric(const ProductPlan &P, const double *Lval, const double *d,
                 // recorded positions; the spill list is the plan's
                 hipLaunchKernelGGL((k_num_rank_pos<RP_G, RP_CH, RP_U>), dim3(P.rank_grid), dim3(RP_G),
                                    (size_t) P.rank_window * sizeof(double), s, P.rrows.p, nmid,
-                                   P.rpos.p, P.ipos.p, P.L.Li, P.L.Lperm, P.A.Ap, Lval, d, Ax,
-                                   Cx, P.rank_window);
+                                   P.rpos.p, P.ipos.p, P.L.Li, P.L.Lperm, P.A.Ap, P.qrange.p,
+                                   Lval, d, Ax, Cx, P.rank_window);
             }
             else
             {

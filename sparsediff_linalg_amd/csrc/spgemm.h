@@ -147,6 +147,7 @@ struct ProductPlan
     DBuf<int> mcnt;                  // ... and number mcnt[ri]
     DBuf<uint16_t> ipos;
     DBuf<RankRow> rrows;             // the refill's per-row descriptors
+    DBuf<int2> qrange;               // one-block L: each L entry's A-row (start, length)
     SymStats stats;
     RefillCfg tuned;   // variant < 0: not tuned, use the default heuristic
     SymTune tune;      // thresholds the symbolic phase and the refill bins used
