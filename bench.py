@@ -208,6 +208,42 @@ def main():
     lib.sdl_synchronize()
     main_ms = float(np.mean([lib.sdl_event_elapsed_ms(a, b) for a, b in ev]))
 
+    # the same refills with the d vectors handed over k-major (SDL_BATCH_D_KMAJOR:
+    # refill r of row k at D[k * ldk + r]), which skips the d transposition;
+    # informative only -- value stays on the refill-major input above
+    kmajor = None
+    if not args.lean and best.variant >= 3:
+        ldk = (K + 15) // 16 * 16
+        pools_k = [DeviceArray.from_numpy(np.ascontiguousarray(
+            np.pad(P.numpy().T, ((0, 0), (0, ldk - K))))) for P in pools]
+        for i in range(args.warmup):
+            check(lib.sdl_btda_refill_batched_ex(h, dAx.ptr, dBx.ptr, pools_k[i % NPOOL].ptr, ldk,
+                                                 K, dC.ptr, ldc, 2, None, None, stream), "kmajor")
+        lib.sdl_event_record(e_all0, stream)
+        for i in range(args.steps):
+            check(lib.sdl_btda_refill_batched_ex(h, dAx.ptr, dBx.ptr, pools_k[i % NPOOL].ptr, ldk,
+                                                 K, dC.ptr, ldc, 2, None, None, stream), "kmajor")
+        lib.sdl_event_record(e_all1, stream)
+        lib.sdl_synchronize()
+        ms_k = lib.sdl_event_elapsed_ms(e_all0, e_all1) / args.steps
+        got = dC.numpy()[:, :nnz]
+        for r in (0, K - 1):
+            one = DeviceArray(max(nnz, 1), np.float64)
+            D_r = pools[(args.steps - 1) % NPOOL]
+            check(lib.sdl_btda_refill(h, dAx.ptr, dBx.ptr, D_r.offset(r * m), one.ptr, stream), "one")
+            lib.sdl_synchronize()
+            assert np.array_equal(one.numpy()[:nnz].view(np.int64), got[r].view(np.int64)), \
+                "k-major batched refill differs from the single refill"
+        kmajor = {"ms_per_step": ms_k, "nnz_out_per_s": K * nnz / (ms_k * 1e-3),
+                  "note": ("d vectors given k-major in 4 rotating 10 MB pools: no "
+                           "transposition, same values; every XCD's L2 re-reads the whole "
+                           "d, from HBM for a cold pool, where the library's transposed "
+                           "buffer is the same each step and stays in the MALL")}
+        del pools_k
+        # the refill-major steps ran last in the checks below: redo one
+        check(step(len(ev) - 1), "refill")
+        lib.sdl_synchronize()
+
     # correctness spot check of the last step against the single-refill path
     D_last = pools[(len(ev) - 1) % NPOOL]  # the event pass ran last
     out_batch = dC.numpy()[:, :nnz]
@@ -299,6 +335,7 @@ def main():
             "nnz_out_per_s": K * nnz / (plain_ms_batch * 1e-3),
         },
         "batched_vs_plain": plain_ms_batch / ms_step,
+        "d_kmajor": kmajor,
         "alloc_s": t_alloc,
         "cpu_baseline": cpu,
     }
