@@ -655,9 +655,19 @@ __device__ __forceinline__ double count_cost(int64_t cap, int64_t t, int nwords,
     return 8.0 * (512.0 + 2.0 * nwords + (double) t);
 }
 
+// Margins of the predicted sizing: a row is a candidate (features and a
+// network evaluation) when a certain prediction could cut its count cost to
+// PRED_CAND of the cost from its term bound, and is lowered when the expected
+// cost of the lowered count plus a possible recount is below PRED_LOWER of it.
+// Held-out graph products (R-MAT) showed the network over-confident far from
+// its training families -- most lowered rows overflowed -- so a lowering must
+// promise at least a halving, and rows promising less are not evaluated.
+constexpr double PRED_CAND = 0.25;
+constexpr double PRED_LOWER = 0.5;
+
 // Rows a prediction could help: those where even a certain prediction of the
-// smallest bin below the term bound would cost under 0.8 of the count from
-// the term bound.  Only they get features and a network evaluation; one
+// smallest bin below the term bound would cost under PRED_CAND of the count
+// from the term bound.  Only they get features and a network evaluation; one
 // atomic per wavefront appends them to list.
 __global__ void __launch_bounds__(256)
     k_pred_candidates(int64_t m, const int64_t *__restrict__ terms,
@@ -671,14 +681,14 @@ __global__ void __launch_bounds__(256)
         const int64_t t = terms[row], u = bound[row];
         if (u > mlp_bin_max(0))
             cand = count_cost(mlp_bin_max(0), t, nwords, lds_words, hash_max, dens) <
-                   0.8 * count_cost(u, t, nwords, lds_words, hash_max, dens);
+                   PRED_CAND * count_cost(u, t, nwords, lds_words, hash_max, dens);
     }
     const unsigned long long v = __ballot(cand);
     const int lane = threadIdx.x & 63;
     int base = 0;
     if (lane == 0 && v) base = atomicAdd(count, (int) __popcll(v));
     base = __shfl(base, 0, 64);
-    if (cand) list[base + __popcll(v & ((1ull << lane) - 1ull))] = (int) row;
+    if (cand && list) list[base + __popcll(v & ((1ull << lane) - 1ull))] = (int) row;
 }
 
 // Count-pass size bound of every row from the network's bin probabilities
@@ -686,7 +696,7 @@ __global__ void __launch_bounds__(256)
 // costs the count with that bound's accumulator, plus, with probability
 // P(nnz > bound_b) = 1 - (p_0 + ... + p_b), a second count from the term
 // bound.  The bound is lowered to the bin of least expected cost when that
-// cost is below 0.8 of the count from the term bound; otherwise the term
+// cost is below PRED_LOWER of the count from the term bound; otherwise the term
 // bound stays.  Uncertain rows (spread probabilities) and rows whose inserts
 // dominate (hub rows) keep their term bound.  pbin gets the tau-rule bin (the
 // smallest bin whose cumulative probability reaches tau) for the statistics.
@@ -730,7 +740,7 @@ __global__ void k_pred_cap(int64_t n, const int *__restrict__ list,
     if (u > 0)
     {
         const double e0 = count_cost(u, t, nwords, lds_words, hash_max, dens);
-        double best = 0.8 * e0;
+        double best = PRED_LOWER * e0;
         double below = 0.0;  // P(nnz <= bound_b)
         for (int b = 0; b < MLP_NBIN_BOUNDED; b++)
         {
@@ -755,12 +765,19 @@ __global__ void k_pred_cap(int64_t n, const int *__restrict__ list,
 // and a failed insert leaves it full, so a count within the bound is exact.
 __global__ void k_pred_recount(int64_t m, const int *__restrict__ rowcnt,
                                const int64_t *__restrict__ cap,
-                               const int64_t *__restrict__ bound, int64_t *__restrict__ recount)
+                               const int64_t *__restrict__ bound, int64_t *__restrict__ recount,
+                               unsigned long long *__restrict__ nrecount)
 {
     int64_t row = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
-    if (row >= m) return;
-    const int64_t u = bound[row];
-    recount[row] = (cap[row] < u && (int64_t) rowcnt[row] > cap[row]) ? u : 0;
+    bool again = false;
+    if (row < m)
+    {
+        const int64_t u = bound[row];
+        again = cap[row] < u && (int64_t) rowcnt[row] > cap[row];
+        recount[row] = again ? u : 0;
+    }
+    const unsigned long long v = __ballot(again);
+    if ((threadIdx.x & 63) == 0 && v) atomicAdd(nrecount, (unsigned long long) __popcll(v));
 }
 
 // Over non-empty rows: [0] counted within the predicted bound, [1] counted
@@ -2315,24 +2332,45 @@ void symbolic_predicted(ProductPlan &P, const RowPredictor *pred, const int64_t 
 {
     const int64_t m_out = P.m_out;
     const unsigned grid = (unsigned) ((m_out + 255) / 256);
-    DBuf<unsigned char> pbin(m_out);
-    DBuf<int64_t> cap(m_out), recount(m_out);
-    // every row starts from its term bound (the exact path's sizes); rows a
-    // prediction could help get features, the network and a cost decision
-    SDL_HIP(hipMemsetAsync(pbin.p, PBIN_NONE, (size_t) m_out, stream()));
-    d2d(cap.p, rbound, (size_t) m_out * sizeof(int64_t));
+    // rows a prediction could help: counted first, so that a product without
+    // any takes the exact path with nothing allocated for the prediction
+    int nc = 0;
     if (pred)
     {
-        DBuf<int> list(m_out), ncand(1);
+        DBuf<int> ncand(1);
+        dzero(ncand.p, sizeof(int));
+        hipLaunchKernelGGL(k_pred_candidates, dim3(grid), dim3(256), 0, stream(), m_out, rterms,
+                           rbound, nwords, lds_bitmap_words(), P.tune.hash_max, P.tune.dens,
+                           (int *) nullptr, ncand.p);
+        SDL_HIP(hipGetLastError());
+        d2h(&nc, ncand.p, sizeof(int));
+    }
+    P.stats.candidate_rows = nc;
+    if (nc == 0)
+    {
+        // no candidate: the exact path itself; every non-empty row kept its
+        // term bound (the count pass's classified rows)
+        symbolic_exact(P, rbound, rowcnt, nwords);
+        P.stats.predicted_rows = 0;
+        P.stats.overflow_rows = 0;
+        P.stats.misbin_rows = 0;
+        P.stats.exact_rows =
+            P.stats.count_hash_rows + P.stats.count_bitmap_rows + P.stats.count_global_rows;
+        return;
+    }
+    DBuf<unsigned char> pbin(m_out);
+    DBuf<int64_t> cap(m_out), recount(m_out);
+    // every row starts from its term bound (the exact path's sizes); the
+    // candidates get features, the network and a cost decision
+    SDL_HIP(hipMemsetAsync(pbin.p, PBIN_NONE, (size_t) m_out, stream()));
+    d2d(cap.p, rbound, (size_t) m_out * sizeof(int64_t));
+    {
+        DBuf<int> list(nc), ncand(1);
         dzero(ncand.p, sizeof(int));
         hipLaunchKernelGGL(k_pred_candidates, dim3(grid), dim3(256), 0, stream(), m_out, rterms,
                            rbound, nwords, lds_bitmap_words(), P.tune.hash_max, P.tune.dens,
                            list.p, ncand.p);
         SDL_HIP(hipGetLastError());
-        int nc = 0;
-        d2h(&nc, ncand.p, sizeof(int));
-        P.stats.candidate_rows = nc;
-        if (nc > 0)
         {
             DBuf<float> F((size_t) nc * MLP_IN), Z((size_t) nc * MLP_OUT);
             hipLaunchKernelGGL(k_row_features, dim3((unsigned) ((nc + 3) / 4)), dim3(256), 0,
@@ -2347,13 +2385,20 @@ void symbolic_predicted(ProductPlan &P, const RowPredictor *pred, const int64_t 
             SDL_HIP(hipGetLastError());
         }
     }
-    symbolic_count(P, cap.p, rowcnt, nwords, true);
-    hipLaunchKernelGGL(k_pred_recount, dim3(grid), dim3(256), 0, stream(), m_out, rowcnt, cap.p,
-                       rbound, recount.p);
-    SDL_HIP(hipGetLastError());
-    symbolic_count(P, recount.p, rowcnt, nwords, false);
-    symbolic_fill(P, rowcnt, nwords);
     DBuf<unsigned long long> st(4);
+    {
+        symbolic_count(P, cap.p, rowcnt, nwords, true);
+        DBuf<unsigned long long> nrec(1);
+        dzero(nrec.p, nrec.bytes());
+        hipLaunchKernelGGL(k_pred_recount, dim3(grid), dim3(256), 0, stream(), m_out, rowcnt,
+                           cap.p, rbound, recount.p, nrec.p);
+        SDL_HIP(hipGetLastError());
+        unsigned long long nr = 0;
+        d2h(&nr, nrec.p, sizeof nr);
+        // the second count only when a lowered bound was exceeded
+        if (nr > 0) symbolic_count(P, recount.p, rowcnt, nwords, false);
+        symbolic_fill(P, rowcnt, nwords);
+    }
     dzero(st.p, st.bytes());
     hipLaunchKernelGGL(k_pred_stats, dim3(grid), dim3(256), 0, stream(), m_out, cap.p, recount.p,
                        pbin.p, rbound, P.Cp.p, st.p);

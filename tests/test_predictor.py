@@ -122,6 +122,15 @@ def cases():
     # uniform columns in a wide space: rows a prediction may size (candidates)
     p, i, _ = W.csr_uniform(15, 1 << 18, 1 << 15, 8, 8, values=False)
     yield "uniform-wide", (p, i), None, 1 << 15, 1 << 15
+    # B^T A with B a dense 16 x 64 pattern and A 16 rows of 300 columns in
+    # 2^20: 64 output rows of 4800 terms and ~4800 entries whose term bound
+    # takes a 128 KB column bitmap, where a certain small prediction would
+    # count at a fifth of the cost (certain candidates; a small prediction for
+    # them overflows)
+    p, i, _ = W.csr_uniform(17, 16, 1 << 20, 300, 300, values=False)
+    bp = np.arange(0, 16 * 64 + 1, 64, dtype=np.int32)
+    bi = np.tile(np.arange(64, dtype=np.int32), 16)
+    yield "wide-bitmap", (p, i), (bp, bi), 1 << 20, 64
 
 
 @pytest.mark.parametrize("which", ["trained", "none", "adversarial"])
