@@ -573,11 +573,16 @@ def main_tune(args):
         tune_s = time.perf_counter() - t0
         print(f"[tune] {label}: tuner {tune_s:.1f} s -> {best}", file=sys.stderr, flush=True)
         default = tuning.default()
+        same = best == default
+        arms = (("built-in", default),) if same else (("built-in", default), ("tuned", best))
         runs = {"built-in": [], "tuned": []}
         dev = [DeviceArray.from_numpy(np.ascontiguousarray(a, np.int32)) for a in (p, i)]
         for _ in range(3):
-            for key, t in (("built-in", default), ("tuned", best)):
+            for key, t in arms:
                 runs[key].append(measure(p, i, n, t, dAx, dd, dev))
+        if same:
+            # the tuner kept the built-in thresholds: one arm, no ratio of noise
+            runs["tuned"] = runs["built-in"]
         # least of 3 interleaved runs (allocation stalls only add time)
         d_best = min(runs["built-in"], key=lambda r: r[0] + r[1])
         t_best_run = min(runs["tuned"], key=lambda r: r[0] + r[1])
@@ -588,7 +593,8 @@ def main_tune(args):
                "tuner_estimate_ms": {"tuned": t_best, "built_in": t_def},
                "built_in_ms": {"plan": d_best[0], "refill": d_best[1], "total": td},
                "tuned_ms": {"plan": t_best_run[0], "refill": t_best_run[1], "total": tt},
-               "gain": td / tt, "structure_same": True}
+               "gain": td / tt, "structure_same": True,
+               "tuner_kept_built_in": same}
         tot_def += td
         tot_tuned += tt
         results.append(rec)
