@@ -321,6 +321,10 @@ typedef struct sdl_hessian sdl_hessian;
 int sdl_hessian_create(int nnodes, int n, const int *m, const int *const *dAp,
                        const int *const *dAi, sdl_hessian **out);
 void sdl_hessian_destroy(sdl_hessian *h);
+/* Streams the node refills are forked over (1 to 4, default 4; 1: in order on
+   the caller's stream).  Forking pays when the nodes are too small to fill
+   the GPU alone; a later capture records the current setting. */
+int sdl_hessian_set_streams(sdl_hessian *h, int streams);
 long long sdl_hessian_nnz(const sdl_hessian *h);
 const long long *sdl_hessian_rowptr(const sdl_hessian *h);
 const int *sdl_hessian_colind(const sdl_hessian *h);

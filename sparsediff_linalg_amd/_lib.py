@@ -144,6 +144,7 @@ SIGNATURES = {
     "sdl_hessian_create": (C.c_int, [C.c_int, C.c_int, c_int_p, C.POINTER(vp), C.POINTER(vp),
                                      C.POINTER(vp)]),
     "sdl_hessian_destroy": (None, [vp]),
+    "sdl_hessian_set_streams": (C.c_int, [vp, C.c_int]),
     "sdl_hessian_nnz": (C.c_longlong, [vp]),
     "sdl_hessian_rowptr": (vp, [vp]),
     "sdl_hessian_colind": (vp, [vp]),

@@ -27,6 +27,7 @@ struct sdl_hessian
     // priority) and join back before the sum, so a captured graph has one
     // branch per stream
     static constexpr int NAUX = 4;
+    int streams = NAUX;  // node refills forked over this many streams (1: in order on s)
     hipStream_t aux[NAUX] = {nullptr};
     hipEvent_t fork = nullptr, join[NAUX] = {nullptr};
 };
@@ -59,8 +60,21 @@ int eval_launches(sdl_hessian *h, const double *const *dAx, const double *const 
                   double *dHx, double *dLx, hipStream_t s)
 {
     const int k = (int) h->nodes.size();
-    const int naux = std::min(k, (int) sdl_hessian::NAUX);
     std::vector<const double *> xs(k);
+    if (h->streams <= 1)
+    {
+        // every node refill in order on s
+        for (int j = 0; j < k; j++)
+        {
+            if (sdl_btda_refill(h->nodes[j], dAx[j], dAx[j], dd ? dd[j] : nullptr, h->Cx[j].p, s))
+                return 1;
+            xs[j] = h->Cx[j].p;
+        }
+        if (sdl_sum_fill(h->sum, xs.data(), nullptr, dHx, s)) return 1;
+        if (dLx && sdl_lower_coo_refresh(h->lower, dHx, dLx, s)) return 1;
+        return 0;
+    }
+    const int naux = std::min(k, h->streams);
     SDL_HIP(hipEventRecord(h->fork, s));
     for (int w = 0; w < naux; w++) SDL_HIP(hipStreamWaitEvent(h->aux[w], h->fork, 0));
     for (int j = 0; j < k; j++)
@@ -143,6 +157,17 @@ int sdl_hessian_create(int nnodes, int n, const int *m, const int *const *dAp,
 }
 
 void sdl_hessian_destroy(sdl_hessian *h) { release(h); }
+
+int sdl_hessian_set_streams(sdl_hessian *h, int streams)
+{
+    if (!h || streams < 1 || streams > sdl_hessian::NAUX)
+    {
+        sdl::set_error("sdl_hessian_set_streams: 1 to 4 streams");
+        return 1;
+    }
+    h->streams = streams;
+    return 0;
+}
 
 long long sdl_hessian_nnz(const sdl_hessian *h) { return h ? sdl_sum_plan_nnz(h->sum) : -1; }
 const long long *sdl_hessian_rowptr(const sdl_hessian *h)

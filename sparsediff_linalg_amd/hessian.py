@@ -31,6 +31,10 @@ class Hessian:
         self.nnz = lib.sdl_hessian_nnz(h)
         self.lower_nnz = lib.sdl_hessian_lower_nnz(h)
 
+    def set_streams(self, streams):
+        """Streams the node refills fork over (1 = in order on one stream)."""
+        check(self.lib.sdl_hessian_set_streams(self.h, int(streams)), "hessian streams")
+
     def _ptrs(self, arrs):
         return (C.c_void_p * self.k)(*[a.ptr for a in arrs]) if arrs is not None else None
 

@@ -66,3 +66,37 @@ def test_hessian_eval_and_graph_replays():
             assert bits_equal(Hx2.numpy()[:H.nnz], Ox), rep
     finally:
         H.close()
+
+
+@pytest.mark.parametrize("streams", [1, 2])
+def test_hessian_node_streams(streams):
+    """The node refills in order on one stream, or forked over two, give the
+    oracle's Hessian bit for bit, eagerly and replayed; an out-of-range stream
+    count is rejected."""
+    lib = load()
+    nodes, n = W.lagrangian_nodes(4, n=20_000, nnodes=5, m_lo=1_000, m_hi=5_000, window=6_000)
+    H = Hessian([(p, i) for p, i, _ in nodes], n)
+    try:
+        with pytest.raises(RuntimeError):
+            H.set_streams(0)
+        with pytest.raises(RuntimeError):
+            H.set_streams(5)
+        H.set_streams(streams)
+        rng = np.random.default_rng(13)
+        ds = [rng.uniform(0.1, 10, len(p) - 1) for p, _, _ in nodes]
+        Ax = [DeviceArray.from_numpy(x) for _, _, x in nodes]
+        D = [DeviceArray.from_numpy(d) for d in ds]
+        Hx = DeviceArray(max(H.nnz, 1), np.float64)
+        Lx = DeviceArray(max(H.lower_nnz, 1), np.float64)
+        _, _, Ox, _, _, lx = oracle_hessian(nodes, n, ds)
+        H.eval(Ax, D, Hx, Lx)
+        lib.sdl_synchronize()
+        assert bits_equal(Hx.numpy()[:H.nnz], Ox) and bits_equal(Lx.numpy()[:H.lower_nnz], lx)
+        Hx.copy_from(np.zeros(max(H.nnz, 1)))
+        H.capture(Ax, D, Hx, Lx)
+        for _ in range(2):
+            H.replay()
+            lib.sdl_synchronize()
+            assert bits_equal(Hx.numpy()[:H.nnz], Ox)
+    finally:
+        H.close()
