@@ -63,6 +63,9 @@ def parse():
     ap.add_argument("--rmat-scale", type=int, default=20,
                     help="learned: R-MAT scale of the A^T A evaluated beside the banded one")
     ap.add_argument("--epochs", type=int, default=100, help="learned: training epochs")
+    ap.add_argument("--coalesce-lists", action="store_true",
+                    help="blocks: coalesce the partials through per-cell pull lists "
+                         "instead of by output row")
     ap.add_argument("--gemm-slice", type=int, default=0,
                     help="blocks-*: reduction slice depth of the block GEMM (16 or 32; 0 = "
                          "library default)")
@@ -1588,6 +1591,8 @@ def main_blocks(args):
     lib = sdl.load()
     if args.gemm_slice:
         check(0 if lib.sdl_block_gemm_slice(args.gemm_slice) > 0 else 1, "gemm slice")
+    if args.coalesce_lists:
+        check(0 if lib.sdl_block_coalesce_mode(1) >= 0 else 1, "coalesce mode")
     w = workloads.stacked_pd_blocks(3)
     m, N = w["m"], w["n"]
     btda = args.workload == "blocks-btda"
@@ -1702,6 +1707,7 @@ def main_blocks(args):
             "gemm_tiles": int(stats[2]), "gemm_tasks": int(stats[3]),
             "coalesce_terms": int(stats[6]), "flops": flops, "alloc_s": t_alloc,
             "gemm_slice": args.gemm_slice or 16,
+            "coalesce": "pull lists" if args.coalesce_lists else "by output row",
             "dropin_fill_ms_incl_pcie": t_dropin * 1e3, "sampled_max_rel_err": worst,
         },
         "roofline": {

@@ -447,6 +447,14 @@ int sdl_block_publish(struct matrix *C);
 /* Reduction slice depth of the block-product GEMM (16 or 32, default 16);
    returns the previous depth, or -1 for an unsupported value. */
 int sdl_block_gemm_slice(int sk);
+/* How the next block-product allocs coalesce per-block partials into a
+   stacked_pd output (stacked_pd_coalesce.c:326-381 sums each cell over its
+   source blocks in ascending order, from +0.0): 0 (default) = by output row
+   where the output has at most 8192 columns and every partial's row and column
+   lists are distinct (a dense LDS row per output row, no per-cell lists),
+   else, or with 1, per-cell pull lists.  Both give the same values bit for bit.
+   Returns the previous mode, or -1 for an unknown one. */
+int sdl_block_coalesce_mode(int mode);
 
 /* ---- synthetic workload generators (host; deterministic in seed) ---- */
 /* Row lengths uniform in [lo, hi]; writes Ap[m+1]; returns nnz. */

@@ -586,7 +586,7 @@ matrix *pd_csr_operand_fresh(matrix *M) { return pd_csr_operand(M); }
 
 // ------------------------------------------------- signature coalescing ----
 Coalesced coalesce_blocks(int m, int n, const std::vector<Blk> &raw,
-                          const std::vector<long long> &raw_off)
+                          const std::vector<long long> &raw_off, bool lists)
 {
     Coalesced co;
     // blocks touching each row, ascending block id (stacked_pd_coalesce.c:101-125)
@@ -627,6 +627,7 @@ Coalesced coalesce_blocks(int m, int n, const std::vector<Blk> &raw,
         co.src_p.push_back((int) co.src_idx.size());
     }
     // per output cell, the raw cells feeding it in raw order (:326-371)
+    if (!lists) return co;
     co.seg.push_back(0);
     for (size_t s = 0; s < sigs.size(); s++)
     {
@@ -654,6 +655,87 @@ Coalesced coalesce_blocks(int m, int n, const std::vector<Blk> &raw,
             }
     }
     return co;
+}
+
+bool rowc_eligible(int m, int n, const std::vector<Blk> &raw)
+{
+    if (g_coalesce_mode != 0 || n <= 0 || n > ROWC_MAX_N) return false;
+    std::vector<int> seen_r(std::max(m, 1), -1), seen_c(n, -1);
+    for (size_t k = 0; k < raw.size(); k++)
+    {
+        for (int r : raw[k].rp)
+        {
+            if (r < 0 || r >= m || seen_r[r] == (int) k) return false;
+            seen_r[r] = (int) k;
+        }
+        for (int c : raw[k].cp)
+        {
+            if (c < 0 || c >= n || seen_c[c] == (int) k) return false;
+            seen_c[c] = (int) k;
+        }
+    }
+    return true;
+}
+
+// Output rows in output order (signature blocks in catalogue order, their
+// rows in row-scan order); each row's sources are its signature's raw blocks
+// in ascending order, source = that block's row holding r.
+void rowc_build(RowCoalesce &P, int m, int n, const std::vector<Blk> &raw,
+                const std::vector<long long> &raw_off, const Coalesced &co)
+{
+    // local row of every (raw block, global row) pair, by global row
+    std::vector<std::vector<std::pair<int, int>>> at(m);
+    for (size_t k = 0; k < raw.size(); k++)
+        for (int ii = 0; ii < raw[k].m0(); ii++) at[raw[k].rp[ii]].push_back({(int) k, ii});
+    std::vector<int> cpoff(raw.size());
+    std::vector<int> cpool;
+    for (size_t k = 0; k < raw.size(); k++)
+    {
+        cpoff[k] = (int) cpool.size();
+        cpool.insert(cpool.end(), raw[k].cp.begin(), raw[k].cp.end());
+    }
+    std::vector<long long> obase, sstart;
+    std::vector<int> sp{0}, slen, scp;
+    long long off = 0, cells = 0;
+    int longest = 1;
+    for (size_t s = 0; s < co.out.size(); s++)
+    {
+        const Blk &o = co.out[s];
+        for (int i = 0; i < o.m0(); i++)
+        {
+            obase.push_back(off + (long long) i * o.n0());
+            // at[r] is in ascending block order and holds exactly the
+            // signature's blocks (a block lists r at most once)
+            for (const auto &kb : at[o.rp[i]])
+            {
+                const Blk &b = raw[kb.first];
+                if (b.n0() == 0) continue;
+                sstart.push_back(raw_off[kb.first] + (long long) kb.second * b.n0());
+                slen.push_back(b.n0());
+                scp.push_back(cpoff[kb.first]);
+                cells += b.n0();
+                longest = std::max(longest, b.n0());
+            }
+            sp.push_back((int) slen.size());
+        }
+        off += o.size();
+    }
+    P.nrows = (int) obase.size();
+    P.ncolw = (n + 31) / 32 * 32;
+    P.h = 1;
+    while (256 * P.h < longest) P.h *= 2;
+    P.cells = cells;
+    if (P.nrows == 0) return;
+    P.obase = upload(obase.data(), (int64_t) obase.size());
+    P.sp = upload(sp.data(), (int64_t) sp.size());
+    if (!slen.empty())
+    {
+        P.sstart = upload(sstart.data(), (int64_t) sstart.size());
+        P.slen = upload(slen.data(), (int64_t) slen.size());
+        P.scp = upload(scp.data(), (int64_t) scp.size());
+    }
+    if (cpool.empty()) cpool.push_back(0);
+    P.cpool = upload(cpool.data(), (int64_t) cpool.size());
 }
 
 // -------------------------------------------------- structural operations ----

@@ -99,6 +99,8 @@ struct ProdB : Bound
     bool to_raw = false;
     DBuf<double> raw;
     PullList coalesce;
+    RowCoalesce rowc;              // used instead of the pull lists when by_row
+    bool by_row = false;
     DBuf<double> d;
     int d_len = 0;
     matrix *left_view = nullptr;   // when the left operand is read through a sparse view
@@ -395,15 +397,25 @@ matrix *finish(Program &pg, bool stack_out, int m, int n, const std::vector<Blk>
     }
     else
     {
-        Coalesced co = coalesce_blocks(m, n, raw, raw_off);
+        const bool by_row = rowc_eligible(m, n, raw);
+        Coalesced co = coalesce_blocks(m, n, raw, raw_off, !by_row);
         std::vector<PDM *> pds;
         for (const Blk &o : co.out) pds.push_back(new_pdm(o, nullptr));
         C = &new_spdm(m, n, pds, co.src_p, co.src_idx)->base;
         b->to_raw = true;
         b->raw.alloc(raw_total);
         b->raw_cells = raw_total;
-        b->coalesce_terms = (long long) co.idx.size();
-        pull_build(b->coalesce, co.seg, co.idx);
+        b->by_row = by_row;
+        if (by_row)
+        {
+            rowc_build(b->rowc, m, n, raw, raw_off, co);
+            b->coalesce_terms = b->rowc.cells;
+        }
+        else
+        {
+            b->coalesce_terms = (long long) co.idx.size();
+            pull_build(b->coalesce, co.seg, co.idx);
+        }
     }
     gemm_upload(b->gemm, pg.gb);
     b->sdmm = std::move(pg.sdmm);
@@ -439,7 +451,11 @@ void launch(ProdB *b, matrix *B, matrix *A, const double *dd, matrix *C, hipEven
         sdmm_run(j->plan, sv, dense, dd, target + j->out_off, stream());
     }
     if (e_gemm) SDL_HIP(hipEventRecord(e_gemm, stream()));
-    if (b->to_raw) pull_sum(b->coalesce, b->raw.p, out_buf(C), stream());
+    if (b->to_raw)
+    {
+        if (b->by_row) rowc_run(b->rowc, b->raw.p, out_buf(C), stream());
+        else pull_sum(b->coalesce, b->raw.p, out_buf(C), stream());
+    }
     if (e1) SDL_HIP(hipEventRecord(e1, stream()));
 }
 

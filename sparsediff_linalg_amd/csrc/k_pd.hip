@@ -59,6 +59,103 @@ __global__ void __launch_bounds__(256)
     for (int i = threadIdx.x; i < len; i += 256) out[o + i] = 0.0 + src[sidx + i];
 }
 
+// Exclusive prefix of v over the G threads of a workgroup: wave shuffles,
+// then the G / 64 wave totals through LDS (wsum[G / 64]).
+template <int G> __device__ __forceinline__ int block_excl_scan(int v, int *wsum)
+{
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    int x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1)
+    {
+        const int y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) wsum[wv] = x;
+    __syncthreads();
+    int base = 0;
+    for (int w = 0; w < wv; w++) base += wsum[w];
+    return base + x - v;
+}
+
+// Row coalesce (RowCoalesce in pd_kernels.h): one workgroup of G threads per
+// output row.  LDS: the dense row (ncolw doubles), the touched-column mask
+// (ncolw / 32 words), each mask word's output rank, the scan's wave totals.
+// NB sources are loaded at a time, H cells per thread per source, all loads
+// unconditional (clamped to the source's last cell) so they are in flight
+// together; the sources are then added in order, one barrier apart.
+template <int G, int NB, int H>
+__global__ __launch_bounds__(G) void k_coalesce_rows(
+    int ncolw, const long long *__restrict__ obase, const int *__restrict__ sp,
+    const long long *__restrict__ sstart, const int *__restrict__ slen,
+    const int *__restrict__ scp, const int *__restrict__ cpool, const double *__restrict__ raw,
+    double *__restrict__ out)
+{
+    extern __shared__ double rowc_lds[];
+    double *dense = rowc_lds;
+    const int nw = ncolw >> 5;
+    unsigned *mask = (unsigned *) (dense + ncolw);
+    int *wpre = (int *) (mask + nw);
+    int *wsum = wpre + nw;
+    const int t = threadIdx.x;
+    const int row = blockIdx.x;
+    for (int c = t; c < ncolw; c += G) dense[c] = 0.0;
+    for (int w = t; w < nw; w += G) mask[w] = 0u;
+    const int s0 = sp[row], s1 = sp[row + 1];
+    __syncthreads();
+    for (int sb = s0; sb < s1; sb += NB)
+    {
+        double v[NB][H];
+        int col[NB][H];
+        int len[NB];
+#pragma unroll
+        for (int u = 0; u < NB; u++)
+        {
+            const int s = min(sb + u, s1 - 1);
+            const long long base = sstart[s];
+            const int cpo = scp[s];
+            len[u] = slen[s];
+#pragma unroll
+            for (int h = 0; h < H; h++)
+            {
+                const int j = min(t + G * h, len[u] - 1);
+                v[u][h] = raw[base + j];
+                col[u][h] = cpool[cpo + j];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < NB; u++)
+        {
+            if (sb + u < s1)  // the same for every thread
+            {
+#pragma unroll
+                for (int h = 0; h < H; h++)
+                {
+                    if (t + G * h < len[u])
+                    {
+                        const int c = col[u][h];
+                        dense[c] = dense[c] + v[u][h];
+                        atomicOr(&mask[c >> 5], 1u << (c & 31));
+                    }
+                }
+                __syncthreads();
+            }
+        }
+    }
+    // output rank of every mask word (nw <= 256 <= G: one word per thread)
+    const int pc = t < nw ? __popc(mask[t]) : 0;
+    const int ex = block_excl_scan<G>(pc, wsum);
+    if (t < nw) wpre[t] = ex;
+    __syncthreads();
+    const long long ob = obase[row];
+    for (int c = t; c < ncolw; c += G)
+    {
+        const unsigned mw = mask[c >> 5];
+        const unsigned bit = 1u << (c & 31);
+        if (mw & bit) out[ob + wpre[c >> 5] + __popc(mw & (bit - 1u))] = dense[c];
+    }
+}
+
 // one wavefront per output list, lanes over the dense dimension
 __global__ void __launch_bounds__(256)
     k_sdmm(int NL, int NC, const int *__restrict__ lp, const int *__restrict__ dr,
@@ -358,6 +455,46 @@ void pull_sum(const PullList &P, const double *src, double *out, hipStream_t s)
     SDL_HIP(hipGetLastError());
 }
 
+int g_coalesce_mode = 0;
+
+namespace {
+template <int G, int NB, int H>
+void rowc_launch(const RowCoalesce &P, const double *raw, double *out, size_t lds, hipStream_t s)
+{
+    static bool attr = false;
+    auto fn = k_coalesce_rows<G, NB, H>;
+    if (!attr)
+    {
+        SDL_HIP(hipFuncSetAttribute((const void *) fn, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int) (ROWC_MAX_N * 8 + ROWC_MAX_N / 4 + 64)));
+        attr = true;
+    }
+    hipLaunchKernelGGL(fn, dim3((unsigned) P.nrows), dim3(G), lds, s, P.ncolw, P.obase.p,
+                       P.sp.p, P.sstart.p, P.slen.p, P.scp.p, P.cpool.p, raw, out);
+}
+} // namespace
+
+void rowc_run(const RowCoalesce &P, const double *raw, double *out, hipStream_t s)
+{
+    if (P.nrows <= 0) return;
+    SDL_REQUIRE(P.ncolw > 0 && P.ncolw <= ROWC_MAX_N && P.ncolw % 32 == 0,
+                "row coalesce: column count out of range");
+    // dense row, mask, word ranks, up to 16 wave totals
+    const size_t lds = (size_t) P.ncolw * 8 + (size_t) (P.ncolw / 32) * 8 + 64;
+    // P.h: 256-cell chunks of the longest source row
+    switch (P.h)
+    {
+    case 1:
+    case 2: rowc_launch<512, 16, 1>(P, raw, out, lds, s); break;
+    case 4: rowc_launch<512, 8, 2>(P, raw, out, lds, s); break;
+    case 8: rowc_launch<512, 4, 4>(P, raw, out, lds, s); break;
+    case 16: rowc_launch<512, 2, 8>(P, raw, out, lds, s); break;
+    case 32: rowc_launch<512, 1, 16>(P, raw, out, lds, s); break;
+    default: SDL_REQUIRE(false, "row coalesce: source rows too long");
+    }
+    SDL_HIP(hipGetLastError());
+}
+
 void sdmm_build(SdmmPlan &P, const std::vector<int> &lp, const std::vector<int> &dr,
                 const std::vector<int> &vp, const std::vector<int> &dk)
 {
@@ -506,6 +643,16 @@ int sdl_block_gemm_slice(int sk)
     if (sk != 16 && sk != 32) return -1;
     const int prev = sdl::g_gemm_slice;
     sdl::g_gemm_slice = sk;
+    return prev;
+}
+// Coalesce of block-product partials chosen at alloc: 0 = by output row
+// where it applies, 1 = per-cell pull lists; returns the previous mode, or -1
+// for an unknown one.
+int sdl_block_coalesce_mode(int mode)
+{
+    if (mode != 0 && mode != 1) return -1;
+    const int prev = sdl::g_coalesce_mode;
+    sdl::g_coalesce_mode = mode;
     return prev;
 }
 #pragma GCC visibility pop

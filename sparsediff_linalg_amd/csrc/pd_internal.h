@@ -90,8 +90,17 @@ struct Coalesced
     std::vector<int> src_p, src_idx;
     std::vector<int> seg, idx;   // pull lists over the output's block-major cells
 };
+// lists = false leaves seg / idx empty (the row coalesce needs neither).
 Coalesced coalesce_blocks(int m, int n, const std::vector<Blk> &raw,
-                          const std::vector<long long> &raw_off);
+                          const std::vector<long long> &raw_off, bool lists = true);
+
+// Whether the row coalesce applies to these raw blocks: n <= ROWC_MAX_N, and
+// every raw block's row and column lists hold distinct in-range indices.
+bool rowc_eligible(int m, int n, const std::vector<Blk> &raw);
+// The row coalesce of raw blocks into co's output blocks (co from
+// coalesce_blocks on the same raw blocks).
+void rowc_build(RowCoalesce &P, int m, int n, const std::vector<Blk> &raw,
+                const std::vector<long long> &raw_off, const Coalesced &co);
 
 // helpers
 bool has_overlap(const std::vector<int> &a, const std::vector<int> &b);

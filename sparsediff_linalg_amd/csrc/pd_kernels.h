@@ -29,6 +29,34 @@ struct PullList
 void pull_build(PullList &P, const std::vector<int> &seg, const std::vector<int> &idx);
 void pull_sum(const PullList &P, const double *src, double *out, hipStream_t s);
 
+// --------------------------------------------------------- row coalesce ----
+// The ordered coalesce of raw partial blocks by output row, with no per-cell
+// lists: output row r's sources are the rows of the raw blocks touching r, in
+// ascending block order, each a contiguous run of the block's n0 cells whose
+// global columns are the block's column list.  One workgroup per output row
+// adds them source by source into a dense LDS row over the n global columns,
+// from +0.0, marking the columns touched, and writes the touched columns in
+// ascending order: the row of its signature block, whose columns are the
+// sorted union of those lists.  Each cell is therefore the same left fold in
+// ascending block order as the pull lists give.  Needs n <= ROWC_MAX_N and
+// distinct row and column lists within every raw block.
+constexpr int ROWC_MAX_N = 8192;
+struct RowCoalesce
+{
+    int nrows = 0;              // output rows (block-major output order)
+    int ncolw = 0;              // n rounded up to a multiple of 32
+    int h = 1;                  // 256-cell chunks of the longest source row
+    DBuf<long long> obase;      // output offset of each row
+    DBuf<int> sp;               // nrows + 1: source range of each row
+    DBuf<long long> sstart;     // raw offset of each source row
+    DBuf<int> slen, scp;        // its cell count and column-list offset in cpool
+    DBuf<int> cpool;            // every raw block's column list
+    long long cells = 0;        // raw cells read per coalesce
+};
+void rowc_run(const RowCoalesce &P, const double *raw, double *out, hipStream_t s);
+// 0: the row coalesce where it applies (default), 1: always the pull lists.
+extern int g_coalesce_mode;
+
 // ---------------------------------------------------------------- sdmm ----
 // Output list L (a row or a column of the output block) sums its entries in
 // order: entry t = (dense index dr, sparse value position vp, d index dk or -1).
