@@ -271,6 +271,22 @@ def main():
     lib.sdl_synchronize()
     plain_ms_batch = lib.sdl_event_elapsed_ms(e_all0, e_all1) / max(args.plain_steps, 1)
 
+    # write floor: the refill's store pattern over the same output rows with
+    # constant values (no reads), the least time any refill of this batch can
+    # take on this GPU; plain / floor bounds the batched-over-plain ratio
+    floor_ms = None
+    if not args.lean:
+        for groups in (1, 8):
+            for _ in range(3):
+                check(lib.sdl_write_rows_kernel(dC.ptr, ldc, K, nnz, groups, stream), "floor")
+            lib.sdl_event_record(e_all0, stream)
+            for _ in range(20):
+                check(lib.sdl_write_rows_kernel(dC.ptr, ldc, K, nnz, groups, stream), "floor")
+            lib.sdl_event_record(e_all1, stream)
+            lib.sdl_synchronize()
+            t = lib.sdl_event_elapsed_ms(e_all0, e_all1) / 20
+            floor_ms = t if floor_ms is None else min(floor_ms, t)
+
     if args.calib_copy_mb > 0:
         nb = args.calib_copy_mb << 20
         src, dst = DeviceArray(nb // 8, np.float64), DeviceArray(nb // 8, np.float64)
@@ -338,6 +354,13 @@ def main():
             "nnz_out_per_s": K * nnz / (plain_ms_batch * 1e-3),
         },
         "batched_vs_plain": plain_ms_batch / ms_step,
+        "write_floor": None if floor_ms is None else {
+            "us": floor_ms * 1e3, "GB_s": K * nnz * 8 / (floor_ms * 1e-3) / 1e9,
+            "step_over_floor": ms_step / floor_ms,
+            "max_batched_vs_plain": plain_ms_batch / floor_ms,
+            "note": ("k_write_rows: the refill's K x nnz FP64 output stored in its own pattern "
+                     "with constant values and no reads; a batched refill cannot beat it, so "
+                     "plain / floor bounds batched_vs_plain on this GPU")},
         "d_kmajor": kmajor,
         "alloc_s": t_alloc,
         "cpu_baseline": cpu,
