@@ -271,7 +271,7 @@ const int *sdl_sum_plan_colind(const sdl_sum_plan *plan);
    back (the reference's bound index maps, matrix_sum.c:90-110). */
 const int *sdl_sum_plan_map(const sdl_sum_plan *plan);
 /* 1 when the unscaled fill streams the operand values through per-entry operand
-   masks (at most 8 operands, each with sorted unique columns per row); 0 when
+   masks (at most 16 operands, each with sorted unique columns per row); 0 when
    it reads them through the entry -> operand-entry lists. */
 int sdl_sum_plan_is_masked(const sdl_sum_plan *plan);
 /* dCx[nnz] = the sum; dX: host array of nops device value pointers.  dD (device,

@@ -16,6 +16,7 @@
 #include <climits>
 #include <cstdlib>
 #include <cstring>
+#include <type_traits>
 
 #include "../../include/sdl_device.h"
 
@@ -141,19 +142,22 @@ __global__ __launch_bounds__(256) void k_sum_fill_src(int64_t nnz, const int *__
 // it) and, per SUM_TILE-entry tile (one wave's), each operand's count before
 // the tile; the in-tile counts come from wave ballots.  Plan bytes: 1 per entry instead of
 // 4 per entry + 4 per operand entry, and the value reads stream.
-constexpr int SUM_MASK_OPS = 8;
-constexpr int SUM_PER = 4;                  // entries per lane
-constexpr int SUM_TILE = 64 * SUM_PER;      // entries per wave
+// Up to 8 operands: one mask byte per entry, 4 entries per lane; 9 to 16:
+// two mask bytes per entry, 2 entries per lane (the lane's loads stay at 32).
+constexpr int SUM_MASK_OPS = 16;
+constexpr int sum_per(int nops) { return nops <= 8 ? 4 : 2; }       // entries per lane
+constexpr int sum_tile(int nops) { return 64 * sum_per(nops); }     // entries per wave
 
+template <typename MT>
 __global__ void k_sum_masks(int64_t nnz, const int *__restrict__ seg,
                             const unsigned *__restrict__ order, const int64_t *__restrict__ off,
-                            int nops, unsigned char *__restrict__ mask)
+                            int nops, MT *__restrict__ mask)
 {
     int64_t e = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= nnz) return;
     unsigned mk = 0;
     for (int s = seg[e]; s < seg[e + 1]; s++) mk |= 1u << op_of(off, nops, order[s]);
-    mask[e] = (unsigned char) mk;
+    mask[e] = (MT) mk;
 }
 
 // bad[k] = 1 when two consecutive entries of operand k do not land in strictly
@@ -168,14 +172,14 @@ __global__ void k_sum_monotone(int64_t items, const int *__restrict__ map,
 }
 
 // base[k * ntiles + tile] = operand k's entries in output entries before
-// tile * SUM_TILE (first j with map_k[j] >= tile * SUM_TILE; map_k increases).
-__global__ void k_sum_tile_base(int ntiles, int nops, const int *__restrict__ map,
+// tile * tsize (first j with map_k[j] >= tile * tsize; map_k increases).
+__global__ void k_sum_tile_base(int ntiles, int tsize, int nops, const int *__restrict__ map,
                                 const int64_t *__restrict__ off, int *__restrict__ base)
 {
     int64_t idx = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= (int64_t) ntiles * nops) return;
     const int k = (int) (idx / ntiles), tile = (int) (idx % ntiles);
-    const int64_t e0 = (int64_t) tile * SUM_TILE;
+    const int64_t e0 = (int64_t) tile * tsize;
     int64_t lo = off[k], hi = off[k + 1];
     while (lo < hi)
     {
@@ -186,7 +190,7 @@ __global__ void k_sum_tile_base(int ntiles, int nops, const int *__restrict__ ma
     base[idx] = (int) (lo - off[k]);
 }
 
-// One wave per tile of SUM_TILE entries: SUM_PER rows of 64, lane i holding
+// One wave per tile of PER * 64 entries: PER rows of 64, lane i holding
 // entry i of each (every load and store coalesced), no LDS and no barrier.
 // Cx[e] = ((+0.0 + x_k0) + x_k1) + ... over the operands in e's mask, ascending
 // k.  NOPS is the operand count, so the loops have no branches and every value
@@ -195,9 +199,9 @@ __global__ void k_sum_tile_base(int ntiles, int nops, const int *__restrict__ ma
 // lane are in flight together.
 // SCALED: each term is the rounded product d[k*m + row] * x, the row read from
 // erow (one int per entry).
-template <int NOPS, bool SCALED>
+template <int NOPS, bool SCALED, typename MT, int PER>
 __global__ __launch_bounds__(256) void k_sum_fill_mask(int64_t nnz, int ntiles,
-                                                      const unsigned char *__restrict__ mask,
+                                                      const MT *__restrict__ mask,
                                                       const int *__restrict__ base,
                                                       const SumSrc src,
                                                       const int *__restrict__ erow,
@@ -207,18 +211,18 @@ __global__ __launch_bounds__(256) void k_sum_fill_mask(int64_t nnz, int ntiles,
     const int lane = threadIdx.x & 63;
     const int tile = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (threadIdx.x >> 6));
     if (tile >= ntiles) return;  // whole waves
-    const int64_t t0 = (int64_t) tile * SUM_TILE + lane;
+    const int64_t t0 = (int64_t) tile * (64 * PER) + lane;
     const unsigned long long lt = (1ull << lane) - 1ull;
     const double *dummy = (const double *) mask;
-    unsigned mk[SUM_PER];
+    unsigned mk[PER];
 #pragma unroll
-    for (int u = 0; u < SUM_PER; u++) mk[u] = mask[t0 + 64 * u];  // padded to whole tiles
+    for (int u = 0; u < PER; u++) mk[u] = mask[t0 + 64 * u];  // padded to whole tiles
     int run[NOPS];
 #pragma unroll
     for (int k = 0; k < NOPS; k++) run[k] = base[(int64_t) k * ntiles + tile];
-    double v[SUM_PER][NOPS];
+    double v[PER][NOPS];
 #pragma unroll
-    for (int u = 0; u < SUM_PER; u++)
+    for (int u = 0; u < PER; u++)
     {
 #pragma unroll
         for (int k = 0; k < NOPS; k++)
@@ -233,7 +237,7 @@ __global__ __launch_bounds__(256) void k_sum_fill_mask(int64_t nnz, int ntiles,
     if (SCALED)
     {
 #pragma unroll
-        for (int u = 0; u < SUM_PER; u++)
+        for (int u = 0; u < PER; u++)
         {
             const int64_t e = t0 + 64 * u;
             const int64_t r = erow[e < nnz ? e : nnz - 1];
@@ -242,7 +246,7 @@ __global__ __launch_bounds__(256) void k_sum_fill_mask(int64_t nnz, int ntiles,
         }
     }
 #pragma unroll
-    for (int u = 0; u < SUM_PER; u++)
+    for (int u = 0; u < PER; u++)
     {
         double acc = 0.0;
 #pragma unroll
@@ -255,15 +259,18 @@ __global__ __launch_bounds__(256) void k_sum_fill_mask(int64_t nnz, int ntiles,
 
 template <int NOPS>
 void launch_fill_mask(unsigned blocks, hipStream_t st, int64_t nnz, int ntiles,
-                      const unsigned char *mask, const int *base, const SumSrc &src,
+                      const void *mask, const int *base, const SumSrc &src,
                       const int *erow, const double *d, int m, double *Cx)
 {
+    typedef typename std::conditional<(NOPS <= 8), unsigned char, unsigned short>::type MT;
+    constexpr int PER = sum_per(NOPS);
+    const MT *mk = (const MT *) mask;
     if (d)
-        hipLaunchKernelGGL((k_sum_fill_mask<NOPS, true>), dim3(blocks), dim3(256), 0, st, nnz,
-                           ntiles, mask, base, src, erow, d, m, Cx);
+        hipLaunchKernelGGL((k_sum_fill_mask<NOPS, true, MT, PER>), dim3(blocks), dim3(256), 0,
+                           st, nnz, ntiles, mk, base, src, erow, d, m, Cx);
     else
-        hipLaunchKernelGGL((k_sum_fill_mask<NOPS, false>), dim3(blocks), dim3(256), 0, st, nnz,
-                           ntiles, mask, base, src, erow, d, m, Cx);
+        hipLaunchKernelGGL((k_sum_fill_mask<NOPS, false, MT, PER>), dim3(blocks), dim3(256), 0,
+                           st, nnz, ntiles, mk, base, src, erow, d, m, Cx);
 }
 
 // erow[e] = row of output entry e (the largest r with Cp[r] <= e).
@@ -348,7 +355,7 @@ struct SumPlan
     std::vector<int> dup;           // operand k repeats an entry
     bool masked = false;            // the operand-mask fill applies
     int ntiles = 0;
-    DBuf<unsigned char> mask;       // nnz operand masks (masked plans)
+    DBuf<unsigned char> mask;       // operand masks, 1 byte (<= 8 operands) or 2 per entry
     DBuf<int> tile_base;            // nops x ntiles (masked plans)
     DBuf<int> erow;                 // row of every output entry (masked plans keeping rows)
     DBuf<const double *> xs;        // value pointers beyond SUM_ARGS operands
@@ -402,15 +409,24 @@ void sum_plan_build(SumPlan &S, int nops, int m, int n, const int *const *dP,
     }
     if (S.masked)
     {
-        S.ntiles = (int) ((S.P.nnz + SUM_TILE - 1) / SUM_TILE);
-        // padded to whole tiles (zero masks), read as 32-bit words
-        S.mask.alloc((size_t) S.ntiles * SUM_TILE);
+        const int tsize = sum_tile(nops);
+        const size_t mbytes = nops <= 8 ? 1 : 2;
+        S.ntiles = (int) ((S.P.nnz + tsize - 1) / tsize);
+        // padded to whole tiles (zero masks)
+        S.mask.alloc((size_t) S.ntiles * tsize * mbytes);
         dzero(S.mask.p, S.mask.bytes());
         S.tile_base.alloc((size_t) S.ntiles * nops);
-        hipLaunchKernelGGL(k_sum_masks, dim3(sum_grid(S.P.nnz)), dim3(256), 0, stream(), S.P.nnz,
-                           S.seg.p, S.P.order.p, S.doff.p, nops, S.mask.p);
+        if (mbytes == 1)
+            hipLaunchKernelGGL(k_sum_masks<unsigned char>, dim3(sum_grid(S.P.nnz)), dim3(256), 0,
+                               stream(), S.P.nnz, S.seg.p, S.P.order.p, S.doff.p, nops,
+                               S.mask.p);
+        else
+            hipLaunchKernelGGL(k_sum_masks<unsigned short>, dim3(sum_grid(S.P.nnz)), dim3(256),
+                               0, stream(), S.P.nnz, S.seg.p, S.P.order.p, S.doff.p, nops,
+                               (unsigned short *) S.mask.p);
         hipLaunchKernelGGL(k_sum_tile_base, dim3(sum_grid((int64_t) S.ntiles * nops)), dim3(256),
-                           0, stream(), S.ntiles, nops, S.map.p, S.doff.p, S.tile_base.p);
+                           0, stream(), S.ntiles, tsize, nops, S.map.p, S.doff.p,
+                           S.tile_base.p);
         SDL_HIP(hipGetLastError());
     }
     if (S.masked && keep_rows)
@@ -451,6 +467,14 @@ void sum_plan_fill(SumPlan &S, const double *const *hX, const double *dd, double
             case 6: fn = &launch_fill_mask<6>; break;
             case 7: fn = &launch_fill_mask<7>; break;
             case 8: fn = &launch_fill_mask<8>; break;
+            case 9: fn = &launch_fill_mask<9>; break;
+            case 10: fn = &launch_fill_mask<10>; break;
+            case 11: fn = &launch_fill_mask<11>; break;
+            case 12: fn = &launch_fill_mask<12>; break;
+            case 13: fn = &launch_fill_mask<13>; break;
+            case 14: fn = &launch_fill_mask<14>; break;
+            case 15: fn = &launch_fill_mask<15>; break;
+            case 16: fn = &launch_fill_mask<16>; break;
             default: break;
             }
             fn(blocks, st, S.P.nnz, S.ntiles, S.mask.p, S.tile_base.p, src, S.erow.p, dd, S.m,

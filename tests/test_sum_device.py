@@ -96,10 +96,11 @@ def test_sum_plan_fill_scaled_and_add():
     S.close()
 
 
-@pytest.mark.parametrize("nops", [1, 3, 8])
+@pytest.mark.parametrize("nops", [1, 3, 8, 9, 12, 16])
 def test_sum_plan_operand_masks(nops):
     """Canonical operands (sorted unique columns), one empty: the fill streams
-    values through per-entry operand masks; many 256-entry tiles."""
+    values through per-entry operand masks (one byte up to 8 operands, two
+    bytes up to 16); many 256- or 128-entry tiles."""
     lib = load()
     rng = np.random.default_rng(40 + nops)
     m, n = 5000, 3000
@@ -112,6 +113,20 @@ def test_sum_plan_operand_masks(nops):
     dd = DeviceArray.from_numpy(d)
     check(lib.sdl_sum_fill(S.h, S.dX, dd.ptr, S.out.ptr, None), "scaled fill")
     assert bits_equal(S.values(), O.sum_n(m, n, ops, d=d)[2])
+    S.close()
+
+
+def test_sum_plan_17_operands_use_lists():
+    """Beyond 16 operands the masks do not apply: the entry lists sum the same
+    way."""
+    lib = load()
+    rng = np.random.default_rng(47)
+    m, n = 900, 700
+    ops = make_ops(rng, m, n, 17)
+    S = DevSum(m, n, ops, keep_rows=0)
+    assert not lib.sdl_sum_plan_is_masked(S.h)
+    check(lib.sdl_sum_fill(S.h, S.dX, None, S.out.ptr, None), "fill")
+    assert bits_equal(S.values(), O.sum_n(m, n, ops)[2])
     S.close()
 
 
