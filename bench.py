@@ -270,6 +270,20 @@ def main():
     lib.sdl_event_record(e_all1, stream)
     lib.sdl_synchronize()
     plain_ms_batch = lib.sdl_event_elapsed_ms(e_all0, e_all1) / max(args.plain_steps, 1)
+    # the same per-call path with the per-term-gather kernel (three indices and
+    # three gathers per term) instead of the default tile refill
+    plain_gather_ms = None
+    if args.plain_steps > 0 and lib.sdl_btda_plan_single_refill(h, 0) == 1:
+        lib.sdl_event_record(e_all0, stream)
+        for i in range(args.plain_steps):
+            D = pools[i % NPOOL]
+            for r in range(K):
+                lib.sdl_btda_refill(h, dAx.ptr, dBx.ptr, D.offset(r * m), dC.offset(r * ldc),
+                                    stream)
+        lib.sdl_event_record(e_all1, stream)
+        lib.sdl_synchronize()
+        plain_gather_ms = lib.sdl_event_elapsed_ms(e_all0, e_all1) / args.plain_steps
+        lib.sdl_btda_plan_single_refill(h, 1)
 
     # write floor: the refill's store pattern over the same output rows with
     # constant values (no reads), the least time any refill of this batch can
@@ -352,6 +366,11 @@ def main():
             "ms_per_K_refills": plain_ms_batch,
             "us_per_refill": plain_ms_batch * 1e3 / K,
             "nnz_out_per_s": K * nnz / (plain_ms_batch * 1e-3),
+            "kernel": "k_num_pull_tile (one launch per d)",
+            "per_term_gathers_us_per_refill": (None if plain_gather_ms is None
+                                               else plain_gather_ms * 1e3 / K),
+            "batched_vs_per_term_gathers": (None if plain_gather_ms is None
+                                            else plain_gather_ms / ms_step),
         },
         "batched_vs_plain": plain_ms_batch / ms_step,
         "write_floor": None if floor_ms is None else {

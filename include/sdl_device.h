@@ -182,6 +182,14 @@ int sdl_btda_autotune(sdl_btda_plan *plan, const double *dAx, const double *dBx,
                       int reps, sdl_refill_cfg *best, float *best_us);
 /* Pin a configuration (NULL: back to the default heuristic). */
 int sdl_btda_set_refill_cfg(sdl_btda_plan *plan, const sdl_refill_cfg *cfg);
+/* Kernel of the plan's single refills (sdl_btda_refill, one d; pull plans):
+   1 = tile refill (each 256-entry tile forms the scale factor L value * d[k]
+   of its rows' L entries once, then reads one L index and one A value per
+   term; the default when available: a pull schedule over a single-block L),
+   0 = per-term gathers (three indices and three gathers per term), -1 = query.
+   Same values bit for bit.  Returns the previous kind (0 / 1), or -1 when
+   kind 1 is not available or kind is unknown. */
+int sdl_btda_plan_single_refill(sdl_btda_plan *plan, int kind);
 
 /* ---- Tuning table (no counterpart in the reference, whose symbolic and
    numeric phases are one fixed CPU algorithm, src/sparse/gustavson.c:20-150)

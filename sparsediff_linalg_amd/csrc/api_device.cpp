@@ -334,6 +334,30 @@ int sdl_btda_set_refill_cfg(sdl_btda_plan *plan, const sdl_refill_cfg *cfg)
     return 0;
 }
 
+int sdl_btda_plan_single_refill(sdl_btda_plan *plan, int kind)
+{
+    if (!plan) return -1;
+    sdl::ProductPlan &P = plan->P;
+    const int prev = P.tile_refill ? 1 : 0;
+    if (kind == 0) P.tile_refill = false;
+    else if (kind == 1)
+    {
+        if (!P.pull || !P.tq.p)
+        {
+            sdl::set_error("sdl_btda_plan_single_refill: no tile refill for this plan "
+                           "(row-accumulator schedule or block-diagonal L)");
+            return -1;
+        }
+        P.tile_refill = true;
+    }
+    else if (kind != -1)
+    {
+        sdl::set_error("sdl_btda_plan_single_refill: kind must be -1, 0 or 1");
+        return -1;
+    }
+    return prev;
+}
+
 // ---- COO coalescing on device-resident triplets ----
 struct sdl_coo_plan
 {

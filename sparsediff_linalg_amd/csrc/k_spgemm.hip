@@ -837,8 +837,8 @@ __global__ void __launch_bounds__(G)
             const int *__restrict__ Ap, const int *__restrict__ Ai,
             const int64_t *__restrict__ Cp, const int *__restrict__ Ci,
             const int64_t *__restrict__ toff, int64_t *__restrict__ ep, int *__restrict__ tA,
-            int *__restrict__ tB, int *__restrict__ tK, int *__restrict__ gscratch,
-            int *__restrict__ err)
+            int *__restrict__ tB, int *__restrict__ tK, int *__restrict__ tq,
+            int *__restrict__ gscratch, int *__restrict__ err)
 {
     __shared__ int s_cols[CAP > 0 ? CAP : 1];
     __shared__ int s_off[CAP > 0 ? CAP : 1];
@@ -942,6 +942,8 @@ __global__ void __launch_bounds__(G)
             tA[j + 1] = aa;
             tB[j + 1] = bb;
         }
+        if (tq)
+            for (int64_t i = s0; i < s1; i++) tq[i] = tK[i];
         for (int64_t i = s0; i < s1; i++) tK[i] = Li[tK[i]] + c.koff;
     }
 }
@@ -966,6 +968,80 @@ __global__ void __launch_bounds__(256)
     {
         double v = Lval[tB[s]];
         if (d) v = v * d[tK[s]];
+        acc = acc + v * Ax[tA[s]];
+    }
+    Cx[e] = acc;
+}
+
+// L entries [qr.x, qr.y) of the output rows holding the entries of 256-entry
+// tile t (single-block L: consecutive rows own consecutive L entries).
+__global__ void k_tile_qrange(int ntiles, int64_t nnz, int64_t m_out,
+                              const int64_t *__restrict__ Cp, const int *__restrict__ Lp,
+                              int2 *__restrict__ qr)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ntiles) return;
+    const int64_t e0 = (int64_t) t * 256, e1 = min(nnz, e0 + 256) - 1;
+    // row of entry e: the last r with Cp[r] <= e
+    auto row_of = [&](int64_t e) {
+        int64_t lo = 0, hi = m_out - 1;
+        while (lo < hi)
+        {
+            const int64_t mid = (lo + hi + 1) >> 1;
+            if (Cp[mid] <= e) lo = mid;
+            else hi = mid - 1;
+        }
+        return lo;
+    };
+    const int64_t r0 = row_of(e0), r1 = row_of(e1);
+    qr[t] = make_int2(Lp[r0], Lp[r1 + 1]);
+}
+
+// Tile refill of a single-block pull plan: one workgroup per 256 entries.
+// The scale factor of every L entry q of the tile's rows, Lval[Lperm[q]] *
+// d[Li[q]], is formed once into LDS (when there are at most TS_CAP of them;
+// else per term); each entry then folds its terms in order, scale(tq) *
+// Ax[tA] from +0.0 -- k_num_pull's products and fold, with one index and one
+// A value read per term instead of three indices and three gathers.
+constexpr int TS_CAP = 1024;
+__global__ void __launch_bounds__(256)
+    k_num_pull_tile(int64_t nnz, const int2 *__restrict__ tqr, const int64_t *__restrict__ ep,
+                    const int *__restrict__ tq, const int *__restrict__ tA,
+                    const int *__restrict__ Li, const int *__restrict__ Lperm,
+                    const double *__restrict__ Lval, const double *__restrict__ d,
+                    const double *__restrict__ Ax, double *__restrict__ Cx)
+{
+    __shared__ double sc[TS_CAP];
+    const int2 qr = tqr[blockIdx.x];
+    const int nq = qr.y - qr.x;
+    const bool table = nq <= TS_CAP;  // the same for the whole workgroup
+    const int64_t e = (int64_t) blockIdx.x * 256 + threadIdx.x;
+    const bool live = e < nnz;
+    // this entry's term range, loaded before the table is built
+    const int64_t s0 = live ? ep[e] : 0, s1 = live ? ep[e + 1] : 0;
+    if (table)
+    {
+        for (int j = threadIdx.x; j < nq; j += 256)
+        {
+            const int q = qr.x + j;
+            double v = Lval[Lperm ? Lperm[q] : q];
+            if (d) v = v * d[Li[q]];
+            sc[j] = v;
+        }
+    }
+    __syncthreads();
+    if (!live) return;
+    double acc = 0.0;
+    for (int64_t s = s0; s < s1; s++)
+    {
+        const int q = tq[s];
+        double v;
+        if (table) v = sc[q - qr.x];
+        else
+        {
+            v = Lval[Lperm ? Lperm[q] : q];
+            if (d) v = v * d[Li[q]];
+        }
         acc = acc + v * Ax[tA[s]];
     }
     Cx[e] = acc;
@@ -2619,6 +2695,7 @@ void product_plan(ProductPlan &P, const LView &L, const AView &A, SymMode mode,
         P.tA.alloc(P.terms);
         P.tB.alloc(P.terms);
         P.tK.alloc(P.terms);
+        if (L.blocks == 1) P.tq.alloc(P.terms);  // the tile refill's L indices
         // rows binned by nnz: LDS lists up to 2048 entries, global beyond
         DBuf<int> err(1);
         dzero(err.p, err.bytes());
@@ -2633,18 +2710,20 @@ void product_plan(ProductPlan &P, const LView &L, const AView &A, SymMode mode,
             if (c == 1)
                 hipLaunchKernelGGL((k_terms<64, 128>), dim3(nrows), dim3(64), 0, stream(), rows,
                                    L.Lm, L.Lp, L.Li, L.Lperm, nL, A.Ap, A.Ai, P.Cp.p, P.Ci.p,
-                                   toff.p, P.ep.p, P.tA.p, P.tB.p, P.tK.p, (int *) nullptr, err.p);
+                                   toff.p, P.ep.p, P.tA.p, P.tB.p, P.tK.p, P.tq.p, (int *) nullptr,
+                                   err.p);
             else if (c == 2)
                 hipLaunchKernelGGL((k_terms<256, 2048>), dim3(nrows), dim3(256), 0, stream(),
                                    rows, L.Lm, L.Lp, L.Li, L.Lperm, nL, A.Ap, A.Ai, P.Cp.p,
-                                   P.Ci.p, toff.p, P.ep.p, P.tA.p, P.tB.p, P.tK.p,
+                                   P.Ci.p, toff.p, P.ep.p, P.tA.p, P.tB.p, P.tK.p, P.tq.p,
                                    (int *) nullptr, err.p);
             else
             {
                 gscratch.alloc(P.nnz);
                 hipLaunchKernelGGL((k_terms<256, 0>), dim3(nrows), dim3(256), 0, stream(), rows,
                                    L.Lm, L.Lp, L.Li, L.Lperm, nL, A.Ap, A.Ai, P.Cp.p, P.Ci.p,
-                                   toff.p, P.ep.p, P.tA.p, P.tB.p, P.tK.p, gscratch.p, err.p);
+                                   toff.p, P.ep.p, P.tA.p, P.tB.p, P.tK.p, P.tq.p, gscratch.p,
+                                   err.p);
             }
             SDL_HIP(hipGetLastError());
         }
@@ -2653,6 +2732,15 @@ void product_plan(ProductPlan &P, const LView &L, const AView &A, SymMode mode,
         int herr = 0;
         d2h(&herr, err.p, sizeof(int));
         SDL_REQUIRE(herr == 0, "product plan: a term fell outside its row's structure");
+        if (P.tq.p)
+        {
+            const int ntiles = (int) ((P.nnz + 255) / 256);
+            P.tqr.alloc(ntiles);
+            hipLaunchKernelGGL(k_tile_qrange, dim3((unsigned) ((ntiles + 255) / 256)), dim3(256), 0,
+                               stream(), ntiles, P.nnz, m_out, P.Cp.p, L.Lp, P.tqr.p);
+            SDL_HIP(hipGetLastError());
+            P.tile_refill = true;
+        }
     }
     else
     {
@@ -2702,7 +2790,13 @@ void product_numeric(const ProductPlan &P, const double *Lval, const double *d,
                      const double *Ax, double *Cx, hipStream_t s)
 {
     if (P.nnz == 0) return;
-    if (P.pull)
+    if (P.pull && P.tile_refill && P.tq.p)
+    {
+        hipLaunchKernelGGL(k_num_pull_tile, dim3((unsigned) ((P.nnz + 255) / 256)), dim3(256), 0,
+                           s, P.nnz, P.tqr.p, P.ep.p, P.tq.p, P.tA.p, P.L.Li, P.L.Lperm, Lval, d,
+                           Ax, Cx);
+    }
+    else if (P.pull)
     {
         hipLaunchKernelGGL(k_num_pull, dim3((unsigned) ((P.nnz + 255) / 256)), dim3(256), 0, s,
                            P.nnz, P.ep.p, P.tA.p, P.tB.p, P.tK.p, Lval, d, Ax, Cx);

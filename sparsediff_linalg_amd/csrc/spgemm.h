@@ -116,6 +116,13 @@ struct ProductPlan
     bool pull = false;
     DBuf<int64_t> ep;
     DBuf<int> tA, tB, tK;
+    // Tile refill (single-block L): each term's L index q (tB = Lperm[q], tK =
+    // Li[q]), and per 256-entry tile the L entries of its rows, so a single
+    // refill forms each L entry's scale factor once per tile.  tile_refill:
+    // single refills use it (else the per-term gathers).
+    DBuf<int> tq;
+    DBuf<int2> tqr;
+    bool tile_refill = false;
     // Row-accumulator schedule (otherwise): rows binned by nnz -- [0, 1) up to
     // 1024 entries (LDS accumulators), [1, 2) up to 65536 (by rank when the
     // column bitmap fits LDS), [2, 3) longer.

@@ -187,3 +187,57 @@ def test_every_refill_configuration_bit_exact(K):
                 assert bits_equal(X[r], ref[r]), (cfg.variant, cfg.rchunk, r)
     finally:
         P.close()
+
+
+@pytest.mark.parametrize("case", ["uniform", "dense_col", "empty", "no_d"])
+def test_single_refill_kernels_agree(case):
+    """The tile refill (scale factors formed once per tile; tiles of more than
+    1024 L entries gather per term) and the per-term-gather refill give the
+    oracle's values bit for bit; the query and the argument checks."""
+    lib = load()
+    rng = np.random.default_rng(61)
+    if case == "dense_col":   # L rows of ~1500 entries: tiles beyond the table
+        m, nA, nB = 5000, 3000, 30
+        Ap, Ai, Ax = random_csr(rng, m, nA, 0.001)
+        Bp, Bi, Bx = random_csr(rng, m, nB, 0.3, dense_col=0)
+    elif case == "empty":     # empty rows and columns on both sides
+        m, nA, nB = 3000, 900, 800
+        Ap, Ai, Ax = random_csr(rng, m, nA, 0.004, empty_row=4, empty_col=9)
+        Bp, Bi, Bx = random_csr(rng, m, nB, 0.004, empty_row=7, empty_col=3)
+    else:
+        m, nA, nB = 8000, 2000, 1500
+        Ap, Ai, Ax = random_csr(rng, m, nA, 0.003)
+        Bp, Bi, Bx = random_csr(rng, m, nB, 0.003)
+    d = None if case == "no_d" else rng.uniform(0.1, 10.0, m)
+    Cp, Ci, Cx, terms = O.btda(m, nA, nB, Ap, Ai, Ax, Bp, Bi, Bx,
+                               d if d is not None else np.ones(m))
+    P = DevPlan(m, nA, nB, Ap, Ai, Bp, Bi)
+    try:
+        assert lib.sdl_btda_plan_is_pull(P.h) == 1
+        assert lib.sdl_btda_plan_single_refill(P.h, -1) == 1   # the default
+        tile = P.refill(Ax, Bx, d)
+        assert lib.sdl_btda_plan_single_refill(P.h, 0) == 1
+        gath = P.refill(Ax, Bx, d)
+        assert lib.sdl_btda_plan_single_refill(P.h, -1) == 0
+        if d is not None:
+            assert bits_equal(tile, Cx), f"max ulp {max_ulp(tile, Cx)}"
+        assert bits_equal(tile, gath)
+        assert lib.sdl_btda_plan_single_refill(P.h, 5) == -1
+        assert lib.sdl_btda_plan_single_refill(P.h, 1) == 0
+    finally:
+        P.close()
+
+
+def test_single_refill_kind_needs_pull():
+    lib = load()
+    rng = np.random.default_rng(11)
+    m, nA, nB = 400, 12, 10
+    Ap, Ai, Ax = random_csr(rng, m, nA, 0.6)
+    Bp, Bi, Bx = random_csr(rng, m, nB, 0.6)
+    P = DevPlan(m, nA, nB, Ap, Ai, Bp, Bi)
+    try:
+        assert not lib.sdl_btda_plan_is_pull(P.h)
+        assert lib.sdl_btda_plan_single_refill(P.h, -1) == 0
+        assert lib.sdl_btda_plan_single_refill(P.h, 1) == -1
+    finally:
+        P.close()
