@@ -161,7 +161,10 @@ int sdl_btda_refill_batched_ex(const sdl_btda_plan *plan, const double *dAx, con
    with 1 / 2 entries per thread; 3 = interleaved (needs the reservation),
    4 / 5 = interleaved with 2 / 4 entries per thread, 6 = interleaved reading
    the operand values through the term lists (unbound values only; bound
-   calls use 3). */
+   calls use 3), 7 = interleaved tile form: per 256-entry tile the scale
+   factors L value * d[k] of the tile rows' L entries for the rchunk refills
+   formed once in LDS, then one L index and one A value per term (unbound
+   values, a single-block L; else 3). */
 typedef struct sdl_refill_cfg
 {
     int variant;

@@ -1404,6 +1404,102 @@ __global__ void __launch_bounds__(256)
     __builtin_nontemporal_store(out, C + e);
 }
 
+// Tile form of the interleaved batched refill (variant 7): grid over (entry
+// tile, chunk of RC refills) as k_num_pull_interleaved.  The workgroup first
+// forms, for every L entry q of its tile's rows, the RC scale factors
+// Lval[Lperm[q]] * d_r[Li[q]] (one aligned run of DT per L entry) into LDS;
+// each term then reads its L index and its A value and adds scale * A value
+// for each of the RC refills -- the products and the fold of
+// k_num_pull_interleaved, with d read per L entry rather than per term and no
+// operand-pair array.  Tiles of more than TB_CAP L entries form each term's
+// factors from DT directly.
+constexpr int TB_CAP = 256;
+template <int RC>
+__global__ void __launch_bounds__(256)
+    k_num_pull_tile_batched(int64_t nnz, int ntiles, int nchunks, const int2 *__restrict__ tqr,
+                            const int64_t *__restrict__ ep, const int *__restrict__ tq,
+                            const int *__restrict__ tA, const int *__restrict__ Li,
+                            const int *__restrict__ Lperm, const double *__restrict__ Lval,
+                            const double *__restrict__ Ax, const double *__restrict__ DT, int ldt,
+                            int K, double *__restrict__ C, int64_t ldc)
+{
+    __shared__ dvec2 tab[TB_CAP * (RC / 2)];  // tab[j * RC/2 + h]: refills r0+2h, r0+2h+1
+    const int b = blockIdx.x;
+    const int xcd = b & 7, sl = b >> 3;
+    const int tile = (sl / nchunks) * 8 + xcd;
+    const int chunk = sl % nchunks;
+    if (tile >= ntiles) return;  // the whole workgroup
+    const int r0 = chunk * RC;
+    const int2 qr = tqr[tile];
+    const int nq = qr.y - qr.x;
+    const bool table = nq <= TB_CAP;  // the same for the whole workgroup
+    const int64_t e = (int64_t) tile * 256 + threadIdx.x;
+    const bool live = e < nnz;
+    const int64_t s0 = live ? ep[e] : 0, s1 = live ? ep[e + 1] : 0;
+    if (table)
+    {
+        for (int j = threadIdx.x; j < nq; j += 256)
+        {
+            const int q = qr.x + j;
+            const double lv = Lval[Lperm ? Lperm[q] : q];
+            const dvec2 *dk = (const dvec2 *) (DT + (int64_t) Li[q] * ldt + r0);
+#pragma unroll
+            for (int h = 0; h < RC / 2; h++)
+            {
+                const dvec2 dv = dk[h];
+                dvec2 t;
+                t.x = lv * dv.x;
+                t.y = lv * dv.y;
+                tab[j * (RC / 2) + h] = t;
+            }
+        }
+    }
+    __syncthreads();
+    if (!live) return;
+    double out[RC];
+#pragma unroll
+    for (int r = 0; r < RC; r++) out[r] = 0.0;
+    for (int64_t s = s0; s < s1; s++)
+    {
+        const int q = tq[s];
+        const double a = Ax[tA[s]];
+        if (table)
+        {
+            const dvec2 *t = tab + (q - qr.x) * (RC / 2);
+#pragma unroll
+            for (int h = 0; h < RC / 2; h++)
+            {
+                const dvec2 f = t[h];
+                out[2 * h] = out[2 * h] + f.x * a;
+                out[2 * h + 1] = out[2 * h + 1] + f.y * a;
+            }
+        }
+        else
+        {
+            const double lv = Lval[Lperm ? Lperm[q] : q];
+            const dvec2 *dk = (const dvec2 *) (DT + (int64_t) Li[q] * ldt + r0);
+#pragma unroll
+            for (int h = 0; h < RC / 2; h++)
+            {
+                const dvec2 dv = dk[h];
+                out[2 * h] = out[2 * h] + (lv * dv.x) * a;
+                out[2 * h + 1] = out[2 * h + 1] + (lv * dv.y) * a;
+            }
+        }
+    }
+    const int rn = min(RC, K - r0);
+    double *cp = C + (int64_t) r0 * ldc + e;
+    if (rn == RC)
+    {
+#pragma unroll
+        for (int r = 0; r < RC; r++) __builtin_nontemporal_store(out[r], cp + (int64_t) r * ldc);
+    }
+    else
+    {
+        for (int r = 0; r < rn; r++) __builtin_nontemporal_store(out[r], cp + (int64_t) r * ldc);
+    }
+}
+
 typedef double dvec4 __attribute__((ext_vector_type(4)));
 
 // Interleaved batched refill with EPT adjacent entries per thread: one
@@ -2912,13 +3008,16 @@ void product_numeric_batched(const ProductPlan &P, const double *Lval, const dou
     // direct operand gathers need the caller's value arrays: bound values use
     // the pair array
     if (c.variant == 6 && (opts & BATCH_VALUES_BOUND)) c.variant = 3;
+    // the tile form needs the plan's L indices and the caller's value arrays
+    if (c.variant == 7 && (!P.tq.p || !P.tqr.p || (opts & BATCH_VALUES_BOUND))) c.variant = 3;
     if (c.variant >= 3)
     {
         const int rc = c.rchunk == 8 ? 8 : 16;
         const int ldt = (int) wld;
         // operand pairs (unless bound) and d transposed to k-major (unless
         // given so), in one launch
-        const bool gather = c.variant != 6 && !(opts & BATCH_VALUES_BOUND) && P.terms > 0;
+        const bool gather = c.variant != 6 && c.variant != 7 && !(opts & BATCH_VALUES_BOUND) &&
+                            P.terms > 0;
         const bool transpose = !(opts & BATCH_D_KMAJOR);
         const int ngather = gather ? (int) ((P.terms + 255) / 256) : 0;
         const int ntrans = transpose ? (int) (((P.A.Am + TR_K - 1) / TR_K) *
@@ -2934,6 +3033,24 @@ void product_numeric_batched(const ProductPlan &P, const double *Lval, const dou
         const double *DT = transpose ? dTp : D;
         const int ldk = transpose ? ldt : (int) ldd;
         if (ev_start) SDL_HIP(hipEventRecord((hipEvent_t) ev_start, s));
+        if (c.variant == 7)
+        {
+            const int ntiles = (int) ((P.nnz + 255) / 256);
+            const int nchunks = (K + rc - 1) / rc;
+            const int64_t blocks = ((int64_t) ntiles + 7) / 8 * 8 * nchunks;
+            if (rc == 8)
+                hipLaunchKernelGGL(k_num_pull_tile_batched<8>, dim3((unsigned) blocks), dim3(256),
+                                   0, s, P.nnz, ntiles, nchunks, P.tqr.p, P.ep.p, P.tq.p, P.tA.p,
+                                   P.L.Li, P.L.Lperm, Lval, Ax, DT, ldk, K, Cx, ldc);
+            else
+                hipLaunchKernelGGL(k_num_pull_tile_batched<16>, dim3((unsigned) blocks),
+                                   dim3(256), 0, s, P.nnz, ntiles, nchunks, P.tqr.p, P.ep.p,
+                                   P.tq.p, P.tA.p, P.L.Li, P.L.Lperm, Lval, Ax, DT, ldk, K, Cx,
+                                   ldc);
+            SDL_HIP(hipGetLastError());
+            if (ev_stop) SDL_HIP(hipEventRecord((hipEvent_t) ev_stop, s));
+            return;
+        }
         if (c.variant == 6)
         {
             const int ntiles = (int) ((P.nnz + 255) / 256);
@@ -3029,8 +3146,10 @@ RefillCfg default_refill_cfg(const ProductPlan &P, int K)
     RefillCfg c;
     if (K <= P.batch_cap)
     {
-        c.variant = 3;
-        c.rchunk = 16;
+        // the tile form where the plan has its L indices (bound calls fall
+        // back to the operand pairs)
+        c.variant = P.tq.p ? 7 : 3;
+        c.rchunk = P.tq.p ? 8 : 16;
         return c;
     }
     c.variant = 2;
@@ -3084,7 +3203,7 @@ std::vector<RefillCfg> refill_candidates()
             c.rchunk = rc;
             v.push_back(c);
         }
-    for (int var = 3; var <= 6; var++)
+    for (int var = 3; var <= 7; var++)
         for (int rc : {8, 16})
         {
             c.variant = var;
