@@ -172,7 +172,7 @@ def main():
     best = sdl_refill_cfg()
     best_us = C.c_float()
     if args.lean:
-        best.variant, best.rchunk = 3, 8
+        best.variant, best.rchunk = 7, 8
         check(lib.sdl_btda_set_refill_cfg(h, C.byref(best)), "set cfg")
         args.plain_steps = 0
     else:
@@ -391,6 +391,8 @@ def main():
 
 
 def refill_kernel_name(cfg):
+    if cfg.variant == 7:
+        return "k_prep_batch (d transposition) + k_num_pull_tile_batched"
     if cfg.variant >= 3:
         return "k_prep_batch (operand pairs + d transposition) + k_num_pull_interleaved"
     return "k_num_pull_rchunk / k_num_pull_batched (direct gathers)"
