@@ -52,6 +52,8 @@ def parse():
     ap.add_argument("--rounds", type=int, default=50,
                     help="multi: fresh d vectors per instance (refill rounds in the graph)")
     ap.add_argument("--instances", type=int, default=4096, help="multi: number of instances")
+    ap.add_argument("--multi-layout", default="tile", choices=["tile", "pairs"],
+                    help="multi: bound values in tile layout or as operand pairs")
     ap.add_argument("--multi-k", type=int, default=8,
                     help="multi: refill rounds per launch (even; output ring of this many slots)")
     ap.add_argument("--workload", default="btda",
@@ -1526,7 +1528,7 @@ def main_multi(args):
     w = workloads.multi_instances(4, ninst=args.instances)
     t_gen = time.perf_counter() - t0
     t0 = time.perf_counter()
-    mp = MultiProduct(w, K=K)
+    mp = MultiProduct(w, K=K, tile=args.multi_layout == "tile")
     lib.sdl_synchronize()
     t_sym = time.perf_counter() - t0
     M = mp.M
@@ -1601,10 +1603,13 @@ def main_multi(args):
                          "instance (each round a fresh d, overwriting C)"),
             "instances": args.instances, "rows": M, "nnz_out": mp.nnz, "terms": mp.terms,
             "groups": len(mp.g), "rounds": R, "rounds_per_launch": K, "gen_s": t_gen,
-            "symbolic_s": t_sym, "tuned": tuned,
+            "symbolic_s": t_sym, "tuned": tuned, "bound_layout": args.multi_layout,
         },
         "roofline": {
-            "bound": "hbm", "kernel": "interleaved refill, K rounds per launch per group",
+            "bound": "hbm",
+            "kernel": ("k_num_pull_tile_batched (bound tile layout)" if args.multi_layout == "tile"
+                       else "k_num_pull_interleaved (bound operand pairs)")
+                      + ", K rounds per launch per group",
             "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": None, "alg_bytes_per_launch": bytes_R,
         },

@@ -138,6 +138,14 @@ int sdl_btda_refill_range(const sdl_btda_plan *plan, const double *dAx, const do
    16*terms bytes).  Batched refills with K <= Kmax then use the interleaved
    kernel; larger K fall back to direct gathers.  Allocates; refills do not. */
 int sdl_btda_plan_reserve(sdl_btda_plan *plan, int Kmax);
+/* As sdl_btda_plan_reserve, with the bound values (sdl_btda_bind_values) kept
+   in tile layout instead of operand pairs: each term's A value (8 bytes per
+   term) and each L entry's value (8 bytes per entry of B), so bound refills run
+   the tile form of the interleaved refill (variant 7) reading them without an
+   index; no pair array is allocated (8*m*Kmax + 8*terms + 8*nnz(B) bytes).
+   Needs a pull schedule (sdl_btda_plan_is_pull) over a single-block L; returns
+   1 otherwise.  Same values bit for bit. */
+int sdl_btda_plan_reserve_tile(sdl_btda_plan *plan, int Kmax);
 
 /* Gather the term operand pairs of the current A / B values into the
    reserved workspace (call again whenever A or B values change). */

@@ -211,6 +211,7 @@ SIGNATURES = {
     "sdl_block_publish": (C.c_int, [vp]),
     "sdl_block_gemm_slice": (C.c_int, [C.c_int]),
     "sdl_btda_plan_single_refill": (C.c_int, [vp, C.c_int]),
+    "sdl_btda_plan_reserve_tile": (C.c_int, [vp, C.c_int]),
     "sdl_block_coalesce_mode": (C.c_int, [C.c_int]),
     # generators
     "sdl_gen_row_lengths_uniform": (C.c_longlong, [C.c_uint64, C.c_int, C.c_int, C.c_int,

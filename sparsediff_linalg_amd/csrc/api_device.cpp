@@ -219,6 +219,9 @@ int sdl_btda_autotune(sdl_btda_plan *plan, const double *dAx, const double *dBx,
     sdl::RefillCfg best_c = sdl::default_refill_cfg(plan->P, K);
     for (const sdl::RefillCfg &c : sdl::refill_candidates())
     {
+        // without an operand-pair array (tile reservation) the pair-reading
+        // variants run as the tile form: time that once, as itself
+        if (!plan->P.tv.p && c.variant >= 3 && c.variant != 7) continue;
         for (int w = 0; w < 2; w++)
             sdl::product_numeric_batched(plan->P, dBx, dD, ldd, K, dAx, dC, ldc, s, &c);
         SDL_HIP(hipEventRecord(e0, s));
@@ -303,6 +306,19 @@ int sdl_btda_plan_reserve(sdl_btda_plan *plan, int Kmax)
 {
     if (!plan || Kmax < 0) return 1;
     sdl::product_reserve_batch(plan->P, Kmax);
+    sdl::sync();
+    return 0;
+}
+
+int sdl_btda_plan_reserve_tile(sdl_btda_plan *plan, int Kmax)
+{
+    if (!plan || Kmax < 0) return 1;
+    if (!plan->P.pull || !plan->P.tq.p)
+    {
+        sdl::set_error("sdl_btda_plan_reserve_tile: needs a pull schedule over a single-block L");
+        return 1;
+    }
+    sdl::product_reserve_batch(plan->P, Kmax, true);
     sdl::sync();
     return 0;
 }

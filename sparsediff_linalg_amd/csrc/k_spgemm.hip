@@ -1003,7 +1003,11 @@ __global__ void k_tile_qrange(int ntiles, int64_t nnz, int64_t m_out,
 // else per term); each entry then folds its terms in order, scale(tq) *
 // Ax[tA] from +0.0 -- k_num_pull's products and fold, with one index and one
 // A value read per term instead of three indices and three gathers.
+// BV (bound values, the plan's tile layout): Lval is the bound L-order value
+// array lv (lv[q] = Lval[Lperm[q]]) and Ax the bound term-order A values ta
+// (ta[s] = Ax[tA[s]]), both read without an index.
 constexpr int TS_CAP = 1024;
+template <bool BV>
 __global__ void __launch_bounds__(256)
     k_num_pull_tile(int64_t nnz, const int2 *__restrict__ tqr, const int64_t *__restrict__ ep,
                     const int *__restrict__ tq, const int *__restrict__ tA,
@@ -1024,7 +1028,7 @@ __global__ void __launch_bounds__(256)
         for (int j = threadIdx.x; j < nq; j += 256)
         {
             const int q = qr.x + j;
-            double v = Lval[Lperm ? Lperm[q] : q];
+            double v = Lval[(!BV && Lperm) ? Lperm[q] : q];
             if (d) v = v * d[Li[q]];
             sc[j] = v;
         }
@@ -1039,12 +1043,24 @@ __global__ void __launch_bounds__(256)
         if (table) v = sc[q - qr.x];
         else
         {
-            v = Lval[Lperm ? Lperm[q] : q];
+            v = Lval[(!BV && Lperm) ? Lperm[q] : q];
             if (d) v = v * d[Li[q]];
         }
-        acc = acc + v * Ax[tA[s]];
+        acc = acc + v * Ax[BV ? s : tA[s]];
     }
     Cx[e] = acc;
+}
+
+// Bound tile layout: ta[s] = Ax[tA[s]] (term order), lv[q] = Lval[Lperm[q]]
+// (L order).
+__global__ void __launch_bounds__(256)
+    k_bind_tile(int64_t terms, int64_t nl, const int *__restrict__ tA,
+                const int *__restrict__ Lperm, const double *__restrict__ Lval,
+                const double *__restrict__ Ax, double *__restrict__ ta, double *__restrict__ lv)
+{
+    const int64_t i = (int64_t) blockIdx.x * 256 + threadIdx.x;
+    if (i < terms) ta[i] = Ax[tA[i]];
+    if (i < nl) lv[i] = Lval[Lperm ? Lperm[i] : i];
 }
 
 constexpr int NT_REG = 4;
@@ -1414,7 +1430,7 @@ __global__ void __launch_bounds__(256)
 // operand-pair array.  Tiles of more than TB_CAP L entries form each term's
 // factors from DT directly.
 constexpr int TB_CAP = 256;
-template <int RC>
+template <int RC, bool BV>
 __global__ void __launch_bounds__(256)
     k_num_pull_tile_batched(int64_t nnz, int ntiles, int nchunks, const int2 *__restrict__ tqr,
                             const int64_t *__restrict__ ep, const int *__restrict__ tq,
@@ -1441,7 +1457,7 @@ __global__ void __launch_bounds__(256)
         for (int j = threadIdx.x; j < nq; j += 256)
         {
             const int q = qr.x + j;
-            const double lv = Lval[Lperm ? Lperm[q] : q];
+            const double lv = Lval[(!BV && Lperm) ? Lperm[q] : q];
             const dvec2 *dk = (const dvec2 *) (DT + (int64_t) Li[q] * ldt + r0);
 #pragma unroll
             for (int h = 0; h < RC / 2; h++)
@@ -1462,7 +1478,7 @@ __global__ void __launch_bounds__(256)
     for (int64_t s = s0; s < s1; s++)
     {
         const int q = tq[s];
-        const double a = Ax[tA[s]];
+        const double a = Ax[BV ? s : tA[s]];
         if (table)
         {
             const dvec2 *t = tab + (q - qr.x) * (RC / 2);
@@ -1476,7 +1492,7 @@ __global__ void __launch_bounds__(256)
         }
         else
         {
-            const double lv = Lval[Lperm ? Lperm[q] : q];
+            const double lv = Lval[(!BV && Lperm) ? Lperm[q] : q];
             const dvec2 *dk = (const dvec2 *) (DT + (int64_t) Li[q] * ldt + r0);
 #pragma unroll
             for (int h = 0; h < RC / 2; h++)
@@ -2888,9 +2904,9 @@ void product_numeric(const ProductPlan &P, const double *Lval, const double *d,
     if (P.nnz == 0) return;
     if (P.pull && P.tile_refill && P.tq.p)
     {
-        hipLaunchKernelGGL(k_num_pull_tile, dim3((unsigned) ((P.nnz + 255) / 256)), dim3(256), 0,
-                           s, P.nnz, P.tqr.p, P.ep.p, P.tq.p, P.tA.p, P.L.Li, P.L.Lperm, Lval, d,
-                           Ax, Cx);
+        hipLaunchKernelGGL(k_num_pull_tile<false>, dim3((unsigned) ((P.nnz + 255) / 256)),
+                           dim3(256), 0, s, P.nnz, P.tqr.p, P.ep.p, P.tq.p, P.tA.p, P.L.Li,
+                           P.L.Lperm, Lval, d, Ax, Cx);
     }
     else if (P.pull)
     {
@@ -2960,9 +2976,17 @@ void product_numeric(const ProductPlan &P, const double *Lval, const double *d,
 void product_bind_values(const ProductPlan &P, const double *Lval, const double *Ax,
                          hipStream_t s)
 {
-    if (!P.pull || P.terms == 0 || !P.tv.p) return;
-    hipLaunchKernelGGL(k_gather_terms, dim3((unsigned) ((P.terms + 255) / 256)), dim3(256), 0, s,
-                       P.terms, P.tA.p, P.tB.p, Lval, Ax, const_cast<double *>(P.tv.p));
+    if (!P.pull || P.terms == 0) return;
+    if (P.tv.p)
+        hipLaunchKernelGGL(k_gather_terms, dim3((unsigned) ((P.terms + 255) / 256)), dim3(256), 0,
+                           s, P.terms, P.tA.p, P.tB.p, Lval, Ax, const_cast<double *>(P.tv.p));
+    if (P.ta.p)
+    {
+        const int64_t n = std::max<int64_t>(P.terms, P.nl);
+        hipLaunchKernelGGL(k_bind_tile, dim3((unsigned) ((n + 255) / 256)), dim3(256), 0, s,
+                           P.terms, P.nl, P.tA.p, P.L.Lperm, Lval, Ax,
+                           const_cast<double *>(P.ta.p), const_cast<double *>(P.lv.p));
+    }
     SDL_HIP(hipGetLastError());
 }
 
@@ -2985,10 +3009,35 @@ void product_numeric_batched(const ProductPlan &P, const double *Lval, const dou
     }
     RefillCfg c = cfg ? *cfg : (P.tuned.variant >= 0 ? P.tuned : default_refill_cfg(P, K));
     const bool kmajor = (opts & BATCH_D_KMAJOR) != 0;
+    const bool bound = (opts & BATCH_VALUES_BOUND) != 0;
+    // bound values in the plan's tile layout (ta, lv) rather than as pairs
+    const bool tileb = bound && !ws && P.ta.p;
+    SDL_REQUIRE(!bound || tileb || tvp, "bound refill: no values bound");
     // only the interleaved kernels read a k-major D
     if (kmajor && c.variant < 3) c = RefillCfg{3, c.rchunk == 16 ? 16 : 8};
     SDL_REQUIRE(!kmajor || K <= cap, "k-major refill beyond the reservation");
     if (c.variant >= 3 && K > cap) c = RefillCfg{2, 8};
+    // without an operand-pair array (a tile reservation) the interleaved
+    // kernels run in tile form
+    if (c.variant >= 3 && (tileb || !tvp)) c.variant = 7;
+    SDL_REQUIRE(c.variant != 7 || tvp || (P.tq.p && P.tqr.p),
+                "batched refill: neither operand pairs nor a tile layout");
+    if (c.variant == 7 && K == 1 && !kmajor && P.tq.p)
+    {
+        // a single refill in tile form: d read in place
+        if (ev_start) SDL_HIP(hipEventRecord((hipEvent_t) ev_start, s));
+        if (tileb)
+            hipLaunchKernelGGL(k_num_pull_tile<true>, dim3((unsigned) ((P.nnz + 255) / 256)),
+                               dim3(256), 0, s, P.nnz, P.tqr.p, P.ep.p, P.tq.p, P.tA.p, P.L.Li,
+                               P.L.Lperm, P.lv.p, D, P.ta.p, Cx);
+        else
+            hipLaunchKernelGGL(k_num_pull_tile<false>, dim3((unsigned) ((P.nnz + 255) / 256)),
+                               dim3(256), 0, s, P.nnz, P.tqr.p, P.ep.p, P.tq.p, P.tA.p, P.L.Li,
+                               P.L.Lperm, Lval, D, Ax, Cx);
+        SDL_HIP(hipGetLastError());
+        if (ev_stop) SDL_HIP(hipEventRecord((hipEvent_t) ev_stop, s));
+        return;
+    }
     if (c.variant >= 3 && K == 1 && !kmajor)
     {
         // a single refill: d read in place, no transposition
@@ -3008,8 +3057,9 @@ void product_numeric_batched(const ProductPlan &P, const double *Lval, const dou
     // direct operand gathers need the caller's value arrays: bound values use
     // the pair array
     if (c.variant == 6 && (opts & BATCH_VALUES_BOUND)) c.variant = 3;
-    // the tile form needs the plan's L indices and the caller's value arrays
-    if (c.variant == 7 && (!P.tq.p || !P.tqr.p || (opts & BATCH_VALUES_BOUND))) c.variant = 3;
+    // the tile form needs the plan's L indices, and the caller's value arrays
+    // or values bound in tile layout
+    if (c.variant == 7 && (!P.tq.p || !P.tqr.p || (bound && !tileb))) c.variant = 3;
     if (c.variant >= 3)
     {
         const int rc = c.rchunk == 8 ? 8 : 16;
@@ -3038,15 +3088,17 @@ void product_numeric_batched(const ProductPlan &P, const double *Lval, const dou
             const int ntiles = (int) ((P.nnz + 255) / 256);
             const int nchunks = (K + rc - 1) / rc;
             const int64_t blocks = ((int64_t) ntiles + 7) / 8 * 8 * nchunks;
-            if (rc == 8)
-                hipLaunchKernelGGL(k_num_pull_tile_batched<8>, dim3((unsigned) blocks), dim3(256),
-                                   0, s, P.nnz, ntiles, nchunks, P.tqr.p, P.ep.p, P.tq.p, P.tA.p,
-                                   P.L.Li, P.L.Lperm, Lval, Ax, DT, ldk, K, Cx, ldc);
-            else
-                hipLaunchKernelGGL(k_num_pull_tile_batched<16>, dim3((unsigned) blocks),
-                                   dim3(256), 0, s, P.nnz, ntiles, nchunks, P.tqr.p, P.ep.p,
-                                   P.tq.p, P.tA.p, P.L.Li, P.L.Lperm, Lval, Ax, DT, ldk, K, Cx,
-                                   ldc);
+            const double *lv = tileb ? P.lv.p : Lval;
+            const double *av = tileb ? P.ta.p : Ax;
+#define SDL_TB(RC_, BV_)                                                                       \
+    hipLaunchKernelGGL((k_num_pull_tile_batched<RC_, BV_>), dim3((unsigned) blocks), dim3(256), 0, \
+                       s, P.nnz, ntiles, nchunks, P.tqr.p, P.ep.p, P.tq.p, P.tA.p, P.L.Li,      \
+                       P.L.Lperm, lv, av, DT, ldk, K, Cx, ldc)
+            if (rc == 8 && tileb) SDL_TB(8, true);
+            else if (rc == 8) SDL_TB(8, false);
+            else if (tileb) SDL_TB(16, true);
+            else SDL_TB(16, false);
+#undef SDL_TB
             SDL_HIP(hipGetLastError());
             if (ev_stop) SDL_HIP(hipEventRecord((hipEvent_t) ev_stop, s));
             return;
@@ -3163,10 +3215,23 @@ int64_t batch_scratch_doubles(const ProductPlan &P, int K)
     return 2 * std::max<int64_t>(P.terms, 1) + (K > 1 ? std::max<int64_t>(P.A.Am, 1) * ld : 0);
 }
 
-void product_reserve_batch(ProductPlan &P, int Kmax)
+void product_reserve_batch(ProductPlan &P, int Kmax, bool tile)
 {
-    if (!P.pull || Kmax <= P.batch_cap) return;
-    if (!P.tv.p) P.tv.alloc((size_t) 2 * std::max<int64_t>(P.terms, 1));
+    if (!P.pull) return;
+    if (tile)
+    {
+        SDL_REQUIRE(P.tq.p && P.tqr.p, "tile reservation: the plan has no tile layout");
+        if (!P.ta.p)
+        {
+            int nl = 0;
+            d2h(&nl, P.L.Lp + P.L.Lm, sizeof(int));
+            P.nl = nl;
+            P.ta.alloc((size_t) std::max<int64_t>(P.terms, 1));
+            P.lv.alloc((size_t) std::max<int64_t>(P.nl, 1));
+        }
+    }
+    if (Kmax <= P.batch_cap) return;
+    if (!P.tv.p && !P.ta.p) P.tv.alloc((size_t) 2 * std::max<int64_t>(P.terms, 1));
     if (Kmax == 1)  // single refills read d in place: no transposed copy
     {
         P.batch_cap = 1;

@@ -123,6 +123,10 @@ struct ProductPlan
     DBuf<int> tq;
     DBuf<int2> tqr;
     bool tile_refill = false;
+    // Bound values in tile layout (a tile reservation instead of operand
+    // pairs): ta[s] = A value of term s, lv[q] = L value of L entry q.
+    DBuf<double> ta, lv;
+    int64_t nl = 0;
     // Row-accumulator schedule (otherwise): rows binned by nnz -- [0, 1) up to
     // 1024 entries (LDS accumulators), [1, 2) up to 65536 (by rank when the
     // column bitmap fits LDS), [2, 3) longer.
@@ -178,8 +182,11 @@ struct BatchScratch
 // Doubles of workspace the interleaved refill of K refills needs.
 int64_t batch_scratch_doubles(const ProductPlan &P, int K);
 
-// Reserve the interleaved batched-refill workspace for up to Kmax refills.
-void product_reserve_batch(ProductPlan &P, int Kmax);
+// Reserve the interleaved batched-refill workspace for up to Kmax refills:
+// the transposed d and, for bound values, operand pairs (16 B per term) or,
+// with tile (single-block pull plans), the tile layout (8 B per term and per
+// L entry).
+void product_reserve_batch(ProductPlan &P, int Kmax, bool tile = false);
 
 // One refill of the entries [e0, e1) only (pull plans): Cx[e0..e1) written,
 // the rest of Cx untouched.  One launch.

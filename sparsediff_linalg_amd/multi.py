@@ -61,19 +61,25 @@ class Group:
         self.ldc = (max(self.nnz, 1) + 63) // 64 * 64
         self.C = None
 
-    def finish(self, lib, K):
-        """Refill workspace, bound operand values and the ring of K output
-        slots (after every group's plan, so no symbolic pass runs beside
-        them)."""
+    def finish(self, lib, K, tile=False):
+        """Refill workspace, bound operand values (operand pairs, or with tile
+        the tile layout: each term's A value and each L entry's B value) and
+        the ring of K output slots (after every group's plan, so no symbolic
+        pass runs beside them)."""
         if self.pull:
-            check(lib.sdl_btda_plan_reserve(self.h, K), "reserve")
+            reserve = lib.sdl_btda_plan_reserve_tile if tile else lib.sdl_btda_plan_reserve
+            check(reserve(self.h, K), "reserve")
             check(lib.sdl_btda_bind_values(self.h, self.A[2].ptr, self.B[2].ptr, None), "bind")
         self.C = DeviceArray(K * self.ldc, np.float64)  # ring of K output slots
 
 
 class MultiProduct:
-    def __init__(self, w, max_terms=1_200_000_000, K=8):
+    def __init__(self, w, max_terms=1_200_000_000, K=8, tile=False):
+        """tile: bind the operand values in tile layout (the refills then run
+        the tile form of the interleaved kernel, reading d once per L entry
+        of a tile instead of once per term)."""
         assert K == 1 or K % 2 == 0, "K must be 1 or even"
+        self.tile = tile
         self.K = K
         self.lib = lib = load()
         self.w = w
@@ -95,7 +101,7 @@ class MultiProduct:
         self.g = [Group(lib, w, a, b, K) for a, b in self.groups]
         lib.sdl_synchronize()
         for g in self.g:
-            g.finish(lib, K)
+            g.finish(lib, K, tile)
         self.nnz = sum(g.nnz for g in self.g)
         self.terms = sum(g.terms for g in self.g)
         lib.sdl_synchronize()

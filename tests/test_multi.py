@@ -125,3 +125,34 @@ def test_autotuned_shape_same_values(setup):
     load().sdl_synchronize()
     for a, b in zip(mp.values(), eager[2]):
         assert np.array_equal(bits(a), bits(b))
+
+
+def test_tile_layout_matches_pairs(setup):
+    """Values bound in tile layout (sdl_btda_plan_reserve_tile): single
+    rounds, k-major rounds and graph replays equal the operand-pair path,
+    which test_instances_match_oracle pins to the oracle."""
+    w, _, D, (DT, ldt), eager = setup
+    lib = load()
+    mt = MultiProduct(w, max_terms=150_000, K=2, tile=True)
+    try:
+        for r in range(ROUNDS):
+            mt.refill_round(D, r)
+            lib.sdl_synchronize()
+            for a, b in zip(mt.values(), eager[r]):
+                assert np.array_equal(bits(a), bits(b)), r
+        mt.refill_rounds(DT, ldt, 0, 2)
+        lib.sdl_synchronize()
+        for slot in (0, 1):
+            for a, b in zip(mt.values(slot), eager[slot]):
+                assert np.array_equal(bits(a), bits(b)), slot
+        g = mt.capture(DT, ldt, ROUNDS)
+        try:
+            for rep in range(2):
+                check(lib.sdl_graph_launch(g, None))
+                lib.sdl_synchronize()
+                for a, b in zip(mt.values(0), eager[2]):
+                    assert np.array_equal(bits(a), bits(b)), rep
+        finally:
+            lib.sdl_graph_destroy(g)
+    finally:
+        mt.close()
