@@ -468,10 +468,11 @@ def main_hessian(args):
         lib.sdl_synchronize()
         return lib.sdl_event_elapsed_ms(e0, e1) / steps
 
-    # node refills forked over 1, 2 or 4 streams: eager and replayed; the line
-    # reports the fastest replay (the choice is a setting of the plan)
+    # node refills in one launch over every node's tiles (0), or one launch
+    # per node in order (1) or forked over 2 or 4 streams: eager and replayed;
+    # the line reports the fastest replay (the choice is a setting)
     by_streams = {}
-    for ns in (1, 2, 4):
+    for ns in (0, 1, 2, 4):
         H.set_streams(ns)
         me = timed(lambda: H.eval(Ax, D, Hx, Lx, stream))
         H.capture(Ax, D, Hx, Lx, stream)
@@ -546,6 +547,10 @@ def main_hessian(args):
     ms_graph_s = timed(lambda: Hs.replay(stream))
     same = bool(np.array_equal(eager_vals.view(np.int64), Hxs.numpy().view(np.int64)))
     assert same, "graph replay differs from the eager evaluation"
+    Hs.set_streams(4)
+    Hs.capture(Axs, Ds, Hxs, Lxs, stream)
+    ms_graph_s4 = timed(lambda: Hs.replay(stream))
+    same = same and bool(np.array_equal(eager_vals.view(np.int64), Hxs.numpy().view(np.int64)))
     Hs.set_streams(1)
     Hs.capture(Axs, Ds, Hxs, Lxs, stream)
     ms_graph_s1 = timed(lambda: Hs.replay(stream))
@@ -555,7 +560,9 @@ def main_hessian(args):
                                      "[500, 4000]"), "nnz_H": Hs.nnz,
                         "eager_ms_per_step": ms_eager_s, "graph_ms_per_step": ms_graph_s,
                         "graph_vs_eager": ms_eager_s / ms_graph_s, "graph_equals_eager": same,
-                        "graph_one_stream_ms_per_step": ms_graph_s1}
+                        "graph_one_stream_ms_per_step": ms_graph_s1,
+                        "graph_four_streams_ms_per_step": ms_graph_s4,
+                        "node_refills": "one launch over every node's tiles (16 nodes per launch)"}
     Hs.close()
     print(json.dumps(out), flush=True)
 

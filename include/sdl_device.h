@@ -340,9 +340,12 @@ typedef struct sdl_hessian sdl_hessian;
 int sdl_hessian_create(int nnodes, int n, const int *m, const int *const *dAp,
                        const int *const *dAi, sdl_hessian **out);
 void sdl_hessian_destroy(sdl_hessian *h);
-/* Streams the node refills are forked over (1 to 4, default 4; 1: in order on
-   the caller's stream).  Forking pays when the nodes are too small to fill
-   the GPU alone; a later capture records the current setting. */
+/* How the node refills are issued: 0 (default) = one launch over every node's
+   256-entry tiles (up to 16 nodes per launch; nodes without the tile layout,
+   e.g. row-accumulator schedules, fall back to 1), 1 = one launch per node in
+   order on the caller's stream, 2 to 4 = one launch per node forked over that
+   many streams.  Same values bit for bit; a later capture records the current
+   setting. */
 int sdl_hessian_set_streams(sdl_hessian *h, int streams);
 long long sdl_hessian_nnz(const sdl_hessian *h);
 const long long *sdl_hessian_rowptr(const sdl_hessian *h);

@@ -7,6 +7,7 @@
 // solver (src/sparse/COO_matrix.c:92-98) -- and here the same steps run as
 // device launches, eagerly or replayed from one HIP graph.
 #include "../../include/sdl_device.h"
+#include "btda_plan.h"
 #include "sparse_ops.h"
 #include "spgemm.h"
 
@@ -23,11 +24,12 @@ struct sdl_hessian
     sdl_sum_plan *sum = nullptr;
     sdl_lower_coo *lower = nullptr;
     sdl_graph *graph = nullptr;
-    // the node refills are independent: they fork onto these streams (default
+    // the node refills are independent: by default one launch covers every
+    // node's tiles (streams == 0); else they fork onto these streams (default
     // priority) and join back before the sum, so a captured graph has one
     // branch per stream
     static constexpr int NAUX = 4;
-    int streams = NAUX;  // node refills forked over this many streams (1: in order on s)
+    int streams = 0;  // 0: one launch; 1: in order on s; 2..4: forked
     hipStream_t aux[NAUX] = {nullptr};
     hipEvent_t fork = nullptr, join[NAUX] = {nullptr};
 };
@@ -61,6 +63,25 @@ int eval_launches(sdl_hessian *h, const double *const *dAx, const double *const 
 {
     const int k = (int) h->nodes.size();
     std::vector<const double *> xs(k);
+    for (int j = 0; j < k; j++) xs[j] = h->Cx[j].p;
+    if (h->streams == 0)
+    {
+        // every node's refill in one launch (A_j^T diag(d_j) A_j: B = A_j, so
+        // both value arrays are dAx[j]), when every node has the tile layout
+        std::vector<const sdl::ProductPlan *> plans(k);
+        std::vector<double *> cx(k);
+        for (int j = 0; j < k; j++)
+        {
+            plans[j] = &h->nodes[j]->P;
+            cx[j] = h->Cx[j].p;
+        }
+        if (sdl::product_numeric_multi(plans.data(), k, dAx, dd, dAx, cx.data(), s))
+        {
+            if (sdl_sum_fill(h->sum, xs.data(), nullptr, dHx, s)) return 1;
+            if (dLx && sdl_lower_coo_refresh(h->lower, dHx, dLx, s)) return 1;
+            return 0;
+        }
+    }
     if (h->streams <= 1)
     {
         // every node refill in order on s
@@ -160,9 +181,9 @@ void sdl_hessian_destroy(sdl_hessian *h) { release(h); }
 
 int sdl_hessian_set_streams(sdl_hessian *h, int streams)
 {
-    if (!h || streams < 1 || streams > sdl_hessian::NAUX)
+    if (!h || streams < 0 || streams > sdl_hessian::NAUX)
     {
-        sdl::set_error("sdl_hessian_set_streams: 1 to 4 streams");
+        sdl::set_error("sdl_hessian_set_streams: 0 to 4 streams");
         return 1;
     }
     h->streams = streams;

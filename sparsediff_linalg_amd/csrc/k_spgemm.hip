@@ -1051,6 +1051,69 @@ __global__ void __launch_bounds__(256)
     Cx[e] = acc;
 }
 
+// Several single refills (products of single-block pull plans) in one launch:
+// the tiles of node 0, then node 1, ...; each workgroup finds its node from the
+// tile offsets and runs k_num_pull_tile's body on it.  The descriptors travel
+// by value in the kernel arguments.
+struct TileNode
+{
+    const int2 *tqr;
+    const int64_t *ep;
+    const int *tq, *tA, *Li, *Lperm;
+    const double *Lval, *d, *Ax;
+    double *Cx;
+    int64_t nnz;
+    int tile0;
+};
+constexpr int TN_MAX = 16;
+struct TileNodes
+{
+    int n;
+    TileNode nd[TN_MAX];
+};
+
+__global__ void __launch_bounds__(256) k_num_pull_tile_multi(const TileNodes N)
+{
+    __shared__ double sc[TS_CAP];
+    int j = 0;
+    for (int t = 1; t < N.n; t++)
+        if (N.nd[t].tile0 <= (int) blockIdx.x) j = t;
+    const TileNode &D = N.nd[j];
+    const int tile = (int) blockIdx.x - D.tile0;
+    const int2 qr = D.tqr[tile];
+    const int nq = qr.y - qr.x;
+    const bool table = nq <= TS_CAP;  // the same for the whole workgroup
+    const int64_t e = (int64_t) tile * 256 + threadIdx.x;
+    const bool live = e < D.nnz;
+    const int64_t s0 = live ? D.ep[e] : 0, s1 = live ? D.ep[e + 1] : 0;
+    if (table)
+    {
+        for (int jj = threadIdx.x; jj < nq; jj += 256)
+        {
+            const int q = qr.x + jj;
+            double v = D.Lval[D.Lperm ? D.Lperm[q] : q];
+            if (D.d) v = v * D.d[D.Li[q]];
+            sc[jj] = v;
+        }
+    }
+    __syncthreads();
+    if (!live) return;
+    double acc = 0.0;
+    for (int64_t s = s0; s < s1; s++)
+    {
+        const int q = D.tq[s];
+        double v;
+        if (table) v = sc[q - qr.x];
+        else
+        {
+            v = D.Lval[D.Lperm ? D.Lperm[q] : q];
+            if (D.d) v = v * D.d[D.Li[q]];
+        }
+        acc = acc + v * D.Ax[D.tA[s]];
+    }
+    D.Cx[e] = acc;
+}
+
 // Bound tile layout: ta[s] = Ax[tA[s]] (term order), lv[q] = Lval[Lperm[q]]
 // (L order).
 __global__ void __launch_bounds__(256)
@@ -2971,6 +3034,43 @@ void product_numeric(const ProductPlan &P, const double *Lval, const double *d,
                                P.Ci.p, Lval, d, Ax, Cx);
     }
     SDL_HIP(hipGetLastError());
+}
+
+bool product_numeric_multi(const ProductPlan *const *plans, int n, const double *const *Lval,
+                           const double *const *d, const double *const *Ax, double *const *Cx,
+                           hipStream_t s)
+{
+    for (int j = 0; j < n; j++)
+        if (!plans[j]->pull || !plans[j]->tile_refill || !plans[j]->tq.p) return false;
+    for (int j0 = 0; j0 < n; j0 += TN_MAX)
+    {
+        TileNodes N{};
+        int tiles = 0;
+        N.n = 0;
+        for (int j = j0; j < n && j < j0 + TN_MAX; j++)
+        {
+            const ProductPlan &P = *plans[j];
+            if (P.nnz == 0) continue;
+            TileNode &D = N.nd[N.n++];
+            D.tqr = P.tqr.p;
+            D.ep = P.ep.p;
+            D.tq = P.tq.p;
+            D.tA = P.tA.p;
+            D.Li = P.L.Li;
+            D.Lperm = P.L.Lperm;
+            D.Lval = Lval[j];
+            D.d = d ? d[j] : nullptr;
+            D.Ax = Ax[j];
+            D.Cx = Cx[j];
+            D.nnz = P.nnz;
+            D.tile0 = tiles;
+            tiles += (int) ((P.nnz + 255) / 256);
+        }
+        if (N.n == 0) continue;
+        hipLaunchKernelGGL(k_num_pull_tile_multi, dim3((unsigned) tiles), dim3(256), 0, s, N);
+        SDL_HIP(hipGetLastError());
+    }
+    return true;
 }
 
 void product_bind_values(const ProductPlan &P, const double *Lval, const double *Ax,

@@ -188,6 +188,14 @@ int64_t batch_scratch_doubles(const ProductPlan &P, int K);
 // L entry).
 void product_reserve_batch(ProductPlan &P, int Kmax, bool tile = false);
 
+// Single refills of n plans in one launch per 16 plans (the tile refill's
+// kernel over every plan's tiles).  Returns false, launching nothing, unless
+// every plan is a pull plan refilled by tiles (sdl_btda_plan_single_refill).
+// d may be null (no d for any plan) or hold a null per plan.
+bool product_numeric_multi(const ProductPlan *const *plans, int n, const double *const *Lval,
+                           const double *const *d, const double *const *Ax, double *const *Cx,
+                           hipStream_t s);
+
 // One refill of the entries [e0, e1) only (pull plans): Cx[e0..e1) written,
 // the rest of Cx untouched.  One launch.
 void product_numeric_range(const ProductPlan &P, const double *Lval, const double *d,

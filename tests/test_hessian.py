@@ -68,17 +68,17 @@ def test_hessian_eval_and_graph_replays():
         H.close()
 
 
-@pytest.mark.parametrize("streams", [1, 2])
+@pytest.mark.parametrize("streams", [0, 1, 2])
 def test_hessian_node_streams(streams):
-    """The node refills in order on one stream, or forked over two, give the
-    oracle's Hessian bit for bit, eagerly and replayed; an out-of-range stream
-    count is rejected."""
+    """The node refills in one launch over every node's tiles, in order on one
+    stream, or forked over two, give the oracle's Hessian bit for bit, eagerly
+    and replayed; an out-of-range stream count is rejected."""
     lib = load()
     nodes, n = W.lagrangian_nodes(4, n=20_000, nnodes=5, m_lo=1_000, m_hi=5_000, window=6_000)
     H = Hessian([(p, i) for p, i, _ in nodes], n)
     try:
         with pytest.raises(RuntimeError):
-            H.set_streams(0)
+            H.set_streams(-1)
         with pytest.raises(RuntimeError):
             H.set_streams(5)
         H.set_streams(streams)
@@ -98,5 +98,26 @@ def test_hessian_node_streams(streams):
             H.replay()
             lib.sdl_synchronize()
             assert bits_equal(Hx.numpy()[:H.nnz], Ox)
+    finally:
+        H.close()
+
+
+def test_hessian_many_nodes_one_launch_each_16():
+    """20 nodes: the default path launches the node refills in groups of 16
+    tiles' descriptors; the Hessian equals the oracle's bit for bit."""
+    lib = load()
+    nodes, n = W.lagrangian_nodes(5, n=12_000, nnodes=20, m_lo=300, m_hi=2_500, window=4_000)
+    H = Hessian([(p, i) for p, i, _ in nodes], n)
+    try:
+        rng = np.random.default_rng(17)
+        ds = [rng.uniform(0.1, 10, len(p) - 1) for p, _, _ in nodes]
+        Ax = [DeviceArray.from_numpy(x) for _, _, x in nodes]
+        D = [DeviceArray.from_numpy(d) for d in ds]
+        Hx = DeviceArray(max(H.nnz, 1), np.float64)
+        Lx = DeviceArray(max(H.lower_nnz, 1), np.float64)
+        H.eval(Ax, D, Hx, Lx)
+        lib.sdl_synchronize()
+        _, _, Ox, _, _, lx = oracle_hessian(nodes, n, ds)
+        assert bits_equal(Hx.numpy()[:H.nnz], Ox) and bits_equal(Lx.numpy()[:H.lower_nnz], lx)
     finally:
         H.close()
