@@ -293,6 +293,18 @@ const int *sdl_sum_plan_map(const sdl_sum_plan *plan);
    masks (at most 16 operands, each with sorted unique columns per row); 0 when
    it reads them through the entry -> operand-entry lists. */
 int sdl_sum_plan_is_masked(const sdl_sum_plan *plan);
+/* The plan's fill: 0 = entry -> operand-entry lists, 1 = operand masks,
+   2 = scatter (operands that rarely share an output entry: at most 32 operands,
+   operand entries <= ratio * output entries, sdl_sum_scatter_ratio, and on
+   average 16 or more consecutive output entries from one operand or at most
+   4,194,304 output entries; each
+   single-source entry written as 0.0 + x straight from its operand, in operand
+   order, the shared entries summed through the lists); -1 for NULL. */
+int sdl_sum_plan_fill_kind(const sdl_sum_plan *plan);
+/* Ratio of operand entries to output entries up to which later sum plans use
+   the scatter fill (default 1.25; 0 = never); ratio < 0 only queries.
+   Returns the previous ratio. */
+double sdl_sum_scatter_ratio(double ratio);
 /* dCx[nnz] = the sum; dX: host array of nops device value pointers.  dD (device,
    nops x m, or NULL): row scales, each term the rounded product dD[k*m + row] * x
    (sum_scaled_matrices_fill_values, CSR_sum.c:116-148, for any count).  Up to 32

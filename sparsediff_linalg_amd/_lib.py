@@ -212,6 +212,8 @@ SIGNATURES = {
     "sdl_block_gemm_slice": (C.c_int, [C.c_int]),
     "sdl_btda_plan_single_refill": (C.c_int, [vp, C.c_int]),
     "sdl_btda_plan_reserve_tile": (C.c_int, [vp, C.c_int]),
+    "sdl_sum_plan_fill_kind": (C.c_int, [vp]),
+    "sdl_sum_scatter_ratio": (C.c_double, [C.c_double]),
     "sdl_block_coalesce_mode": (C.c_int, [C.c_int]),
     # generators
     "sdl_gen_row_lengths_uniform": (C.c_longlong, [C.c_uint64, C.c_int, C.c_int, C.c_int,

@@ -91,16 +91,19 @@ struct SumSrc
 // offsets, so the only dependent loads are seg -> order -> value.  Items go
 // four at a time with unconditional loads (slots past the entry's last item
 // repeat its first item and are dropped), so four loads are in flight.
+// elist (may be null): the entries to fill, e = elist[i] for i < nnz.
 template <bool SCALED>
 __global__ __launch_bounds__(256) void k_sum_fill_src(int64_t nnz, const int *__restrict__ seg,
                                                       const unsigned *__restrict__ order,
                                                       const SumSrc src, int nops,
                                                       const int *__restrict__ row,
                                                       const double *__restrict__ d, int m,
-                                                      double *__restrict__ Cx)
+                                                      double *__restrict__ Cx,
+                                                      const int *__restrict__ elist)
 {
-    int64_t e = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= nnz) return;
+    const int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nnz) return;
+    const int64_t e = elist ? elist[i] : i;
     const int s1 = seg[e + 1];
     double acc = 0.0;
     for (int s = seg[e]; s < s1; s += 4)
@@ -133,6 +136,76 @@ __global__ __launch_bounds__(256) void k_sum_fill_src(int64_t nnz, const int *__
             if (u < cnt) acc = acc + v[u];
     }
     __builtin_nontemporal_store(acc, Cx + e);
+}
+
+// ---- scatter fill (operands that rarely share an output entry) ----
+// Output entries fed by one operand entry: Cx[e] = 0.0 + x (or the rounded
+// product d * x), the operand entry t streamed in order and written through
+// smap[t] = e (-1 for entries of several operand entries, which the list fill
+// sums over mlist).
+template <bool SCALED>
+__global__ __launch_bounds__(256) void k_sum_fill_single(int64_t items,
+                                                         const int *__restrict__ smap,
+                                                         const SumSrc src, int nops,
+                                                         const int *__restrict__ row,
+                                                         const double *__restrict__ d, int m,
+                                                         double *__restrict__ Cx)
+{
+    const int64_t t = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= items) return;
+    const int e = smap[t];
+    if (e < 0) return;
+    const double *base = src.x[0];
+    int64_t o = 0;
+    int k = 0;
+    for (int j = 1; j < nops; j++)
+    {
+        if (t >= src.off[j])
+        {
+            base = src.x[j];
+            o = src.off[j];
+            k = j;
+        }
+    }
+    double v = base[t - o];
+    if (SCALED) v = d[(int64_t) k * m + row[t]] * v;
+    Cx[e] = 0.0 + v;
+}
+
+// smap[t] = map[t] when that entry has one operand entry, else -1.
+__global__ void k_sum_single_map(int64_t items, const int *__restrict__ map,
+                                 const int *__restrict__ seg, int *__restrict__ smap)
+{
+    const int64_t t = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= items) return;
+    const int e = map[t];
+    smap[t] = seg[e + 1] - seg[e] == 1 ? e : -1;
+}
+
+// Operand changes along the output: cnt += entries e whose first operand
+// entry belongs to another operand than entry e + 1's.
+__global__ void k_sum_switches(int64_t nnz, const int *__restrict__ seg,
+                               const unsigned *__restrict__ order,
+                               const int64_t *__restrict__ off, int nops,
+                               unsigned long long *__restrict__ cnt)
+{
+    const int64_t e = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (e + 1 >= nnz) return;
+    if (op_of(off, nops, order[seg[e]]) != op_of(off, nops, order[seg[e + 1]])) atomicAdd(cnt, 1ull);
+}
+
+// flag[e] = entry e has several operand entries
+__global__ void k_sum_multi_flag(int64_t nnz, const int *__restrict__ seg, int *__restrict__ flag)
+{
+    const int64_t e = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < nnz) flag[e] = seg[e + 1] - seg[e] > 1 ? 1 : 0;
+}
+
+__global__ void k_sum_multi_list(int64_t nnz, const int *__restrict__ flag,
+                                 const int64_t *__restrict__ pos, int *__restrict__ list)
+{
+    const int64_t e = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < nnz && flag[e]) list[pos[e]] = (int) e;
 }
 
 // ---- operand-mask fill (up to SUM_MASK_OPS operands, sorted unique columns) --
@@ -359,7 +432,19 @@ struct SumPlan
     DBuf<int> tile_base;            // nops x ntiles (masked plans)
     DBuf<int> erow;                 // row of every output entry (masked plans keeping rows)
     DBuf<const double *> xs;        // value pointers beyond SUM_ARGS operands
+    bool scatter = false;           // the scatter fill applies (few shared entries)
+    DBuf<int> smap;                 // item -> its entry if single-source, else -1
+    DBuf<int> mlist;                // entries of several items (scatter plans)
+    int64_t nmulti = 0;
 };
+
+double g_sum_scatter_ratio = 1.25;
+// mean run of consecutive output entries from one operand the scatter fill
+// needs (16 doubles: 128 bytes, one line)
+constexpr double SUM_SCATTER_RUN = 16.0;
+// ... or an output small enough (32 MB: the eight XCDs' L2) that the caches
+// merge its scattered writes
+constexpr int64_t SUM_SCATTER_CACHED = 4 << 20;
 
 void sum_plan_build(SumPlan &S, int nops, int m, int n, const int *const *dP,
                     const int *const *dI, const int64_t *nnz, bool keep_rows)
@@ -395,7 +480,42 @@ void sum_plan_build(SumPlan &S, int nops, int m, int n, const int *const *dP,
     S.dup.assign(nops, 0);
     d2h(S.dup.data(), ddup.p, (size_t) nops * sizeof(int));
     S.P.seg.reset();
-    if (nops <= SUM_MASK_OPS && S.P.nnz > 0)
+    // operands that rarely share an output entry and each cover long runs of
+    // consecutive output entries (so their scattered writes still fill whole
+    // lines): the scatter fill
+    bool scatter = nops <= SUM_ARGS && S.P.nnz > 0 &&
+                   (double) items <= g_sum_scatter_ratio * (double) S.P.nnz;
+    if (scatter && nops > 1)
+    {
+        DBuf<unsigned long long> sw(1);
+        dzero(sw.p, sw.bytes());
+        hipLaunchKernelGGL(k_sum_switches, dim3(sum_grid(S.P.nnz)), dim3(256), 0, stream(),
+                           S.P.nnz, S.seg.p, S.P.order.p, S.doff.p, nops, sw.p);
+        SDL_HIP(hipGetLastError());
+        unsigned long long h = 0;
+        d2h(&h, sw.p, sizeof(h));
+        scatter = (double) S.P.nnz >= SUM_SCATTER_RUN * (double) (h + 1) ||
+                  S.P.nnz <= SUM_SCATTER_CACHED;
+    }
+    if (scatter)
+    {
+        S.scatter = true;
+        S.smap.alloc(items);
+        hipLaunchKernelGGL(k_sum_single_map, dim3(sum_grid(items)), dim3(256), 0, stream(), items,
+                           S.map.p, S.seg.p, S.smap.p);
+        DBuf<int> flag(S.P.nnz);
+        DBuf<int64_t> pos((size_t) S.P.nnz + 1);
+        hipLaunchKernelGGL(k_sum_multi_flag, dim3(sum_grid(S.P.nnz)), dim3(256), 0, stream(),
+                           S.P.nnz, S.seg.p, flag.p);
+        S.nmulti = scan_i32_to_i64(flag.p, pos.p, S.P.nnz, true);
+        S.mlist.alloc((size_t) std::max<int64_t>(S.nmulti, 1));
+        if (S.nmulti > 0)
+            hipLaunchKernelGGL(k_sum_multi_list, dim3(sum_grid(S.P.nnz)), dim3(256), 0, stream(),
+                               S.P.nnz, flag.p, pos.p, S.mlist.p);
+        SDL_HIP(hipGetLastError());
+        sync();
+    }
+    if (!S.scatter && nops <= SUM_MASK_OPS && S.P.nnz > 0)
     {
         DBuf<int> dbad(nops);
         dzero(dbad.p, dbad.bytes());
@@ -454,7 +574,30 @@ void sum_plan_fill(SumPlan &S, const double *const *hX, const double *dd, double
         SumSrc src{};
         for (int k = 0; k < S.nops; k++) src.x[k] = hX[k];
         for (int k = 0; k <= S.nops; k++) src.off[k] = S.off[k];
-        if (S.masked)
+        if (S.scatter)
+        {
+            const int64_t items = S.off[S.nops];
+            if (dd)
+                hipLaunchKernelGGL(k_sum_fill_single<true>, dim3(sum_grid(items)), dim3(256), 0, st,
+                                   items, S.smap.p, src, S.nops, S.row.p, dd, S.m, dCx);
+            else
+                hipLaunchKernelGGL(k_sum_fill_single<false>, dim3(sum_grid(items)), dim3(256), 0,
+                                   st, items, S.smap.p, src, S.nops, (const int *) nullptr,
+                                   (const double *) nullptr, S.m, dCx);
+            if (S.nmulti > 0)
+            {
+                if (dd)
+                    hipLaunchKernelGGL(k_sum_fill_src<true>, dim3(sum_grid(S.nmulti)), dim3(256), 0,
+                                       st, S.nmulti, S.seg.p, S.P.order.p, src, S.nops, S.row.p,
+                                       dd, S.m, dCx, S.mlist.p);
+                else
+                    hipLaunchKernelGGL(k_sum_fill_src<false>, dim3(sum_grid(S.nmulti)), dim3(256),
+                                       0, st, S.nmulti, S.seg.p, S.P.order.p, src, S.nops,
+                                       (const int *) nullptr, (const double *) nullptr, S.m, dCx,
+                                       S.mlist.p);
+            }
+        }
+        else if (S.masked)
         {
             const unsigned blocks = (unsigned) ((S.ntiles + 3) / 4);
             auto *fn = &launch_fill_mask<1>;
@@ -482,11 +625,12 @@ void sum_plan_fill(SumPlan &S, const double *const *hX, const double *dd, double
         }
         else if (dd)
             hipLaunchKernelGGL(k_sum_fill_src<true>, dim3(grid), dim3(256), 0, st, S.P.nnz,
-                               S.seg.p, S.P.order.p, src, S.nops, S.row.p, dd, S.m, dCx);
+                               S.seg.p, S.P.order.p, src, S.nops, S.row.p, dd, S.m, dCx,
+                               (const int *) nullptr);
         else
             hipLaunchKernelGGL(k_sum_fill_src<false>, dim3(grid), dim3(256), 0, st, S.P.nnz,
                                S.seg.p, S.P.order.p, src, S.nops, (const int *) nullptr,
-                               (const double *) nullptr, S.m, dCx);
+                               (const double *) nullptr, S.m, dCx, (const int *) nullptr);
     }
     else
     {
@@ -764,6 +908,19 @@ const long long *sdl_sum_plan_rowptr(const sdl_sum_plan *p)
 const int *sdl_sum_plan_colind(const sdl_sum_plan *p) { return p ? p->S.P.Ci.p : nullptr; }
 const int *sdl_sum_plan_map(const sdl_sum_plan *p) { return p ? p->S.map.p : nullptr; }
 int sdl_sum_plan_is_masked(const sdl_sum_plan *p) { return p && p->S.masked ? 1 : 0; }
+
+int sdl_sum_plan_fill_kind(const sdl_sum_plan *p)
+{
+    if (!p) return -1;
+    return p->S.scatter ? 2 : (p->S.masked ? 1 : 0);
+}
+
+double sdl_sum_scatter_ratio(double ratio)
+{
+    const double prev = sdl::g_sum_scatter_ratio;
+    if (ratio >= 0.0) sdl::g_sum_scatter_ratio = ratio;
+    return prev;
+}
 
 int sdl_sum_fill(sdl_sum_plan *p, const double *const *dX, const double *dD, double *dCx,
                  void *stream)

@@ -105,8 +105,12 @@ def test_sum_plan_operand_masks(nops):
     rng = np.random.default_rng(40 + nops)
     m, n = 5000, 3000
     ops = make_ops(rng, m, n, nops, empty_op=1 if nops > 2 else None)
-    S = DevSum(m, n, ops, keep_rows=1)
-    assert lib.sdl_sum_plan_is_masked(S.h)
+    prev = lib.sdl_sum_scatter_ratio(0.0)  # these operands rarely overlap: force the masks
+    try:
+        S = DevSum(m, n, ops, keep_rows=1)
+    finally:
+        lib.sdl_sum_scatter_ratio(prev)
+    assert lib.sdl_sum_plan_is_masked(S.h) and lib.sdl_sum_plan_fill_kind(S.h) == 1
     check(lib.sdl_sum_fill(S.h, S.dX, None, S.out.ptr, None), "fill")
     assert bits_equal(S.values(), O.sum_n(m, n, ops)[2])
     d = rng.uniform(-2, 2, (nops, m))
@@ -123,11 +127,74 @@ def test_sum_plan_17_operands_use_lists():
     rng = np.random.default_rng(47)
     m, n = 900, 700
     ops = make_ops(rng, m, n, 17)
-    S = DevSum(m, n, ops, keep_rows=0)
-    assert not lib.sdl_sum_plan_is_masked(S.h)
+    prev = lib.sdl_sum_scatter_ratio(0.0)
+    try:
+        S = DevSum(m, n, ops, keep_rows=0)
+    finally:
+        lib.sdl_sum_scatter_ratio(prev)
+    assert not lib.sdl_sum_plan_is_masked(S.h) and lib.sdl_sum_plan_fill_kind(S.h) == 0
     check(lib.sdl_sum_fill(S.h, S.dX, None, S.out.ptr, None), "fill")
     assert bits_equal(S.values(), O.sum_n(m, n, ops)[2])
     S.close()
+
+
+@pytest.mark.parametrize("nops,dup", [(2, None), (12, 3), (30, 7)])
+def test_sum_plan_scatter_fill(nops, dup):
+    """Operands that rarely share an output entry and cover long runs of it
+    (column bands, plus a few entries anywhere; for dup, an operand repeating
+    every third entry): single-source entries written straight from their
+    operand, the shared ones through the lists; plain and row-scaled fills bit
+    for bit against the oracle.  Interleaved operands keep the other fills."""
+    lib = load()
+    rng = np.random.default_rng(50 + nops)
+    m, w = 2000, 300
+    n = w * nops
+    ops = []
+    for k in range(nops):  # operand k: a band of columns, plus ~200 entries anywhere
+        band = rng.random((m, w)) < 0.2
+        full = np.zeros((m, n), bool)
+        full[:, k * w:(k + 1) * w] = band
+        full |= rng.random((m, n)) < 0.1 / n
+        p = np.concatenate([[0], np.cumsum(full.sum(1))]).astype(np.int32)
+        i = np.nonzero(full)[1].astype(np.int32)
+        ops.append((p, i, rng.standard_normal(len(i))))
+    if dup is not None:  # repeat every third entry of one operand in place
+        p, i, x = ops[dup]
+        reps = (np.arange(len(i)) % 3 == 0).astype(int)
+        rows = np.repeat(np.arange(m), np.diff(p))
+        cnt = np.bincount(rows, weights=1 + reps, minlength=m).astype(np.int64)
+        ops[dup] = (np.concatenate([[0], np.cumsum(cnt)]).astype(np.int32),
+                    np.repeat(i, 1 + reps).astype(np.int32), np.repeat(x, 1 + reps) * 0.75)
+    S = DevSum(m, n, ops, keep_rows=1)
+    assert lib.sdl_sum_plan_fill_kind(S.h) == 2
+    Cp, Ci, Cx, _ = O.sum_n(m, n, ops)
+    cp, ci = S.structure()
+    assert np.array_equal(cp, Cp) and np.array_equal(ci, Ci)
+    check(lib.sdl_sum_fill(S.h, S.dX, None, S.out.ptr, None), "fill")
+    assert bits_equal(S.values(), Cx)
+    d = rng.uniform(-2, 2, (nops, m))
+    dd = DeviceArray.from_numpy(d)
+    check(lib.sdl_sum_fill(S.h, S.dX, dd.ptr, S.out.ptr, None), "scaled fill")
+    assert bits_equal(S.values(), O.sum_n(m, n, ops, d=d)[2])
+    S.close()
+
+
+def test_sum_plan_interleaved_large_keeps_lists_or_masks():
+    """Operands that rarely share an entry but interleave within rows, into an
+    output beyond 4,194,304 entries: no scatter fill (its writes would land in
+    shared lines); the values bit for bit against the oracle."""
+    lib = load()
+    rng = np.random.default_rng(58)
+    m, n = 20000, 2000
+    ops = [random_csr(rng, m, n, 0.06) for _ in range(2)]
+    S = DevSum(m, n, ops, keep_rows=0)
+    try:
+        assert S.nnz > 4 << 20
+        assert lib.sdl_sum_plan_fill_kind(S.h) != 2
+        check(lib.sdl_sum_fill(S.h, S.dX, None, S.out.ptr, None), "fill")
+        assert bits_equal(S.values(), O.sum_n(m, n, ops)[2])
+    finally:
+        S.close()
 
 
 def test_sum_plan_unsorted_columns():
