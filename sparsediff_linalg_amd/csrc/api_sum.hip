@@ -190,8 +190,11 @@ __global__ void k_sum_switches(int64_t nnz, const int *__restrict__ seg,
                                unsigned long long *__restrict__ cnt)
 {
     const int64_t e = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
-    if (e + 1 >= nnz) return;
-    if (op_of(off, nops, order[seg[e]]) != op_of(off, nops, order[seg[e + 1]])) atomicAdd(cnt, 1ull);
+    const bool sw = e + 1 < nnz &&
+                    op_of(off, nops, order[seg[e]]) != op_of(off, nops, order[seg[e + 1]]);
+    // one atomic per wavefront
+    const unsigned long long b = __ballot(sw);
+    if ((threadIdx.x & 63) == 0 && b) atomicAdd(cnt, (unsigned long long) __popcll(b));
 }
 
 // flag[e] = entry e has several operand entries
