@@ -90,6 +90,19 @@ int sdl_btda_plan_structure(const sdl_btda_plan *plan, int *hp, int *hi);
    the first count pass's hash / LDS-bitmap / global-bitmap rows (the count
    pass sizes each row from min(terms, column span), or from its prediction). */
 int sdl_btda_plan_stats(const sdl_btda_plan *plan, long long *out11);
+/* Rows counted (first count pass) and filled in span windows (2 values):
+   rows whose columns all lie in a span of at most 8192 columns, marked in an
+   LDS array over that span instead of a hash table or a column bitmap. */
+int sdl_btda_plan_window_rows(const sdl_btda_plan *plan, long long *out2);
+/* Span windows in the symbolic phase of plans created afterwards: 1
+   (default) on, 0 off (hash tables and column bitmaps only); -1 only
+   queries.  Both give the same structure.  Returns the previous mode, or -1
+   for an unknown one. */
+int sdl_symbolic_window(int mode);
+/* Lanes per L entry of the span-window kernels: 4, 8, 16, 32 or 64, or 0
+   (default) to pick them from the product's mean A-row length; -1 only
+   queries.  Returns the previous setting, or -1 for an invalid one. */
+int sdl_symbolic_window_lanes(int lanes);
 /* Refill schedule (5 values): pull schedule (0/1), by-rank rows (row-
    accumulator plans), recorded positions for them (0/1), then with recorded
    positions the listed terms of entries of several terms and the rows left

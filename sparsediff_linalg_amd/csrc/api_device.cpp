@@ -137,6 +137,31 @@ int sdl_btda_plan_stats(const sdl_btda_plan *plan, long long *o)
     return 0;
 }
 
+int sdl_btda_plan_window_rows(const sdl_btda_plan *plan, long long *o)
+{
+    if (!plan || !o) return 1;
+    o[0] = plan->P.stats.count_window_rows;
+    o[1] = plan->P.stats.window_rows;
+    return 0;
+}
+
+int sdl_symbolic_window(int mode)
+{
+    if (mode < -1 || mode > 1) return -1;
+    const int prev = sdl::g_sym_window;
+    if (mode >= 0) sdl::g_sym_window = mode;
+    return prev;
+}
+
+int sdl_symbolic_window_lanes(int lanes)
+{
+    if (lanes < -1 || lanes > 64 || (lanes > 0 && (lanes < 4 || (lanes & (lanes - 1)))))
+        return -1;
+    const int prev = sdl::g_sym_window_lanes;
+    if (lanes >= 0) sdl::g_sym_window_lanes = lanes;
+    return prev;
+}
+
 int sdl_btda_refill(const sdl_btda_plan *plan, const double *dAx, const double *dBx,
                     const double *dd, double *dCx, void *stream)
 {

@@ -25,11 +25,18 @@
 
 namespace sdl {
 
+int g_sym_window = 1;
+int g_sym_window_lanes = 0;
+
 namespace {
 
-constexpr int NCLASS = 11;      // 0 empty, 1..8 hash 64..8192, 9 LDS bitmap, 10 global bitmap
+// 0 empty, 1..8 hash 64..8192, 9 LDS bitmap, 10 global bitmap, 11..16 span
+// windows of 256..8192 columns
+constexpr int NCLASS = 17;
 constexpr int CLS_BITMAP = 9;
 constexpr int CLS_GLOBAL = 10;
+constexpr int CLS_WIN0 = 11;
+constexpr int WIN_MIN = 256, WIN_MAX = 8192;
 constexpr int HASH_MAX = 8192;
 
 int g_lds_bitmap_words = -1;    // largest bitmap held in LDS by one workgroup
@@ -329,12 +336,14 @@ __global__ void __launch_bounds__(256)
 // Per output row: its term count, and (bound non-null) its nnz bound
 // min(terms, column span): a row's columns all lie between the smallest and
 // the largest column of its A rows (arange, from k_arow_range), so the span
-// bounds the distinct count exactly.  16 lanes per row, four rows per
-// wavefront.
+// bounds the distinct count exactly.  With win_lo non-null also the span
+// itself: its first column and width (0 for a row without terms).  16 lanes
+// per row, four rows per wavefront.
 __global__ void __launch_bounds__(256)
     k_row_terms(int64_t m_out, int Lm, const int *__restrict__ Lp, const int *__restrict__ Li,
                 int nL, const int *__restrict__ Ap, const int2 *__restrict__ arange,
-                int64_t *__restrict__ terms, int64_t *__restrict__ bound)
+                int64_t *__restrict__ terms, int64_t *__restrict__ bound,
+                int *__restrict__ win_lo = nullptr, int *__restrict__ win_span = nullptr)
 {
     const int64_t row = (int64_t) blockIdx.x * 16 + (threadIdx.x >> 4);
     const int sub = threadIdx.x & 15;
@@ -366,6 +375,11 @@ __global__ void __launch_bounds__(256)
     if (!live || sub != 0) return;
     terms[row] = t;
     if (bound) bound[row] = t > 0 ? min(t, (int64_t) cmax - cmin + 1) : 0;
+    if (win_lo)
+    {
+        win_lo[row] = t > 0 ? cmin : 0;
+        win_span[row] = t > 0 ? cmax - cmin + 1 : 0;
+    }
 }
 
 // Accumulator class of a row whose distinct count is at most cap: a hash
@@ -393,11 +407,48 @@ __device__ __forceinline__ int classify(int64_t cap, int nwords, int lds_words,
     return CLS_GLOBAL;
 }
 
-// mode 0: symbolic accumulator classes (classify with hash_max / dens);
-// mode 1: 0 empty, 1 <= t1, 2 <= t2, 3 larger
+// Class of a row of `terms` terms whose columns lie in a span of `span`
+// columns: the span window (CLS_WIN0 + j, W = 256 << j columns) when the span
+// fits WIN_MAX and the window's slots, which it clears and scans, are at most
+// twice the hash table the row would get or half its terms (a hash insert
+// costs a compare-and-swap and probes, a window mark one store; the fill pass
+// also sorts the table); against the column bitmap, when the window's slots
+// exceed the bitmap's words by less than the row's terms (consecutive
+// columns put up to 32 lanes on one bitmap word, whose atomic ORs
+// serialise, where the window spreads them over the LDS banks).  Otherwise
+// classify's choice.
+__device__ __forceinline__ int classify_span(int64_t cap, int64_t span, int64_t terms, int nwords,
+                                             int lds_words, int hash_max, int dens)
+{
+    if (cap <= 0) return 0;
+    if (span > 0 && span <= WIN_MAX)
+    {
+        int j = 0;
+        while ((WIN_MIN << j) < span) j++;
+        const int64_t wcols = (int64_t) WIN_MIN << j;
+        const bool bitmap =
+            nwords <= lds_words && (cap * dens >= nwords || 2 * cap > hash_max);
+        if (bitmap)
+        {
+            if (wcols <= (int64_t) nwords + terms) return CLS_WIN0 + j;
+        }
+        else
+        {
+            int64_t hs = 64;
+            while (hs < 2 * cap) hs <<= 1;
+            if (wcols <= 2 * hs || 2 * wcols <= terms) return CLS_WIN0 + j;
+        }
+    }
+    return classify(cap, nwords, lds_words, hash_max, dens);
+}
+
+// mode 0: symbolic accumulator classes (classify with hash_max / dens, or
+// classify_span where span is given); mode 1: 0 empty, 1 <= t1, 2 <= t2, 3
+// larger
 template <typename T>
 __global__ void k_classify(int64_t m, const T *__restrict__ cap, int mode, int nwords,
                            int lds_words, int hash_max, int dens, int t1, int t2,
+                           const int *__restrict__ span, const int64_t *__restrict__ terms,
                            unsigned char *__restrict__ cls, int *__restrict__ counts)
 {
     __shared__ int hist[NCLASS];
@@ -407,7 +458,9 @@ __global__ void k_classify(int64_t m, const T *__restrict__ cap, int mode, int n
     if (row < m)
     {
         int64_t v = (int64_t) cap[row];
-        int c = mode == 0 ? classify(v, nwords, lds_words, hash_max, dens)
+        int c = mode == 0 ? (span ? classify_span(v, span[row], terms[row], nwords, lds_words,
+                                                  hash_max, dens)
+                                  : classify(v, nwords, lds_words, hash_max, dens))
                           : (v <= 0 ? 0 : (v <= t1 ? 1 : (v <= t2 ? 2 : 3)));
         cls[row] = (unsigned char) c;
         atomicAdd(&hist[c], 1);
@@ -555,6 +608,91 @@ __global__ void __launch_bounds__(G)
         }
         __syncthreads();
         if (!GLOBAL) break;
+    }
+}
+
+// Span-window accumulator of a row whose columns all lie in [lo, lo + W)
+// (lo = its smallest column, from k_row_terms): an LDS mark per column of the
+// window, set by plain stores -- any number of terms may mark one column, so
+// there is no atomic, no probe and no overflow -- counted by wavefront ballots
+// and, in the fill pass, read out in ascending order with no sort.  sg lanes
+// take one L entry at a time and stride over its A row, four loads in flight
+// (sg divides G).  A column outside the window cannot occur (the span bounds
+// every column of the row) and would be dropped by the range check.
+template <int G, int W, int PASS>
+__global__ void __launch_bounds__(G)
+    k_sym_window(const int *__restrict__ rows, int Lm, const int *__restrict__ Lp,
+                 const int *__restrict__ Li, int nL, const int *__restrict__ Ap,
+                 const int *__restrict__ Ai, const int *__restrict__ win_lo, int sg,
+                 int *__restrict__ rowcnt, const int64_t *__restrict__ Cp,
+                 int *__restrict__ Ci)
+{
+    static_assert(W % G == 0 && W / (G / 64) >= 64, "window slots per wavefront");
+    constexpr int NW = G / 64, PER = W / NW;
+    __shared__ int mark[W];
+    __shared__ int wcnt[NW];
+    const int wid = (int) threadIdx.x >> 6, lane = (int) threadIdx.x & 63;
+    const int row = rows[blockIdx.x];
+    const int lo = win_lo[row];
+    for (int i = (int) threadIdx.x; i < W; i += G) mark[i] = 0;
+    __syncthreads();
+    const RowCtx c = row_ctx(row, Lm, Lp, nL);
+    const int grp = (int) threadIdx.x / sg, ngrp = G / sg, sl = (int) threadIdx.x - grp * sg;
+    for (int q = c.q0 + grp; q < c.q1; q += ngrp)
+    {
+        const int k = Li[q] + c.koff;
+        const int st = Ap[k], en = Ap[k + 1];
+        for (int t = st + sl; t < en; t += 4 * sg)
+        {
+            int col[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++)
+            {
+                // a slot past the row reloads its last entry and is dropped
+                const int tu = t + u * sg;
+                const int a = Ai[min(tu, en - 1)];
+                col[u] = tu < en ? a : lo - 1;
+            }
+#pragma unroll
+            for (int u = 0; u < 4; u++)
+            {
+                const unsigned o = (unsigned) (col[u] - lo);
+                if (o < (unsigned) W) mark[o] = 1;
+            }
+        }
+    }
+    __syncthreads();
+    // wavefront w owns window slots [w * PER, (w + 1) * PER)
+    const int s0 = wid * PER;
+    int cnt = 0;
+    for (int i = s0 + lane; i < s0 + PER; i += 64) cnt += __popcll(__ballot(mark[i] != 0));
+    if (lane == 0) wcnt[wid] = cnt;
+    __syncthreads();
+    if (PASS == 0)
+    {
+        if (threadIdx.x == 0)
+        {
+            int tot = 0;
+            for (int w = 0; w < NW; w++) tot += wcnt[w];
+            rowcnt[row] = tot;
+        }
+    }
+    else
+    {
+        // (the row's count came from the count pass; writes stay below the
+        // row's end whatever the marks)
+        int64_t pos = Cp[row];
+        const int64_t end = Cp[row + 1];
+        for (int w = 0; w < wid; w++) pos += wcnt[w];
+        const unsigned long long below = (1ull << lane) - 1ull;
+        for (int i = s0 + lane; i < s0 + PER; i += 64)
+        {
+            const bool on = mark[i] != 0;
+            const unsigned long long b = __ballot(on);
+            const int64_t at = pos + __popcll(b & below);
+            if (on && at < end) Ci[at] = lo + i;
+            pos += __popcll(b);
+        }
     }
 }
 
@@ -2447,7 +2585,8 @@ struct Bins
 
 template <typename T>
 void bin_rows(Bins &B, const T *cap, int64_t m, int nwords, const SymTune &tune, int mode = 0,
-              int t1 = 0, int t2 = 0)
+              int t1 = 0, int t2 = 0, const int *span = nullptr,
+              const int64_t *terms = nullptr)
 {
     DBuf<unsigned char> cls(m);
     DBuf<int> counts(NCLASS);
@@ -2455,8 +2594,8 @@ void bin_rows(Bins &B, const T *cap, int64_t m, int nwords, const SymTune &tune,
     unsigned grid = (unsigned) ((m + 255) / 256);
     if (m > 0)
         hipLaunchKernelGGL(k_classify<T>, dim3(grid), dim3(256), 0, stream(), m, cap, mode,
-                           nwords, lds_bitmap_words(), tune.hash_max, tune.dens, t1, t2, cls.p,
-                           counts.p);
+                           nwords, lds_bitmap_words(), tune.hash_max, tune.dens, t1, t2, span,
+                           terms, cls.p, counts.p);
     SDL_HIP(hipGetLastError());
     d2h(B.count.data(), counts.p, NCLASS * sizeof(int));
     B.off[0] = 0;
@@ -2511,6 +2650,37 @@ void run_symbolic_class(int c, const Bins &B, const ProductPlan &P, int nwords, 
                            (unsigned *) nullptr);
         break;
     }
+    case CLS_WIN0 + 0:
+    case CLS_WIN0 + 1:
+    case CLS_WIN0 + 2:
+    case CLS_WIN0 + 3:
+    case CLS_WIN0 + 4:
+    case CLS_WIN0 + 5:
+    {
+        SDL_REQUIRE(P.win_lo != nullptr, "symbolic: span window class without spans");
+        const int sg = P.win_sg;
+        SDL_REQUIRE(sg >= 4 && sg <= 64 && (sg & (sg - 1)) == 0, "symbolic: window lanes per entry");
+#define SDL_WIN_G(G, W)                                                                       \
+    hipLaunchKernelGGL((k_sym_window<G, W, PASS>), dim3(nrows), dim3(G), 0, stream(), rows,   \
+                       L.Lm, L.Lp, L.Li, nL, A.Ap, A.Ai, P.win_lo, sg, rowcnt, Cp, Ci)
+        // one wavefront per row up to 1024-column windows (measured fastest on
+        // banded and strided products: more rows in flight per CU), more for
+        // the larger windows, whose rows carry more terms
+        SDL_REQUIRE(sg <= 64, "symbolic: window lanes per entry");
+        switch (c - CLS_WIN0)
+        {
+        case 0: SDL_WIN_G(64, 256); break;
+        case 1: SDL_WIN_G(64, 512); break;
+        case 2: SDL_WIN_G(64, 1024); break;
+        case 3: SDL_WIN_G(128, 2048); break;
+        case 4: SDL_WIN_G(256, 4096); break;
+        case 5: SDL_WIN_G(256, 8192); break;
+        default: break;
+        }
+#undef SDL_WIN_G
+#undef SDL_WIN_CASE
+        break;
+    }
     case CLS_GLOBAL:
     {
         int grid = std::min(nrows, 512);
@@ -2533,7 +2703,8 @@ void run_symbolic_class(int c, const Bins &B, const ProductPlan &P, int nwords, 
 void symbolic_count(ProductPlan &P, const int64_t *cap, int *rowcnt, int nwords, bool first)
 {
     Bins B;
-    bin_rows<int64_t>(B, cap, P.m_out, nwords, P.tune);
+    bin_rows<int64_t>(B, cap, P.m_out, nwords, P.tune, 0, 0, 0,
+                      g_sym_window ? P.win_span : nullptr, P.win_terms);
     for (int c = 1; c < NCLASS; c++) run_symbolic_class<0>(c, B, P, nwords, rowcnt, nullptr, nullptr);
     if (first)
     {
@@ -2541,6 +2712,8 @@ void symbolic_count(ProductPlan &P, const int64_t *cap, int *rowcnt, int nwords,
         for (int c = 1; c < CLS_BITMAP; c++) P.stats.count_hash_rows += B.count[c];
         P.stats.count_bitmap_rows = B.count[CLS_BITMAP];
         P.stats.count_global_rows = B.count[CLS_GLOBAL];
+        P.stats.count_window_rows = 0;
+        for (int c = CLS_WIN0; c < NCLASS; c++) P.stats.count_window_rows += B.count[c];
     }
     sync();
 }
@@ -2554,11 +2727,13 @@ void symbolic_fill(ProductPlan &P, const int *rowcnt, int nwords)
     P.stats.nnz = P.nnz;
     P.Ci.alloc(P.nnz);
     Bins B2;
-    bin_rows<int>(B2, rowcnt, m_out, nwords, P.tune);
+    bin_rows<int>(B2, rowcnt, m_out, nwords, P.tune, 0, 0, 0,
+                  g_sym_window ? P.win_span : nullptr, P.win_terms);
     for (int c = 1; c < NCLASS; c++)
     {
         run_symbolic_class<1>(c, B2, P, nwords, nullptr, P.Cp.p, P.Ci.p);
         if (c >= 1 && c <= 8) P.stats.hash_rows += B2.count[c];
+        if (c >= CLS_WIN0) P.stats.window_rows += B2.count[c];
     }
     P.stats.bitmap_rows = B2.count[CLS_BITMAP];
     P.stats.global_rows = B2.count[CLS_GLOBAL];
@@ -2605,8 +2780,8 @@ void symbolic_predicted(ProductPlan &P, const RowPredictor *pred, const int64_t 
         P.stats.predicted_rows = 0;
         P.stats.overflow_rows = 0;
         P.stats.misbin_rows = 0;
-        P.stats.exact_rows =
-            P.stats.count_hash_rows + P.stats.count_bitmap_rows + P.stats.count_global_rows;
+        P.stats.exact_rows = P.stats.count_hash_rows + P.stats.count_bitmap_rows +
+                             P.stats.count_global_rows + P.stats.count_window_rows;
         return;
     }
     DBuf<unsigned char> pbin(m_out);
@@ -2819,9 +2994,15 @@ void product_plan(ProductPlan &P, const LView &L, const AView &A, SymMode mode,
         return;
     }
 
-    // per-row term bound and its scan
+    // per-row term bound, column span and the term scan
     DBuf<int64_t> rterms(m_out);
     DBuf<int64_t> rbound(m_out);
+    DBuf<int> wlo, wspan;
+    if (g_sym_window)
+    {
+        wlo.alloc(m_out);
+        wspan.alloc(m_out);
+    }
     {
         DBuf<int2> arange(std::max(A.Am, 1));
         if (A.Am > 0)
@@ -2829,13 +3010,30 @@ void product_plan(ProductPlan &P, const LView &L, const AView &A, SymMode mode,
                                stream(), A.Am, A.Ap, A.Ai, arange.p);
         hipLaunchKernelGGL(k_row_terms, dim3((unsigned) ((m_out + 15) / 16)), dim3(256), 0,
                            stream(), m_out, L.Lm, L.Lp, L.Li, nL, A.Ap, arange.p, rterms.p,
-                           rbound.p);
+                           rbound.p, wlo.p, wspan.p);
         SDL_HIP(hipGetLastError());
     }
     SDL_HIP(hipGetLastError());
     DBuf<int64_t> toff(m_out + 1);
     P.terms = scan_i64(rterms.p, toff.p, m_out, true);
     P.stats.terms = P.terms;
+    if (g_sym_window)
+    {
+        // lanes per L entry of the window kernels: a power of two in [4, 64]
+        // near 1/16 of the mean A-row length (terms per L entry): groups of a
+        // few lanes keep many L entries in flight, and four unrolled slots per
+        // lane still cover most of a row in a pass (measured on banded rows of
+        // 17..129 entries)
+        int nle = 0;
+        d2h(&nle, L.Lp + L.Lm, sizeof(int));
+        const double mean = (double) P.terms / std::max(1.0, (double) nle * L.blocks);
+        int sg = 4;
+        while (sg < 64 && 32.0 * sg <= mean) sg <<= 1;
+        P.win_sg = g_sym_window_lanes ? g_sym_window_lanes : sg;
+        P.win_lo = wlo.p;
+        P.win_span = wspan.p;
+        P.win_terms = rterms.p;
+    }
 
     // thresholds: forced (the tuner), the tuning table's for this shape, or
     // the defaults
@@ -2860,6 +3058,8 @@ void product_plan(ProductPlan &P, const LView &L, const AView &A, SymMode mode,
     if (mode == SymMode::Predicted)
         symbolic_predicted(P, pred, rterms.p, rbound.p, rowcnt.p, nwords);
     else symbolic_exact(P, rbound.p, rowcnt.p, nwords);
+    P.win_lo = P.win_span = nullptr;
+    P.win_terms = nullptr;
 
     // refill schedule (reference policy: a plan iff terms <= 3 nnz, gustavson.h:45-52)
     P.pull = allow_pull && P.terms > 0 && (force_pull || P.terms <= 3 * P.nnz) &&

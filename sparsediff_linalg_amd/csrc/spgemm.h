@@ -36,6 +36,11 @@ enum class SymMode
                     // over their predicted bound are counted again from the term bound
 };
 
+// Span windows in the symbolic phase: 0 off, 1 on (default).
+extern int g_sym_window;
+// Lanes per L entry of the window kernels (0: from the mean A-row length).
+extern int g_sym_window_lanes;
+
 struct SymStats
 {
     long long rows = 0, terms = 0, nnz = 0;
@@ -46,6 +51,7 @@ struct SymStats
     long long exact_rows = 0;          // predicted path: term bound kept (not lowered)
     long long candidate_rows = 0;      // predicted path: rows given features and the network
     long long count_hash_rows = 0, count_bitmap_rows = 0, count_global_rows = 0;  // first count pass
+    long long window_rows = 0, count_window_rows = 0;  // span windows: fill pass, first count pass
 };
 
 // Launch configuration of the batched pull refill (picked by the autotuner):
@@ -160,6 +166,12 @@ struct ProductPlan
     DBuf<RankRow> rrows;             // the refill's per-row descriptors
     DBuf<int2> qrange;               // one-block L: each L entry's A-row (start, length)
     SymStats stats;
+    // Span windows of the symbolic phase (set while the plan is built, then
+    // cleared): each output row's smallest column, column span and term
+    // count, and the lanes per L entry of the window kernels.
+    const int *win_lo = nullptr, *win_span = nullptr;
+    const int64_t *win_terms = nullptr;
+    int win_sg = 64;
     RefillCfg tuned;   // variant < 0: not tuned, use the default heuristic
     SymTune tune;      // thresholds the symbolic phase and the refill bins used
     ShapeSig sig;

@@ -231,15 +231,17 @@ def test_span_bound_sizes_the_count_pass():
     """A^T A of a strided band (4096 rows of 129 consecutive columns starting
     4 apart, in 2^20 columns): each output row has ~4160 terms, so the term
     bound would count it in a 2^20-bit column bitmap, but its columns span
-    ~253, so the exact path counts every row in an LDS hash table.  Structure
-    and values against the oracle."""
+    ~253, so the exact path counts every row in an LDS hash table or a span
+    window.  Structure and values against the oracle."""
     n = 1 << 20
     p, i, _ = W.strided_band_csr(1 << 12, n, 129, 4)
     pl = Plan((p, i), None, n, n)
     st = plan_stats(pl)
+    win = (C.c_longlong * 2)()
+    check(load().sdl_btda_plan_window_rows(pl.h, win), "window rows")
     cp, ci = pl.structure()
     nonempty = int(np.count_nonzero(np.diff(cp)))
-    assert st[8] == nonempty and st[9] == 0 and st[10] == 0, st
+    assert st[8] + win[0] == nonempty and st[9] == 0 and st[10] == 0, (st, list(win))
     d = np.linspace(0.5, 3.0, len(p) - 1)
     x, (Cp, Ci, Cx, _) = oracle_ata(p, i, n, d)
     assert np.array_equal(cp, Cp.astype(np.int32)) and np.array_equal(ci, Ci)
