@@ -218,6 +218,7 @@ SIGNATURES = {
     "sdl_btda_plan_window_rows": (C.c_int, [vp, c_ll_p]),
     "sdl_symbolic_window": (C.c_int, [C.c_int]),
     "sdl_symbolic_window_lanes": (C.c_int, [C.c_int]),
+    "sdl_rowacc_window": (C.c_int, [C.c_int]),
     # generators
     "sdl_gen_row_lengths_uniform": (C.c_longlong, [C.c_uint64, C.c_int, C.c_int, C.c_int,
                                                    c_int_p]),

@@ -153,6 +153,14 @@ int sdl_symbolic_window(int mode)
     return prev;
 }
 
+int sdl_rowacc_window(int mode)
+{
+    if (mode < -1 || mode > 1) return -1;
+    const int prev = sdl::g_rowacc_window;
+    if (mode >= 0) sdl::g_rowacc_window = mode;
+    return prev;
+}
+
 int sdl_symbolic_window_lanes(int lanes)
 {
     if (lanes < -1 || lanes > 64 || (lanes > 0 && (lanes < 4 || (lanes & (lanes - 1)))))

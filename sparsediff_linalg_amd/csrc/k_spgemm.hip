@@ -27,6 +27,7 @@ namespace sdl {
 
 int g_sym_window = 1;
 int g_sym_window_lanes = 0;
+int g_rowacc_window = 1;
 
 namespace {
 
@@ -473,7 +474,8 @@ __global__ void k_classify(int64_t m, const T *__restrict__ cap, int mode, int n
 // most rows of a product share a class, and per-row atomics on one counter
 // serialise.
 __global__ void k_scatter_rows(int64_t m, const unsigned char *__restrict__ cls,
-                               int *__restrict__ cursor, int *__restrict__ rows)
+                               int *__restrict__ cursor, int *__restrict__ rows,
+                               const int *__restrict__ ids = nullptr)
 {
     const int64_t row = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
     const int lane = threadIdx.x & 63;
@@ -487,7 +489,8 @@ __global__ void k_scatter_rows(int64_t m, const unsigned char *__restrict__ cls,
         int base = 0;
         if (lane == leader) base = atomicAdd(&cursor[cl], __popcll(same));
         base = __shfl(base, leader, 64);
-        if (c == cl) rows[base + __popcll(same & ((1ull << lane) - 1ull))] = (int) row;
+        if (c == cl)
+            rows[base + __popcll(same & ((1ull << lane) - 1ull))] = ids ? ids[row] : (int) row;
         todo &= ~same;
     }
 }
@@ -1883,6 +1886,143 @@ __global__ void __launch_bounds__(G)
         for (int i = threadIdx.x; i < cnt; i += G) Cx[cbase + i] = s_val[i];
 }
 
+// Window class of a row-accumulator row (ids[i]): 1 when its column span
+// exceeds 1024 (the LDS accumulator with a column search), else 2 / 3 / 4 for
+// a span of at most 256 / 512 / 1024 columns.  Rows hold at least one entry,
+// in ascending columns.
+__global__ void k_rowacc_span_class(int n, const int *__restrict__ ids,
+                                    const int64_t *__restrict__ Cp, const int *__restrict__ Ci,
+                                    unsigned char *__restrict__ cls, int *__restrict__ counts)
+{
+    __shared__ int hist[5];
+    if (threadIdx.x < 5) hist[threadIdx.x] = 0;
+    __syncthreads();
+    const int i = (int) (blockIdx.x * blockDim.x + threadIdx.x);
+    if (i < n)
+    {
+        const int row = ids[i];
+        const int64_t b = Cp[row], e = Cp[row + 1];
+        const int span = Ci[e - 1] - Ci[b] + 1;
+        const int c = span <= 256 ? 2 : span <= 512 ? 3 : span <= 1024 ? 4 : 1;
+        cls[i] = (unsigned char) c;
+        atomicAdd(&hist[c], 1);
+    }
+    __syncthreads();
+    if (threadIdx.x < 5 && hist[threadIdx.x]) atomicAdd(&counts[threadIdx.x], hist[threadIdx.x]);
+}
+
+// Ordered row accumulator, one wavefront per row, for rows whose columns span
+// at most WS: a dense LDS accumulator over [lo, lo + WS) (lo = the row's first
+// column) indexed by column - lo, so a term needs no column search.  k walks
+// L's row in storage order; the columns of one A row are distinct, so one
+// A row's terms update distinct slots together, and the wavefront's LDS
+// accesses complete in issue order, so consecutive k update a slot in
+// ascending k with no barrier.  The row is then gathered through its columns.
+// Descriptors of 64 L entries at a time are loaded one per lane and read with
+// readlane; the first U * 64 terms of each A row are loaded before the
+// previous row's updates (index 0 stands in for a slot without a term:
+// unconditional loads measured faster than masked ones unless whole chunks
+// are empty, so U covers the mean A row).
+template <int WS, int U>
+__global__ void __launch_bounds__(256)
+    k_num_rowacc_win(const int *__restrict__ rows, int nrows, int Lm, const int *__restrict__ Lp,
+                     const int *__restrict__ Li, const int *__restrict__ Lperm, int nL,
+                     const int *__restrict__ Ap, const int *__restrict__ Ai,
+                     const int64_t *__restrict__ Cp, const int *__restrict__ Ci,
+                     const double *__restrict__ Lval, const double *__restrict__ d,
+                     const double *__restrict__ Ax, double *__restrict__ Cx)
+{
+    __shared__ double acc[4][WS];
+    const int w = (int) threadIdx.x >> 6, lane = (int) threadIdx.x & 63;
+    const int ri = (int) blockIdx.x * 4 + w;
+    if (ri >= nrows) return;  // whole wavefronts; no workgroup barrier follows
+    double *a = acc[w];
+    const int row = rows[ri];
+    const int64_t cb = Cp[row];
+    const int cnt = (int) (Cp[row + 1] - cb);
+    const int lo = Ci[cb];
+    for (int i = lane; i < WS; i += 64) a[i] = 0.0;
+    const RowCtx c = row_ctx(row, Lm, Lp, nL);
+    for (int qb = c.q0; qb < c.q1; qb += 64)
+    {
+        const int nq = min(64, c.q1 - qb);
+        int ma = 0, mb = 0;
+        double mv = 0.0;
+        if (lane < nq)
+        {
+            const int q = qb + lane;
+            const int k = Li[q] + c.koff;
+            double v = Lval[Lperm ? Lperm[q] : q];
+            if (d) v = v * d[k];
+            ma = Ap[k];
+            mb = Ap[k + 1];
+            mv = v;
+        }
+        const int mvlo = __double2loint(mv), mvhi = __double2hiint(mv);
+        int ta = __builtin_amdgcn_readlane(ma, 0), tb = __builtin_amdgcn_readlane(mb, 0);
+        int colc[U];
+        double xc[U];
+#pragma unroll
+        for (int u = 0; u < U; u++)
+        {
+            const int t = ta + lane + 64 * u;
+            colc[u] = Ai[t < tb ? t : 0];
+            xc[u] = Ax[t < tb ? t : 0];
+        }
+        for (int j = 0; j < nq; j++)
+        {
+            const int a0 = ta, b0 = tb;
+            const double v = __hiloint2double(__builtin_amdgcn_readlane(mvhi, j),
+                                              __builtin_amdgcn_readlane(mvlo, j));
+            const bool more = j + 1 < nq;
+            const int jn = min(j + 1, 63);
+            ta = more ? __builtin_amdgcn_readlane(ma, jn) : 0;
+            tb = more ? __builtin_amdgcn_readlane(mb, jn) : 0;
+            int coln[U];
+            double xn[U];
+#pragma unroll
+            for (int u = 0; u < U; u++)
+            {
+                const int t = ta + lane + 64 * u;
+                coln[u] = Ai[t < tb ? t : 0];
+                xn[u] = Ax[t < tb ? t : 0];
+            }
+            // the U chunks' slots are distinct (one A row's columns): all
+            // reads, then all writes (a slot without a term reads slot 0 and
+            // writes nothing)
+            unsigned o[U];
+            double old[U];
+#pragma unroll
+            for (int u = 0; u < U; u++)
+            {
+                o[u] = a0 + lane + 64 * u < b0 ? (unsigned) (colc[u] - lo) : (unsigned) WS;
+                old[u] = a[o[u] < (unsigned) WS ? o[u] : 0u];
+            }
+#pragma unroll
+            for (int u = 0; u < U; u++)
+                if (o[u] < (unsigned) WS) a[o[u]] = old[u] + v * xc[u];
+            for (int t = a0 + lane + 64 * U; t < b0; t += 64)  // A rows longer than 64 U
+            {
+                const unsigned o = (unsigned) (Ai[t] - lo);
+                if (o < (unsigned) WS) a[o] = a[o] + v * Ax[t];
+            }
+            // (no code motion of memory operations across consecutive k)
+            __builtin_amdgcn_wave_barrier();
+#pragma unroll
+            for (int u = 0; u < U; u++)
+            {
+                colc[u] = coln[u];
+                xc[u] = xn[u];
+            }
+        }
+    }
+    for (int i = lane; i < cnt; i += 64)
+    {
+        const unsigned o = (unsigned) (Ci[cb + i] - lo);
+        Cx[cb + i] = o < (unsigned) WS ? a[o] : 0.0;
+    }
+}
+
 // Long rows with a dense accumulator of n doubles per workgroup (the
 // reference's acc[n], gustavson.c:107-143): a persistent grid strides over the
 // rows; per row the entries' slots are zeroed, L's row is walked in ascending
@@ -3017,6 +3157,11 @@ void product_plan(ProductPlan &P, const LView &L, const AView &A, SymMode mode,
     DBuf<int64_t> toff(m_out + 1);
     P.terms = scan_i64(rterms.p, toff.p, m_out, true);
     P.stats.terms = P.terms;
+    {
+        int nle = 0;
+        d2h(&nle, L.Lp + L.Lm, sizeof(int));
+        P.mean_arow = (double) P.terms / std::max(1.0, (double) nle * L.blocks);
+    }
     if (g_sym_window)
     {
         // lanes per L entry of the window kernels: a power of two in [4, 64]
@@ -3024,9 +3169,7 @@ void product_plan(ProductPlan &P, const LView &L, const AView &A, SymMode mode,
         // few lanes keep many L entries in flight, and four unrolled slots per
         // lane still cover most of a row in a pass (measured on banded rows of
         // 17..129 entries)
-        int nle = 0;
-        d2h(&nle, L.Lp + L.Lm, sizeof(int));
-        const double mean = (double) P.terms / std::max(1.0, (double) nle * L.blocks);
+        const double mean = P.mean_arow;
         int sg = 4;
         while (sg < 64 && 32.0 * sg <= mean) sg <<= 1;
         P.win_sg = g_sym_window_lanes ? g_sym_window_lanes : sg;
@@ -3135,6 +3278,33 @@ void product_plan(ProductPlan &P, const LView &L, const AView &A, SymMode mode,
         d2d(P.rbins.p, BR.rows.p + BR.off[1], (size_t) small * sizeof(int));
         d2d(P.rbins.p + small, BR.rows.p + BR.off[2], (size_t) mid * sizeof(int));
         d2d(P.rbins.p + small + mid, BR.rows.p + BR.off[3], (size_t) large * sizeof(int));
+        // the LDS-accumulator rows by column span: rows within 256 / 512 /
+        // 1024 columns take the window accumulators (after the others)
+        for (int c = 0; c < 5; c++) P.rwin_off[c] = c == 0 ? 0 : small;
+        if (g_rowacc_window && small > 0)
+        {
+            DBuf<unsigned char> cls(small);
+            DBuf<int> counts(5), tmp(small);
+            dzero(counts.p, counts.bytes());
+            const unsigned g = (unsigned) ((small + 255) / 256);
+            hipLaunchKernelGGL(k_rowacc_span_class, dim3(g), dim3(256), 0, stream(), small,
+                               P.rbins.p, P.Cp.p, P.Ci.p, cls.p, counts.p);
+            SDL_HIP(hipGetLastError());
+            int cnt[5];
+            d2h(cnt, counts.p, sizeof(cnt));
+            int cur[5] = {0, 0, 0, 0, 0};
+            for (int c = 1; c < 5; c++)
+            {
+                cur[c] = P.rwin_off[c - 1];
+                P.rwin_off[c] = P.rwin_off[c - 1] + cnt[c];
+            }
+            SDL_REQUIRE(P.rwin_off[4] == small, "product plan: window classes do not cover the rows");
+            h2d(counts.p, cur, sizeof(cur));
+            hipLaunchKernelGGL(k_scatter_rows, dim3(g), dim3(256), 0, stream(), (int64_t) small,
+                               cls.p, counts.p, tmp.p, (const int *) P.rbins.p);
+            SDL_HIP(hipGetLastError());
+            d2d(P.rbins.p, tmp.p, (size_t) small * sizeof(int));
+        }
         P.rank = mid > 0 && nwords <= RANK_MAX_WORDS;
         if (P.rank)
         {
@@ -3182,10 +3352,27 @@ void product_numeric(const ProductPlan &P, const double *Lval, const double *d,
         const int n0 = P.rbin_off[1] - P.rbin_off[0];
         const int *long_rows = P.rbins.p + P.rbin_off[1];
         int n1 = P.rbin_off[3] - P.rbin_off[1];
-        if (n0)
-            hipLaunchKernelGGL((k_num_rowacc<256, 1024>), dim3(n0), dim3(256), 0, s, P.rbins.p,
+        const int nsearch = P.rwin_off[1];
+        if (nsearch)
+            hipLaunchKernelGGL((k_num_rowacc<256, 1024>), dim3(nsearch), dim3(256), 0, s,
+                               P.rbins.p, P.L.Lm, P.L.Lp, P.L.Li, P.L.Lperm, nL, P.A.Ap, P.A.Ai,
+                               P.Cp.p, P.Ci.p, Lval, d, Ax, Cx);
+        for (int c = 2; c < 5 && n0; c++)
+        {
+            const int nr = P.rwin_off[c] - P.rwin_off[c - 1];
+            if (nr == 0) continue;
+            const int *wr = P.rbins.p + P.rwin_off[c - 1];
+            // A-row chunks of 64 loaded ahead: enough for the mean A row
+            const int u = P.mean_arow <= 64 ? 1 : P.mean_arow <= 128 ? 2 : P.mean_arow <= 192 ? 3 : 4;
+#define SDL_RW(WS)                                                                         \
+    (u == 1 ? k_num_rowacc_win<WS, 1> : u == 2 ? k_num_rowacc_win<WS, 2>                   \
+                                      : u == 3 ? k_num_rowacc_win<WS, 3> : k_num_rowacc_win<WS, 4>)
+            auto fn = c == 2 ? SDL_RW(256) : c == 3 ? SDL_RW(512) : SDL_RW(1024);
+#undef SDL_RW
+            hipLaunchKernelGGL(fn, dim3((unsigned) ((nr + 3) / 4)), dim3(256), 0, s, wr, nr,
                                P.L.Lm, P.L.Lp, P.L.Li, P.L.Lperm, nL, P.A.Ap, P.A.Ai, P.Cp.p,
                                P.Ci.p, Lval, d, Ax, Cx);
+        }
         if (P.rank)
         {
             // rows of the rank bin; its spills and the longer rows follow

@@ -40,6 +40,9 @@ enum class SymMode
 extern int g_sym_window;
 // Lanes per L entry of the window kernels (0: from the mean A-row length).
 extern int g_sym_window_lanes;
+// Window accumulators for row-accumulator rows of narrow column span: 0 off,
+// 1 on (default); read when a plan is built.
+extern int g_rowacc_window;
 
 struct SymStats
 {
@@ -138,6 +141,10 @@ struct ProductPlan
     // column bitmap fits LDS), [2, 3) longer.
     DBuf<int> rbins;                 // row ids grouped by class
     int rbin_off[8] = {0};           // class c is rbins[rbin_off[c] .. rbin_off[c+1])
+    // bin 0 split by column span: [0, rwin_off[1]) the LDS accumulator with a
+    // column search, then the window accumulators of 256 / 512 / 1024 columns
+    int rwin_off[5] = {0};
+    double mean_arow = 0;            // terms per L entry (the mean A-row length)
     bool rank = false;               // bin 1 by rank; its spills in spill[0 .. *nspill)
     int rank_grid = 0;
     DBuf<int> spill, nspill;

@@ -1,0 +1,77 @@
+"""Row-accumulator refills (plans without a pull schedule: terms > 3 nnz, the
+reference's accumulator policy, gustavson.h:45-52) of rows whose columns span
+at most 1024 columns run one wavefront per row with a dense LDS accumulator
+over the span; the other rows keep the workgroup accumulator with a column
+search.  Both fold each entry's terms from +0.0 in L storage order
+(gustavson.c:117-141), so refills with the windows on and off must agree bit
+for bit, and with the oracle's values.  Cases: a band (spans of 129), B^T A
+with spans of 200..12000 columns (every window size, the search rows and
+rows beyond the 1024-entry bin), and rows drawn from a few wide patterns
+(many terms per entry, spans too wide for a window)."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from sparsediff_linalg_amd import DeviceArray, check, load, workloads as W
+from sparsediff_linalg_amd.predictor import Plan
+from tests.test_symbolic_window import oracle_btda, window_ladder
+
+pytestmark = pytest.mark.gpu
+
+
+def patterns(seed, m=3000, n=5000, npat=12, width=90):
+    """m rows, each a copy of one of npat random patterns of `width` columns
+    in [0, n): every output entry of A^T A collects ~m / npat terms."""
+    rng = np.random.default_rng(seed)
+    pats = [np.sort(rng.choice(n, width, replace=False)) for _ in range(npat)]
+    which = rng.integers(0, npat, m)
+    p = np.concatenate([[0], np.cumsum([width] * m)]).astype(np.int32)
+    return p, np.concatenate([pats[w] for w in which]).astype(np.int32), n
+
+
+def cases():
+    p, i, _ = W.banded_csr(4096, 32)
+    yield "band32", (p, i), None, 4096, 4096
+    A, B, nA, nB = window_ladder(5, blocks=6)
+    yield "window-ladder", A, B, nA, nB
+    p, i, n = patterns(3)
+    yield "patterns", (p, i), None, n, n
+
+
+def refill(A, B, nA, nB, xA, xB, d, mode):
+    lib = load()
+    prev = lib.sdl_rowacc_window(mode)
+    try:
+        pl = Plan(A, B, nA, nB)
+    finally:
+        lib.sdl_rowacc_window(prev)
+    assert not lib.sdl_btda_plan_is_pull(pl.h)
+    dA, dB, dd = (DeviceArray.from_numpy(v) for v in (xA, xB, d))
+    out = DeviceArray(max(pl.nnz(), 1), np.float64)
+    check(lib.sdl_btda_refill(pl.h, dA.ptr, dB.ptr, dd.ptr, out.ptr, None), "refill")
+    lib.sdl_synchronize()
+    cp, ci = pl.structure()
+    x = out.numpy()[:pl.nnz()].copy()
+    pl.close()
+    return cp, ci, x
+
+
+@pytest.mark.parametrize("name", ["band32", "window-ladder", "patterns"])
+def test_window_accumulator_matches_oracle_and_search(name):
+    A, B, nA, nB = next((c[1], c[2], c[3], c[4]) for c in cases() if c[0] == name)
+    m = len(A[0]) - 1
+    xA, xB, d, Cp, Ci, Cx = oracle_btda(A, B, m, nA, nB)
+    on = refill(A, B, nA, nB, xA, xB, d, 1)
+    off = refill(A, B, nA, nB, xA, xB, d, 0)
+    for cp, ci, x in (on, off):
+        assert np.array_equal(cp, Cp.astype(np.int32)) and np.array_equal(ci, Ci), name
+        assert np.array_equal(x.view(np.int64), Cx.view(np.int64)), name
+
+
+def test_rowacc_window_knob():
+    lib = load()
+    prev = lib.sdl_rowacc_window(-1)
+    assert prev in (0, 1)
+    assert lib.sdl_rowacc_window(2) == -1 and lib.sdl_rowacc_window(-2) == -1
+    assert lib.sdl_rowacc_window(prev) == prev
