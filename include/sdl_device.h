@@ -110,6 +110,12 @@ int sdl_symbolic_window_lanes(int lanes);
    queries.  Both give the same values bit for bit.  Returns the previous
    mode, or -1 for an unknown one. */
 int sdl_rowacc_window(int mode);
+/* Term expansion of the symbolic phase's LDS column bitmaps: 1 (default)
+   groups of lanes per L entry striding over its A row (as the span windows),
+   0 the load-balanced expansion (a binary search per term); -1 only queries.
+   Both give the same structure.  Returns the previous mode, or -1 for an
+   unknown one. */
+int sdl_symbolic_lane_groups(int mode);
 /* Refill schedule (5 values): pull schedule (0/1), by-rank rows (row-
    accumulator plans), recorded positions for them (0/1), then with recorded
    positions the listed terms of entries of several terms and the rows left

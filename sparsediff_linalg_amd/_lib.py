@@ -219,6 +219,7 @@ SIGNATURES = {
     "sdl_symbolic_window": (C.c_int, [C.c_int]),
     "sdl_symbolic_window_lanes": (C.c_int, [C.c_int]),
     "sdl_rowacc_window": (C.c_int, [C.c_int]),
+    "sdl_symbolic_lane_groups": (C.c_int, [C.c_int]),
     # generators
     "sdl_gen_row_lengths_uniform": (C.c_longlong, [C.c_uint64, C.c_int, C.c_int, C.c_int,
                                                    c_int_p]),

@@ -153,6 +153,14 @@ int sdl_symbolic_window(int mode)
     return prev;
 }
 
+int sdl_symbolic_lane_groups(int mode)
+{
+    if (mode < -1 || mode > 1) return -1;
+    const int prev = sdl::g_sym_groups;
+    if (mode >= 0) sdl::g_sym_groups = mode;
+    return prev;
+}
+
 int sdl_rowacc_window(int mode)
 {
     if (mode < -1 || mode > 1) return -1;

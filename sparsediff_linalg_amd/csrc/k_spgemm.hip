@@ -28,6 +28,7 @@ namespace sdl {
 int g_sym_window = 1;
 int g_sym_window_lanes = 0;
 int g_rowacc_window = 1;
+int g_sym_groups = 1;
 
 namespace {
 
@@ -233,6 +234,39 @@ __device__ __forceinline__ void for_each_col(Expand<G> &ex, int q0, int q1, int 
                 if (col[u] >= 0) f(col[u]);
         }
         __syncthreads();
+    }
+}
+
+// Calls f(col) for the column of every term of output row (b, r): groups of
+// sg lanes (sg divides G) take one L entry at a time and stride over its A
+// row, U loads in flight per lane (a slot past the row reloads its last
+// entry and is dropped).  No workgroup barrier; the terms of one L entry are
+// visited by one group.
+template <int G, int U = 4, typename F>
+__device__ __forceinline__ void for_each_col_groups(int sg, int q0, int q1, int koff,
+                                                    const int *__restrict__ Li,
+                                                    const int *__restrict__ Ap,
+                                                    const int *__restrict__ Ai, F &&f)
+{
+    const int grp = (int) threadIdx.x / sg, ngrp = G / sg, sl = (int) threadIdx.x - grp * sg;
+    for (int q = q0 + grp; q < q1; q += ngrp)
+    {
+        const int k = Li[q] + koff;
+        const int st = Ap[k], en = Ap[k + 1];
+        for (int t = st + sl; t < en; t += U * sg)
+        {
+            int col[U];
+#pragma unroll
+            for (int u = 0; u < U; u++)
+            {
+                const int tu = t + u * sg;
+                const int a = Ai[min(tu, en - 1)];
+                col[u] = tu < en ? a : -1;
+            }
+#pragma unroll
+            for (int u = 0; u < U; u++)
+                if (col[u] >= 0) f(col[u]);
+        }
     }
 }
 
@@ -501,7 +535,7 @@ __global__ void __launch_bounds__(G)
     k_sym_hash(const int *__restrict__ rows, int Lm, const int *__restrict__ Lp,
                const int *__restrict__ Li, int nL, const int *__restrict__ Ap,
                const int *__restrict__ Ai, int *__restrict__ rowcnt,
-               const int64_t *__restrict__ Cp, int *__restrict__ Ci)
+               const int64_t *__restrict__ Cp, int *__restrict__ Ci, int sg)
 {
     __shared__ int tab[S];
     __shared__ Expand<G> ex;
@@ -517,7 +551,8 @@ __global__ void __launch_bounds__(G)
     // counted again from its term bound, so its remaining terms are skipped:
     // without this, a row far above its table would probe the full table for
     // every new column.  Exact bounds never get there.
-    for_each_col<G, 4>(ex, c.q0, c.q1, c.koff, Li, Ap, Ai, [&](int col) {
+    // (lane groups per L entry when sg > 0, else the load-balanced expansion)
+    auto put = [&](int col) {
         if (PASS == 0)
         {
             if (*(volatile int *) &filled > S / 2) return;
@@ -527,7 +562,10 @@ __global__ void __launch_bounds__(G)
         {
             hash_insert<S>(tab, col);
         }
-    });
+    };
+    if (sg > 0) for_each_col_groups<G>(sg, c.q0, c.q1, c.koff, Li, Ap, Ai, put);
+    else for_each_col<G, 4>(ex, c.q0, c.q1, c.koff, Li, Ap, Ai, put);
+    __syncthreads();
     if (PASS == 0)
     {
         long long cnt = 0;
@@ -552,7 +590,7 @@ __global__ void __launch_bounds__(G)
                  const int *__restrict__ Lp, const int *__restrict__ Li, int nL,
                  const int *__restrict__ Ap, const int *__restrict__ Ai,
                  int *__restrict__ rowcnt, const int64_t *__restrict__ Cp,
-                 int *__restrict__ Ci, unsigned *__restrict__ gscratch)
+                 int *__restrict__ Ci, unsigned *__restrict__ gscratch, int sg)
 {
     extern __shared__ unsigned lds_bm[];
     __shared__ Expand<G> ex;
@@ -577,36 +615,74 @@ __global__ void __launch_bounds__(G)
         if (GLOBAL) __threadfence();
         __syncthreads();
         RowCtx c = row_ctx(row, Lm, Lp, nL);
-        // (four loads in flight for LDS bitmaps; one for global ones, whose L2
-        // atomics measured slower batched)
-        for_each_col<G, GLOBAL ? 1 : 4>(ex, c.q0, c.q1, c.koff, Li, Ap, Ai,
-                                        [&](int col) { atomicOr(&bm[col >> 5], 1u << (col & 31)); });
+        // LDS bitmaps: lane groups per L entry (sg > 0), 16 loads in flight
+        // per lane (one workgroup per CU: the in-flight loads set the read
+        // rate over the rows' long A rows; 4 / 8 measured slower), or the
+        // load-balanced expansion; global ones: the expansion with one load in
+        // flight (their L2 atomics measured slower batched)
+        auto put = [&](int col) { atomicOr(&bm[col >> 5], 1u << (col & 31)); };
+        if (!GLOBAL && sg > 0) for_each_col_groups<G, 16>(sg, c.q0, c.q1, c.koff, Li, Ap, Ai, put);
+        else for_each_col<G, GLOBAL ? 1 : 4>(ex, c.q0, c.q1, c.koff, Li, Ap, Ai, put);
         if (GLOBAL) __threadfence();
         __syncthreads();
-        // contiguous word range per thread keeps the output ascending
-        int per = (nwords + G - 1) / G;
-        int w0 = min(nwords, (int) threadIdx.x * per), w1 = min(nwords, w0 + per);
-        int cnt = 0;
-        for (int w = w0; w < w1; w++) cnt += __popc(word(w));
         if (PASS == 0)
         {
+            // lanes read consecutive words (no LDS bank conflicts)
+            int cnt = 0;
+            for (int w = (int) threadIdx.x; w < nwords; w += G) cnt += __popc(word(w));
             long long tot = block_sum_ll<G>((long long) cnt, red);
             if (threadIdx.x == 0) rowcnt[row] = (int) tot;
         }
         else
         {
-            int tot;
-            int pos = block_excl_scan_int<G>(cnt, &tot, wsum);
-            int64_t base = Cp[row] + pos;
-            for (int w = w0; w < w1; w++)
+            // wavefront v owns words [v * pw, (v + 1) * pw), 256 at a time,
+            // four consecutive words per lane: the wavefronts' counts give
+            // their output offsets, then each chunk's columns go out in
+            // ascending order at the chunk's running offset plus the lane's
+            // exclusive prefix of the chunk's counts (one prefix per 256 words)
+            constexpr int R = 4;
+            const int lane = (int) threadIdx.x & 63, wv = (int) threadIdx.x >> 6;
+            const int pw = (nwords + G - 1) / G * 64;
+            const int w0 = min(nwords, wv * pw), w1 = min(nwords, w0 + pw);
+            int cnt = 0;
+            for (int w = w0 + lane; w < w1; w += 64) cnt += __popc(word(w));
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) cnt += __shfl_xor(cnt, off, 64);
+            if (lane == 0) wsum[wv] = cnt;
+            __syncthreads();
+            int64_t base = Cp[row];
+            for (int v = 0; v < wv; v++) base += wsum[v];
+            for (int wb = w0; wb < w1; wb += 64 * R)
             {
-                unsigned bits = word(w);
-                while (bits)
+                const int wl = wb + lane * R;
+                unsigned bits[R];
+                int c = 0;
+#pragma unroll
+                for (int r = 0; r < R; r++)
                 {
-                    int b = __ffs(bits) - 1;
-                    bits &= bits - 1;
-                    Ci[base++] = w * 32 + b;
+                    bits[r] = wl + r < w1 ? word(wl + r) : 0u;
+                    c += __popc(bits[r]);
                 }
+                int inc = c;
+#pragma unroll
+                for (int off = 1; off < 64; off <<= 1)
+                {
+                    const int y = __shfl_up(inc, off, 64);
+                    if (lane >= off) inc += y;
+                }
+                int64_t at = base + inc - c;
+#pragma unroll
+                for (int r = 0; r < R; r++)
+                {
+                    unsigned b32 = bits[r];
+                    while (b32)
+                    {
+                        const int b = __ffs(b32) - 1;
+                        b32 &= b32 - 1;
+                        Ci[at++] = (wl + r) * 32 + b;
+                    }
+                }
+                base += __shfl(inc, 63, 64);
             }
         }
         __syncthreads();
@@ -640,30 +716,10 @@ __global__ void __launch_bounds__(G)
     for (int i = (int) threadIdx.x; i < W; i += G) mark[i] = 0;
     __syncthreads();
     const RowCtx c = row_ctx(row, Lm, Lp, nL);
-    const int grp = (int) threadIdx.x / sg, ngrp = G / sg, sl = (int) threadIdx.x - grp * sg;
-    for (int q = c.q0 + grp; q < c.q1; q += ngrp)
-    {
-        const int k = Li[q] + c.koff;
-        const int st = Ap[k], en = Ap[k + 1];
-        for (int t = st + sl; t < en; t += 4 * sg)
-        {
-            int col[4];
-#pragma unroll
-            for (int u = 0; u < 4; u++)
-            {
-                // a slot past the row reloads its last entry and is dropped
-                const int tu = t + u * sg;
-                const int a = Ai[min(tu, en - 1)];
-                col[u] = tu < en ? a : lo - 1;
-            }
-#pragma unroll
-            for (int u = 0; u < 4; u++)
-            {
-                const unsigned o = (unsigned) (col[u] - lo);
-                if (o < (unsigned) W) mark[o] = 1;
-            }
-        }
-    }
+    for_each_col_groups<G>(sg, c.q0, c.q1, c.koff, Li, Ap, Ai, [&](int col) {
+        const unsigned o = (unsigned) (col - lo);
+        if (o < (unsigned) W) mark[o] = 1;
+    });
     __syncthreads();
     // wavefront w owns window slots [w * PER, (w + 1) * PER)
     const int s0 = wid * PER;
@@ -2764,7 +2820,8 @@ void run_symbolic_class(int c, const Bins &B, const ProductPlan &P, int nwords, 
 #define SDL_HASH_CASE(CLS, G, S)                                                          \
     case CLS:                                                                             \
         hipLaunchKernelGGL((k_sym_hash<G, S, PASS>), dim3(nrows), dim3(G), 0, stream(), rows, \
-                           L.Lm, L.Lp, L.Li, nL, A.Ap, A.Ai, rowcnt, Cp, Ci);            \
+                           L.Lm, L.Lp, L.Li, nL, A.Ap, A.Ai, rowcnt, Cp, Ci,             \
+                           g_sym_groups ? std::min(P.sym_sg, G) : 0);                    \
         break;
     switch (c)
     {
@@ -2787,7 +2844,7 @@ void run_symbolic_class(int c, const Bins &B, const ProductPlan &P, int nwords, 
                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds));
         hipLaunchKernelGGL(fn, dim3(nrows), dim3(wide ? 1024 : 256), lds, stream(), rows, nrows,
                            nwords, L.Lm, L.Lp, L.Li, nL, A.Ap, A.Ai, rowcnt, Cp, Ci,
-                           (unsigned *) nullptr);
+                           (unsigned *) nullptr, g_sym_groups ? P.sym_sg : 0);
         break;
     }
     case CLS_WIN0 + 0:
@@ -2798,7 +2855,7 @@ void run_symbolic_class(int c, const Bins &B, const ProductPlan &P, int nwords, 
     case CLS_WIN0 + 5:
     {
         SDL_REQUIRE(P.win_lo != nullptr, "symbolic: span window class without spans");
-        const int sg = P.win_sg;
+        const int sg = P.sym_sg;
         SDL_REQUIRE(sg >= 4 && sg <= 64 && (sg & (sg - 1)) == 0, "symbolic: window lanes per entry");
 #define SDL_WIN_G(G, W)                                                                       \
     hipLaunchKernelGGL((k_sym_window<G, W, PASS>), dim3(nrows), dim3(G), 0, stream(), rows,   \
@@ -2827,7 +2884,7 @@ void run_symbolic_class(int c, const Bins &B, const ProductPlan &P, int nwords, 
         DBuf<unsigned> scratch((size_t) grid * nwords);
         hipLaunchKernelGGL((k_sym_bitmap<256, PASS, true>), dim3(grid), dim3(256), 0,
                            stream(), rows, nrows, nwords, L.Lm, L.Lp, L.Li, nL, A.Ap, A.Ai,
-                           rowcnt, Cp, Ci, scratch.p);
+                           rowcnt, Cp, Ci, scratch.p, 0);
         SDL_HIP(hipGetLastError());
         sync();
         break;
@@ -3162,9 +3219,8 @@ void product_plan(ProductPlan &P, const LView &L, const AView &A, SymMode mode,
         d2h(&nle, L.Lp + L.Lm, sizeof(int));
         P.mean_arow = (double) P.terms / std::max(1.0, (double) nle * L.blocks);
     }
-    if (g_sym_window)
     {
-        // lanes per L entry of the window kernels: a power of two in [4, 64]
+        // lanes per L entry of the window and LDS-bitmap kernels: a power of two in [4, 64]
         // near 1/16 of the mean A-row length (terms per L entry): groups of a
         // few lanes keep many L entries in flight, and four unrolled slots per
         // lane still cover most of a row in a pass (measured on banded rows of
@@ -3172,7 +3228,10 @@ void product_plan(ProductPlan &P, const LView &L, const AView &A, SymMode mode,
         const double mean = P.mean_arow;
         int sg = 4;
         while (sg < 64 && 32.0 * sg <= mean) sg <<= 1;
-        P.win_sg = g_sym_window_lanes ? g_sym_window_lanes : sg;
+        P.sym_sg = g_sym_window_lanes ? g_sym_window_lanes : sg;
+    }
+    if (g_sym_window)
+    {
         P.win_lo = wlo.p;
         P.win_span = wspan.p;
         P.win_terms = rterms.p;

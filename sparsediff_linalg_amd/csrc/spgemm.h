@@ -40,6 +40,9 @@ enum class SymMode
 extern int g_sym_window;
 // Lanes per L entry of the window kernels (0: from the mean A-row length).
 extern int g_sym_window_lanes;
+// Lane groups per L entry in the LDS column-bitmap kernels: 0 off (the
+// load-balanced term expansion), 1 on (default).
+extern int g_sym_groups;
 // Window accumulators for row-accumulator rows of narrow column span: 0 off,
 // 1 on (default); read when a plan is built.
 extern int g_rowacc_window;
@@ -178,7 +181,7 @@ struct ProductPlan
     // count, and the lanes per L entry of the window kernels.
     const int *win_lo = nullptr, *win_span = nullptr;
     const int64_t *win_terms = nullptr;
-    int win_sg = 64;
+    int sym_sg = 64;
     RefillCfg tuned;   // variant < 0: not tuned, use the default heuristic
     SymTune tune;      // thresholds the symbolic phase and the refill bins used
     ShapeSig sig;

@@ -1,7 +1,8 @@
 """Span windows in the symbolic phase: rows whose columns all lie in a span of
 at most 8192 columns are counted and filled in an LDS mark array over that
-span (no hash table, no sort).  With windows off and on, and at every
-lanes-per-L-entry setting, plans must give the
+span (no hash table, no sort).  With windows off and on, at every
+lanes-per-L-entry setting, and with the hash and bitmap kernels expanding
+terms by lane groups or by binary search, plans must give the
 oracle's structure (gustavson.c:43-68 count, :207-231 fill order: ascending
 columns per row), and a refill over a window-built plan the oracle's values.
 Cases: banded and strided bands (narrow spans), spans of every window size
@@ -87,11 +88,15 @@ def cases():
     yield "band-btda", (p, i), (bp, bi), 5000, 5000, True
 
 
-@pytest.mark.parametrize("mode,lanes", [(0, 0), (1, 0), (1, 4), (1, 16), (1, 64)])
-def test_window_structure_matches_oracle(mode, lanes):
+@pytest.mark.parametrize("mode,lanes,groups", [(0, 0, 0), (0, 0, 1), (0, 4, 1), (1, 0, 1),
+                                               (1, 4, 1), (1, 16, 1), (1, 64, 1)])
+def test_window_structure_matches_oracle(mode, lanes, groups):
+    """Windows off / on, lanes per L entry automatic or forced, and the hash
+    and bitmap kernels' term expansion by lane groups or by binary search."""
     lib = load()
     prev = lib.sdl_symbolic_window(mode)
     prev_lanes = lib.sdl_symbolic_window_lanes(lanes)
+    prev_groups = lib.sdl_symbolic_lane_groups(groups)
     try:
         for name, A, B, nA, nB, narrow in cases():
             m = len(A[0]) - 1
@@ -109,6 +114,7 @@ def test_window_structure_matches_oracle(mode, lanes):
     finally:
         lib.sdl_symbolic_window(prev)
         lib.sdl_symbolic_window_lanes(prev_lanes)
+        lib.sdl_symbolic_lane_groups(prev_groups)
 
 
 def test_window_knobs_checked():
@@ -123,6 +129,10 @@ def test_window_knobs_checked():
     for bad in (1, 2, 3, 12, 128, -2):
         assert lib.sdl_symbolic_window_lanes(bad) == -1
     assert lib.sdl_symbolic_window_lanes(-1) == lanes
+    groups = lib.sdl_symbolic_lane_groups(-1)
+    assert groups in (0, 1)
+    assert lib.sdl_symbolic_lane_groups(2) == -1 and lib.sdl_symbolic_lane_groups(-2) == -1
+    assert lib.sdl_symbolic_lane_groups(-1) == groups
 
 
 def test_window_plan_refill_and_shuffled_rows():
