@@ -1037,8 +1037,15 @@ __global__ void __launch_bounds__(G)
             int *__restrict__ tB, int *__restrict__ tK, int *__restrict__ tq,
             int *__restrict__ gscratch, int *__restrict__ err)
 {
+    // In global memory the row's columns are searched through a sample of
+    // every ss-th column held in LDS: the sample's binary search picks a
+    // segment of at most ss columns, searched in global memory (a few
+    // accesses within one or two cache lines instead of ~log2(cnt) across
+    // the row).
+    constexpr int NSAMP = CAP > 0 ? 1 : 2048;
     __shared__ int s_cols[CAP > 0 ? CAP : 1];
     __shared__ int s_off[CAP > 0 ? CAP : 1];
+    __shared__ int samp[NSAMP];
     __shared__ Expand<G> ex;
     __shared__ int wsum[G / 64];
     const int row = rows[blockIdx.x];
@@ -1046,6 +1053,7 @@ __global__ void __launch_bounds__(G)
     const int cnt = (int) (Cp[row + 1] - cbase);
     const int *cols;
     int *off;
+    int ss = 1, ns = 0;
     if (CAP > 0)
     {
         for (int i = threadIdx.x; i < cnt; i += G) s_cols[i] = Ci[cbase + i];
@@ -1056,14 +1064,25 @@ __global__ void __launch_bounds__(G)
     {
         cols = Ci + cbase;
         off = gscratch + cbase;
+        while ((int64_t) ss * NSAMP < cnt) ss <<= 1;
+        ns = (cnt + ss - 1) / ss;
+        for (int i = threadIdx.x; i < ns; i += G) samp[i] = cols[i * ss];
     }
+    // lower bound of col among the row's columns
+    auto find = [&](int col) -> int {
+        if (CAP > 0) return lower_bound_int(cols, cnt, col);
+        // last sample <= col (the segment holding col, if any)
+        const int i = max(0, lower_bound_int(samp, ns, col + 1) - 1);
+        const int b = i * ss, e = min(cnt, b + ss);
+        return b + lower_bound_int(cols + b, e - b, col);
+    };
     for (int i = threadIdx.x; i < cnt; i += G) off_st<CAP>(off, i, 0);
     off_fence<CAP>();
     __syncthreads();
     RowCtx c = row_ctx(row, Lm, Lp, nL);
     for_each_term<G>(ex, c.q0, c.q1, c.koff, Li, Ap, [&](int t, int, int) {
-        const int pos = lower_bound_int(cols, cnt, Ai[t]);
-        if (pos < cnt) atomicAdd(&off[pos], 1);
+        const int pos = find(Ai[t]);
+        if (pos < cnt && cols[pos] == Ai[t]) atomicAdd(&off[pos], 1);
         else atomicOr(err, 1);
     });
     off_fence<CAP>();
@@ -1089,8 +1108,9 @@ __global__ void __launch_bounds__(G)
     }
     const int64_t rend = toff[row + 1];
     for_each_term<G>(ex, c.q0, c.q1, c.koff, Li, Ap, [&](int t, int q, int k) {
-        int pos = lower_bound_int(cols, cnt, Ai[t]);
-        if (pos >= cnt)  // the row's columns miss a term: report, never write
+        const int col = Ai[t];
+        int pos = find(col);
+        if (pos >= cnt || cols[pos] != col)  // the row's columns miss a term: report, never write
         {
             atomicOr(err, 1);
             return;
