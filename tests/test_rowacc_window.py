@@ -62,9 +62,8 @@ def test_window_accumulator_matches_oracle_and_search(name):
     A, B, nA, nB = next((c[1], c[2], c[3], c[4]) for c in cases() if c[0] == name)
     m = len(A[0]) - 1
     xA, xB, d, Cp, Ci, Cx = oracle_btda(A, B, m, nA, nB)
-    on = refill(A, B, nA, nB, xA, xB, d, 1)
-    off = refill(A, B, nA, nB, xA, xB, d, 0)
-    for cp, ci, x in (on, off):
+    outs = [refill(A, B, nA, nB, xA, xB, d, mode) for mode in (0, 1)]
+    for cp, ci, x in outs:
         assert np.array_equal(cp, Cp.astype(np.int32)) and np.array_equal(ci, Ci), name
         assert np.array_equal(x.view(np.int64), Cx.view(np.int64)), name
 
