@@ -74,3 +74,44 @@ def test_rowacc_window_knob():
     assert prev in (0, 1)
     assert lib.sdl_rowacc_window(2) == -1 and lib.sdl_rowacc_window(-2) == -1
     assert lib.sdl_rowacc_window(prev) == prev
+
+
+def test_kron_blocks_through_windows():
+    """C = kron(I_p, A) J through the drop-in BA_kron product (L = A repeated
+    over p blocks, block b's L entries reading J rows offset by b n): A banded
+    (41 entries per row), J's row t the 40 columns from 4 t, so every output
+    row spans ~200 columns with ~8 terms per entry -- span-window symbolic
+    rows and window-accumulator refills in every block.  Windows on and off,
+    structure and values bit-exact against the oracle."""
+    import ctypes as C
+    from oracle import oracle as O
+    from tests import dropin as D
+    lib = load()
+    m = n = 300
+    p = 4
+    Ap, Ai, _ = W.banded_csr(n, 20)
+    Ax = np.linspace(-1.0, 1.0, int(Ap[-1]))
+    rows = n * p
+    ncols = 4 * rows + 40
+    Jp = np.arange(0, rows * 40 + 1, 40, dtype=np.int32)
+    Ji = (4 * np.repeat(np.arange(rows), 40) + np.tile(np.arange(40), rows)).astype(np.int32)
+    Jx = np.linspace(0.5, 2.0, len(Ji))
+    Op, Oi, Ox, terms = O.spgemm(m, Ap, Ai, Ax, p, None, rows, ncols, Jp, Ji, Jx)
+    assert terms > 3 * Op[-1]
+    L = D.lib()
+    for mode in (1, 0):
+        prev = (lib.sdl_symbolic_window(mode), lib.sdl_rowacc_window(mode))
+        try:
+            A = D.sparse(Ap, Ai, Ax, m, n)
+            J = D.sparse(Jp, Ji, Jx, rows, ncols)
+            Cm = L.BA_kron_matrices_alloc(A, p, J)
+            L.BA_kron_matrices_fill_values(A, p, J, Cm)
+            q, i, xx = D.mat_csr(Cm)
+            assert np.array_equal(np.asarray(q), Op) and np.array_equal(np.asarray(i), Oi), mode
+            assert np.array_equal(np.asarray(xx).view(np.int64), Ox.view(np.int64)), mode
+            L.free_matrix(Cm)
+            L.free_matrix(J)
+            L.free_matrix(A)
+        finally:
+            lib.sdl_symbolic_window(prev[0])
+            lib.sdl_rowacc_window(prev[1])
