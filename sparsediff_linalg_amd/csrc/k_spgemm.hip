@@ -315,6 +315,30 @@ template <int G, int S> __device__ __forceinline__ void bitonic_sort(unsigned *a
     }
 }
 
+// The same sort over the first n keys (n a power of two, at most the array).
+template <int G> __device__ __forceinline__ void bitonic_sort_n(unsigned *a, int n)
+{
+    for (int k = 2; k <= n; k <<= 1)
+    {
+        for (int j = k >> 1; j > 0; j >>= 1)
+        {
+            for (int p = (int) threadIdx.x; p < n / 2; p += G)
+            {
+                int i = (p / j) * 2 * j + (p % j);
+                int ixj = i + j;
+                unsigned x = a[i], y = a[ixj];
+                bool up = (i & k) == 0;
+                if ((x > y) == up)
+                {
+                    a[i] = y;
+                    a[ixj] = x;
+                }
+            }
+            __syncthreads();
+        }
+    }
+}
+
 __device__ __forceinline__ int lower_bound_int(const int *a, int n, int x)
 {
     int lo = 0, hi = n;
@@ -540,10 +564,14 @@ __global__ void __launch_bounds__(G)
     __shared__ int tab[S];
     __shared__ Expand<G> ex;
     __shared__ long long red[G / 64];
-    __shared__ int filled;
+    __shared__ int filled, ncomp;
     const int row = rows[blockIdx.x];
     for (int i = threadIdx.x; i < S; i += G) tab[i] = -1;
-    if (threadIdx.x == 0) filled = 0;
+    if (threadIdx.x == 0)
+    {
+        filled = 0;
+        ncomp = 0;
+    }
     __syncthreads();
     RowCtx c = row_ctx(row, Lm, Lp, nL);
     // Tables hold at least twice their row's bound.  In the count pass a row
@@ -575,9 +603,36 @@ __global__ void __launch_bounds__(G)
     }
     else
     {
-        bitonic_sort<G, S>((unsigned *) tab);
+        // the occupied slots (at most half the table) gathered to its front in
+        // any order, padded with empty keys to a power of two n, and only
+        // those n keys sorted (empty = 0xffffffff sorts last)
+        static_assert(S % G == 0, "table slots per thread");
+        constexpr int PER = S / G;
+        const int lane = (int) threadIdx.x & 63;
+        int v[PER];
+#pragma unroll
+        for (int i = 0; i < PER; i++) v[i] = tab[(int) threadIdx.x + i * G];
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < PER; i++)
+        {
+            const bool occ = v[i] != -1;
+            const unsigned long long b = __ballot(occ);
+            int at = 0;
+            if (lane == 0 && b) at = atomicAdd(&ncomp, __popcll(b));
+            at = __shfl(at, 0, 64);
+            if (occ) tab[at + __popcll(b & ((1ull << lane) - 1ull))] = v[i];
+        }
+        __syncthreads();
+        const int got = ncomp;
+        int n = 1;
+        while (n < got) n <<= 1;
+        for (int i = got + (int) threadIdx.x; i < n; i += G) tab[i] = -1;
+        __syncthreads();
+        bitonic_sort_n<G>((unsigned *) tab, n);
         int64_t base = Cp[row];
         int cnt = (int) (Cp[row + 1] - base);
+        cnt = min(cnt, got);
         for (int i = threadIdx.x; i < cnt; i += G) Ci[base + i] = tab[i];
     }
 }
