@@ -103,12 +103,13 @@ int sdl_symbolic_window(int mode);
    (default) to pick them from the product's mean A-row length; -1 only
    queries.  Returns the previous setting, or -1 for an invalid one. */
 int sdl_symbolic_window_lanes(int lanes);
-/* Row-accumulator refills (plans without a pull schedule) of rows whose
-   columns span at most 1024 columns: 1 (default) a wavefront per row with a
-   dense LDS accumulator over the span, 0 the workgroup accumulator with a
-   column search for every row; read when a plan is created, -1 only
-   queries.  Both give the same values bit for bit.  Returns the previous
-   mode, or -1 for an unknown one. */
+/* Row-accumulator refills (plans without a pull schedule) of rows of at most
+   1024 entries: 1 (default) a wavefront per row, with a dense LDS accumulator
+   over the row's span when it is at most 1024 columns, else over its entries
+   with the columns searched in LDS; 0 the workgroup accumulator with a column
+   search for every row; read when a plan is created, -1 only queries.  Both
+   give the same values bit for bit.  Returns the previous mode, or -1 for an
+   unknown one. */
 int sdl_rowacc_window(int mode);
 /* Term expansion of the symbolic phase's LDS column bitmaps: 1 (default)
    groups of lanes per L entry striding over its A row (as the span windows),

@@ -2017,16 +2017,18 @@ __global__ void __launch_bounds__(G)
         for (int i = threadIdx.x; i < cnt; i += G) Cx[cbase + i] = s_val[i];
 }
 
-// Window class of a row-accumulator row (ids[i]): 1 when its column span
-// exceeds 1024 (the LDS accumulator with a column search), else 2 / 3 / 4 for
-// a span of at most 256 / 512 / 1024 columns.  Rows hold at least one entry,
-// in ascending columns.
+// Wavefront class of a row-accumulator row (ids[i]) of at most 1024 entries:
+// 2 / 3 / 4 for a column span of at most 256 / 512 / 1024 (the window
+// accumulator), else 5 / 6 / 7 for at most 256 / 512 / 1024 entries (the
+// accumulator over the row's columns with a search); 1 for a longer row (the
+// workgroup accumulator).  Rows hold at least one entry, in ascending columns.
+constexpr int RW_CLASSES = 8;
 __global__ void k_rowacc_span_class(int n, const int *__restrict__ ids,
                                     const int64_t *__restrict__ Cp, const int *__restrict__ Ci,
                                     unsigned char *__restrict__ cls, int *__restrict__ counts)
 {
-    __shared__ int hist[5];
-    if (threadIdx.x < 5) hist[threadIdx.x] = 0;
+    __shared__ int hist[RW_CLASSES];
+    if (threadIdx.x < RW_CLASSES) hist[threadIdx.x] = 0;
     __syncthreads();
     const int i = (int) (blockIdx.x * blockDim.x + threadIdx.x);
     if (i < n)
@@ -2034,12 +2036,15 @@ __global__ void k_rowacc_span_class(int n, const int *__restrict__ ids,
         const int row = ids[i];
         const int64_t b = Cp[row], e = Cp[row + 1];
         const int span = Ci[e - 1] - Ci[b] + 1;
-        const int c = span <= 256 ? 2 : span <= 512 ? 3 : span <= 1024 ? 4 : 1;
+        const int64_t cnt = e - b;
+        const int c = span <= 256 ? 2 : span <= 512 ? 3 : span <= 1024 ? 4
+                    : cnt <= 256  ? 5 : cnt <= 512  ? 6 : cnt <= 1024 ? 7 : 1;
         cls[i] = (unsigned char) c;
         atomicAdd(&hist[c], 1);
     }
     __syncthreads();
-    if (threadIdx.x < 5 && hist[threadIdx.x]) atomicAdd(&counts[threadIdx.x], hist[threadIdx.x]);
+    if (threadIdx.x < RW_CLASSES && hist[threadIdx.x])
+        atomicAdd(&counts[threadIdx.x], hist[threadIdx.x]);
 }
 
 // Ordered row accumulator, one wavefront per row, for rows whose columns span
@@ -2054,7 +2059,10 @@ __global__ void k_rowacc_span_class(int n, const int *__restrict__ ids,
 // previous row's updates (index 0 stands in for a slot without a term:
 // unconditional loads measured faster than masked ones unless whole chunks
 // are empty, so U covers the mean A row).
-template <int WS, int U>
+//   SEARCH: rows of at most WS entries whose span exceeds the windows: the
+//   accumulator holds the row's entries in order, the row's columns staged in
+//   LDS, and a term's slot is found by binary search among them.
+template <int WS, int U, bool SEARCH = false>
 __global__ void __launch_bounds__(256)
     k_num_rowacc_win(const int *__restrict__ rows, int nrows, int Lm, const int *__restrict__ Lp,
                      const int *__restrict__ Li, const int *__restrict__ Lperm, int nL,
@@ -2064,6 +2072,7 @@ __global__ void __launch_bounds__(256)
                      const double *__restrict__ Ax, double *__restrict__ Cx)
 {
     __shared__ double acc[4][WS];
+    __shared__ int scol[SEARCH ? 4 : 1][SEARCH ? WS : 1];
     const int w = (int) threadIdx.x >> 6, lane = (int) threadIdx.x & 63;
     const int ri = (int) blockIdx.x * 4 + w;
     if (ri >= nrows) return;  // whole wavefronts; no workgroup barrier follows
@@ -2072,6 +2081,16 @@ __global__ void __launch_bounds__(256)
     const int64_t cb = Cp[row];
     const int cnt = (int) (Cp[row + 1] - cb);
     const int lo = Ci[cb];
+    const int ncol = min(cnt, WS);
+    int *sc = scol[SEARCH ? w : 0];
+    if (SEARCH)
+        for (int i = lane; i < ncol; i += 64) sc[i] = Ci[cb + i];
+    // slot of a column: its offset in the window, or its position in the row
+    auto slot = [&](int col) -> unsigned {
+        if (!SEARCH) return (unsigned) (col - lo);
+        const int p = lower_bound_int(sc, ncol, col);
+        return p < ncol && sc[p] == col ? (unsigned) p : (unsigned) WS;
+    };
     for (int i = lane; i < WS; i += 64) a[i] = 0.0;
     const RowCtx c = row_ctx(row, Lm, Lp, nL);
     for (int qb = c.q0; qb < c.q1; qb += 64)
@@ -2126,7 +2145,7 @@ __global__ void __launch_bounds__(256)
 #pragma unroll
             for (int u = 0; u < U; u++)
             {
-                o[u] = a0 + lane + 64 * u < b0 ? (unsigned) (colc[u] - lo) : (unsigned) WS;
+                o[u] = a0 + lane + 64 * u < b0 ? slot(colc[u]) : (unsigned) WS;
                 old[u] = a[o[u] < (unsigned) WS ? o[u] : 0u];
             }
 #pragma unroll
@@ -2134,7 +2153,7 @@ __global__ void __launch_bounds__(256)
                 if (o[u] < (unsigned) WS) a[o[u]] = old[u] + v * xc[u];
             for (int t = a0 + lane + 64 * U; t < b0; t += 64)  // A rows longer than 64 U
             {
-                const unsigned o = (unsigned) (Ai[t] - lo);
+                const unsigned o = slot(Ai[t]);
                 if (o < (unsigned) WS) a[o] = a[o] + v * Ax[t];
             }
             // (no code motion of memory operations across consecutive k)
@@ -2149,7 +2168,7 @@ __global__ void __launch_bounds__(256)
     }
     for (int i = lane; i < cnt; i += 64)
     {
-        const unsigned o = (unsigned) (Ci[cb + i] - lo);
+        const unsigned o = SEARCH ? (unsigned) i : (unsigned) (Ci[cb + i] - lo);
         Cx[cb + i] = o < (unsigned) WS ? a[o] : 0.0;
     }
 }
@@ -3414,25 +3433,26 @@ void product_plan(ProductPlan &P, const LView &L, const AView &A, SymMode mode,
         d2d(P.rbins.p + small + mid, BR.rows.p + BR.off[3], (size_t) large * sizeof(int));
         // the LDS-accumulator rows by column span: rows within 256 / 512 /
         // 1024 columns take the window accumulators (after the others)
-        for (int c = 0; c < 5; c++) P.rwin_off[c] = c == 0 ? 0 : small;
+        for (int c = 0; c < RW_CLASSES; c++) P.rwin_off[c] = c == 0 ? 0 : small;
         if (g_rowacc_window && small > 0)
         {
             DBuf<unsigned char> cls(small);
-            DBuf<int> counts(5), tmp(small);
+            DBuf<int> counts(RW_CLASSES), tmp(small);
             dzero(counts.p, counts.bytes());
             const unsigned g = (unsigned) ((small + 255) / 256);
             hipLaunchKernelGGL(k_rowacc_span_class, dim3(g), dim3(256), 0, stream(), small,
                                P.rbins.p, P.Cp.p, P.Ci.p, cls.p, counts.p);
             SDL_HIP(hipGetLastError());
-            int cnt[5];
+            int cnt[RW_CLASSES];
             d2h(cnt, counts.p, sizeof(cnt));
-            int cur[5] = {0, 0, 0, 0, 0};
-            for (int c = 1; c < 5; c++)
+            int cur[RW_CLASSES] = {0};
+            for (int c = 1; c < RW_CLASSES; c++)
             {
                 cur[c] = P.rwin_off[c - 1];
                 P.rwin_off[c] = P.rwin_off[c - 1] + cnt[c];
             }
-            SDL_REQUIRE(P.rwin_off[4] == small, "product plan: window classes do not cover the rows");
+            SDL_REQUIRE(P.rwin_off[RW_CLASSES - 1] == small,
+                        "product plan: window classes do not cover the rows");
             h2d(counts.p, cur, sizeof(cur));
             hipLaunchKernelGGL(k_scatter_rows, dim3(g), dim3(256), 0, stream(), (int64_t) small,
                                cls.p, counts.p, tmp.p, (const int *) P.rbins.p);
@@ -3491,17 +3511,20 @@ void product_numeric(const ProductPlan &P, const double *Lval, const double *d,
             hipLaunchKernelGGL((k_num_rowacc<256, 1024>), dim3(nsearch), dim3(256), 0, s,
                                P.rbins.p, P.L.Lm, P.L.Lp, P.L.Li, P.L.Lperm, nL, P.A.Ap, P.A.Ai,
                                P.Cp.p, P.Ci.p, Lval, d, Ax, Cx);
-        for (int c = 2; c < 5 && n0; c++)
+        for (int c = 2; c < RW_CLASSES && n0; c++)
         {
             const int nr = P.rwin_off[c] - P.rwin_off[c - 1];
             if (nr == 0) continue;
             const int *wr = P.rbins.p + P.rwin_off[c - 1];
             // A-row chunks of 64 loaded ahead: enough for the mean A row
             const int u = P.mean_arow <= 64 ? 1 : P.mean_arow <= 128 ? 2 : P.mean_arow <= 192 ? 3 : 4;
-#define SDL_RW(WS)                                                                         \
-    (u == 1 ? k_num_rowacc_win<WS, 1> : u == 2 ? k_num_rowacc_win<WS, 2>                   \
-                                      : u == 3 ? k_num_rowacc_win<WS, 3> : k_num_rowacc_win<WS, 4>)
-            auto fn = c == 2 ? SDL_RW(256) : c == 3 ? SDL_RW(512) : SDL_RW(1024);
+#define SDL_RW(WS, SR)                                                                      \
+    (u == 1 ? k_num_rowacc_win<WS, 1, SR> : u == 2 ? k_num_rowacc_win<WS, 2, SR>            \
+                                          : u == 3 ? k_num_rowacc_win<WS, 3, SR>            \
+                                                   : k_num_rowacc_win<WS, 4, SR>)
+            auto fn = c == 2 ? SDL_RW(256, false) : c == 3 ? SDL_RW(512, false)
+                    : c == 4 ? SDL_RW(1024, false) : c == 5 ? SDL_RW(256, true)
+                    : c == 6 ? SDL_RW(512, true) : SDL_RW(1024, true);
 #undef SDL_RW
             hipLaunchKernelGGL(fn, dim3((unsigned) ((nr + 3) / 4)), dim3(256), 0, s, wr, nr,
                                P.L.Lm, P.L.Lp, P.L.Li, P.L.Lperm, nL, P.A.Ap, P.A.Ai, P.Cp.p,

@@ -144,9 +144,11 @@ struct ProductPlan
     // column bitmap fits LDS), [2, 3) longer.
     DBuf<int> rbins;                 // row ids grouped by class
     int rbin_off[8] = {0};           // class c is rbins[rbin_off[c] .. rbin_off[c+1])
-    // bin 0 split by column span: [0, rwin_off[1]) the LDS accumulator with a
-    // column search, then the window accumulators of 256 / 512 / 1024 columns
-    int rwin_off[5] = {0};
+    // bin 0 split for the wavefront accumulators: [0, rwin_off[1]) the
+    // workgroup accumulator, then classes 2..4 the windows of 256 / 512 /
+    // 1024 columns, classes 5..7 the column searches over 256 / 512 / 1024
+    // entries (class c is rbins[rwin_off[c - 1] .. rwin_off[c]))
+    int rwin_off[8] = {0};
     double mean_arow = 0;            // terms per L entry (the mean A-row length)
     bool rank = false;               // bin 1 by rank; its spills in spill[0 .. *nspill)
     int rank_grid = 0;
