@@ -90,10 +90,13 @@ int sdl_btda_plan_structure(const sdl_btda_plan *plan, int *hp, int *hi);
    the first count pass's hash / LDS-bitmap / global-bitmap rows (the count
    pass sizes each row from min(terms, column span), or from its prediction). */
 int sdl_btda_plan_stats(const sdl_btda_plan *plan, long long *out11);
-/* Rows counted (first count pass) and filled in span windows (2 values):
-   rows whose columns all lie in a span of at most 8192 columns, marked in an
-   LDS array over that span instead of a hash table or a column bitmap. */
-int sdl_btda_plan_window_rows(const sdl_btda_plan *plan, long long *out2);
+/* Window rows of a plan (4 values): rows counted (first count pass) and
+   filled in span windows -- rows whose columns all lie in a span of at most
+   8192 columns, marked in an LDS array over that span instead of a hash
+   table or a column bitmap; then, for row-accumulator plans, rows of at most
+   1024 entries refilled one wavefront per row, and rows of more entries
+   (without recorded positions) refilled in 2048- / 4096-column windows. */
+int sdl_btda_plan_window_rows(const sdl_btda_plan *plan, long long *out4);
 /* Span windows in the symbolic phase of plans created afterwards: 1
    (default) on, 0 off (hash tables and column bitmaps only); -1 only
    queries.  Both give the same structure.  Returns the previous mode, or -1
@@ -103,13 +106,15 @@ int sdl_symbolic_window(int mode);
    (default) to pick them from the product's mean A-row length; -1 only
    queries.  Returns the previous setting, or -1 for an invalid one. */
 int sdl_symbolic_window_lanes(int lanes);
-/* Row-accumulator refills (plans without a pull schedule) of rows of at most
-   1024 entries: 1 (default) a wavefront per row, with a dense LDS accumulator
+/* Row-accumulator refills (plans without a pull schedule): 1 (default) rows
+   of at most 1024 entries one wavefront per row, with a dense LDS accumulator
    over the row's span when it is at most 1024 columns, else over its entries
-   with the columns searched in LDS; 0 the workgroup accumulator with a column
-   search for every row; read when a plan is created, -1 only queries.  Both
-   give the same values bit for bit.  Returns the previous mode, or -1 for an
-   unknown one. */
+   with the columns searched in LDS; and longer rows spanning at most 4096
+   columns, when the plan records no term positions, one wavefront per row
+   over a 2048- or 4096-column window.  0: the workgroup accumulator with a
+   column search and the by-rank kernel.  Read when a plan is created, -1 only
+   queries.  Both give the same values bit for bit.  Returns the previous
+   mode, or -1 for an unknown one. */
 int sdl_rowacc_window(int mode);
 /* Term expansion of the symbolic phase's LDS column bitmaps: 1 (default)
    groups of lanes per L entry striding over its A row (as the span windows),

@@ -2047,6 +2047,29 @@ __global__ void k_rowacc_span_class(int n, const int *__restrict__ ids,
         atomicAdd(&counts[threadIdx.x], hist[threadIdx.x]);
 }
 
+// Window class of a rank-bin row (ids[i]): 2 / 3 for a column span of at
+// most 2048 / 4096, else 1.
+__global__ void k_rowacc_wide_class(int n, const int *__restrict__ ids,
+                                    const int64_t *__restrict__ Cp, const int *__restrict__ Ci,
+                                    unsigned char *__restrict__ cls, int *__restrict__ counts)
+{
+    __shared__ int hist[4];
+    if (threadIdx.x < 4) hist[threadIdx.x] = 0;
+    __syncthreads();
+    const int i = (int) (blockIdx.x * blockDim.x + threadIdx.x);
+    if (i < n)
+    {
+        const int row = ids[i];
+        const int64_t b = Cp[row], e = Cp[row + 1];
+        const int span = Ci[e - 1] - Ci[b] + 1;
+        const int c = span <= 2048 ? 2 : span <= 4096 ? 3 : 1;
+        cls[i] = (unsigned char) c;
+        atomicAdd(&hist[c], 1);
+    }
+    __syncthreads();
+    if (threadIdx.x < 4 && hist[threadIdx.x]) atomicAdd(&counts[threadIdx.x], hist[threadIdx.x]);
+}
+
 // Ordered row accumulator, one wavefront per row, for rows whose columns span
 // at most WS: a dense LDS accumulator over [lo, lo + WS) (lo = the row's first
 // column) indexed by column - lo, so a term needs no column search.  k walks
@@ -2062,8 +2085,10 @@ __global__ void k_rowacc_span_class(int n, const int *__restrict__ ids,
 //   SEARCH: rows of at most WS entries whose span exceeds the windows: the
 //   accumulator holds the row's entries in order, the row's columns staged in
 //   LDS, and a term's slot is found by binary search among them.
-template <int WS, int U, bool SEARCH = false>
-__global__ void __launch_bounds__(256)
+//   NWV: wavefronts (rows) per workgroup; the 2048- and 4096-column windows
+//   of rank-bin rows run one per workgroup.
+template <int WS, int U, bool SEARCH = false, int NWV = 4>
+__global__ void __launch_bounds__(64 * NWV)
     k_num_rowacc_win(const int *__restrict__ rows, int nrows, int Lm, const int *__restrict__ Lp,
                      const int *__restrict__ Li, const int *__restrict__ Lperm, int nL,
                      const int *__restrict__ Ap, const int *__restrict__ Ai,
@@ -2071,10 +2096,10 @@ __global__ void __launch_bounds__(256)
                      const double *__restrict__ Lval, const double *__restrict__ d,
                      const double *__restrict__ Ax, double *__restrict__ Cx)
 {
-    __shared__ double acc[4][WS];
-    __shared__ int scol[SEARCH ? 4 : 1][SEARCH ? WS : 1];
+    __shared__ double acc[NWV][WS];
+    __shared__ int scol[SEARCH ? NWV : 1][SEARCH ? WS : 1];
     const int w = (int) threadIdx.x >> 6, lane = (int) threadIdx.x & 63;
-    const int ri = (int) blockIdx.x * 4 + w;
+    const int ri = (int) blockIdx.x * NWV + w;
     if (ri >= nrows) return;  // whole wavefronts; no workgroup barrier follows
     double *a = acc[w];
     const int row = rows[ri];
@@ -3481,6 +3506,36 @@ void product_plan(ProductPlan &P, const LView &L, const AView &A, SymMode mode,
             P.acc.alloc((size_t) P.dense_grid * std::max<int64_t>(P.n, 1));
         }
         if (P.rank && !P.rank_refill) record_rank_positions(P, L, A, rterms.p, rowcnt.p, mid);
+        // rank-bin rows without recorded positions (many terms per entry)
+        // whose columns span at most 4096: the window accumulator, one
+        // wavefront per workgroup, first in the bin
+        P.rwide_off[0] = P.rwide_off[1] = P.rwide_off[2] = 0;
+        if (g_rowacc_window && mid > 0 && !P.rankpos && !P.rank_refill)
+        {
+            int *mrows = P.rbins.p + P.rbin_off[1];
+            DBuf<unsigned char> cls(mid);
+            DBuf<int> counts(4), tmp(mid);
+            dzero(counts.p, counts.bytes());
+            const unsigned g = (unsigned) ((mid + 255) / 256);
+            hipLaunchKernelGGL(k_rowacc_wide_class, dim3(g), dim3(256), 0, stream(), mid, mrows,
+                               P.Cp.p, P.Ci.p, cls.p, counts.p);
+            SDL_HIP(hipGetLastError());
+            int cnt[4];
+            d2h(cnt, counts.p, sizeof(cnt));
+            if (cnt[2] + cnt[3] > 0)
+            {
+                // order: span <= 2048 (class 2), span <= 4096 (class 3), then
+                // the rest (class 1); cur[c] is class c's first slot
+                int cur[4] = {0, cnt[2] + cnt[3], 0, cnt[2]};
+                P.rwide_off[1] = cnt[2];
+                P.rwide_off[2] = cnt[2] + cnt[3];
+                h2d(counts.p, cur, sizeof(cur));
+                hipLaunchKernelGGL(k_scatter_rows, dim3(g), dim3(256), 0, stream(), (int64_t) mid,
+                                   cls.p, counts.p, tmp.p, (const int *) mrows);
+                SDL_HIP(hipGetLastError());
+                d2d(mrows, tmp.p, (size_t) mid * sizeof(int));
+            }
+        }
         sync();
     }
 }
@@ -3530,10 +3585,30 @@ void product_numeric(const ProductPlan &P, const double *Lval, const double *d,
                                P.L.Lm, P.L.Lp, P.L.Li, P.L.Lperm, nL, P.A.Ap, P.A.Ai, P.Cp.p,
                                P.Ci.p, Lval, d, Ax, Cx);
         }
-        if (P.rank)
+        // rank-bin rows in the wide windows (first in the bin)
+        const int nwide = P.rwide_off[2];
+        for (int c = 1; c <= 2 && nwide; c++)
+        {
+            const int nr = P.rwide_off[c] - P.rwide_off[c - 1];
+            if (nr == 0) continue;
+            const int *wr = long_rows + P.rwide_off[c - 1];
+            const int u = P.mean_arow <= 64 ? 1 : P.mean_arow <= 128 ? 2 : P.mean_arow <= 192 ? 3 : 4;
+#define SDL_RW1(WS)                                                                          \
+    (u == 1 ? k_num_rowacc_win<WS, 1, false, 1> : u == 2 ? k_num_rowacc_win<WS, 2, false, 1> \
+                                                : u == 3 ? k_num_rowacc_win<WS, 3, false, 1> \
+                                                         : k_num_rowacc_win<WS, 4, false, 1>)
+            auto fn = c == 1 ? SDL_RW1(2048) : SDL_RW1(4096);
+#undef SDL_RW1
+            hipLaunchKernelGGL(fn, dim3((unsigned) nr), dim3(64), 0, s, wr, nr, P.L.Lm, P.L.Lp,
+                               P.L.Li, P.L.Lperm, nL, P.A.Ap, P.A.Ai, P.Cp.p, P.Ci.p, Lval, d, Ax,
+                               Cx);
+        }
+        long_rows += nwide;
+        n1 -= nwide;
+        if (P.rank && P.rbin_off[2] - P.rbin_off[1] > nwide)
         {
             // rows of the rank bin; its spills and the longer rows follow
-            const int nmid = P.rbin_off[2] - P.rbin_off[1];
+            const int nmid = P.rbin_off[2] - P.rbin_off[1] - nwide;
             const int nwords = (int) ((P.n + 31) / 32);
             const size_t lds = rank_lds_bytes(nwords);
             static bool attr_set = false;

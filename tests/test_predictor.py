@@ -237,7 +237,7 @@ def test_span_bound_sizes_the_count_pass():
     p, i, _ = W.strided_band_csr(1 << 12, n, 129, 4)
     pl = Plan((p, i), None, n, n)
     st = plan_stats(pl)
-    win = (C.c_longlong * 2)()
+    win = (C.c_longlong * 4)()
     check(load().sdl_btda_plan_window_rows(pl.h, win), "window rows")
     cp, ci = pl.structure()
     nonempty = int(np.count_nonzero(np.diff(cp)))

@@ -10,6 +10,8 @@ rows beyond the 1024-entry bin), and rows drawn from a few wide patterns
 (many terms per entry, spans too wide for a window)."""
 import ctypes as C
 
+import ctypes as C
+
 import numpy as np
 import pytest
 
@@ -37,9 +39,13 @@ def cases():
     yield "window-ladder", A, B, nA, nB
     p, i, n = patterns(3)
     yield "patterns", (p, i), None, n, n
+    p, i, n = wide_blocks()
+    yield "wide-blocks", (p, i), None, n, n
+    p, i, n = wide_blocks(5, nblocks=40, ncols=2048)
+    yield "wide-blocks-2048", (p, i), None, n, n
 
 
-def refill(A, B, nA, nB, xA, xB, d, mode):
+def refill(A, B, nA, nB, xA, xB, d, mode, wide=None):
     lib = load()
     prev = lib.sdl_rowacc_window(mode)
     try:
@@ -47,6 +53,12 @@ def refill(A, B, nA, nB, xA, xB, d, mode):
     finally:
         lib.sdl_rowacc_window(prev)
     assert not lib.sdl_btda_plan_is_pull(pl.h)
+    win = (C.c_longlong * 4)()
+    check(lib.sdl_btda_plan_window_rows(pl.h, win), "window rows")
+    if mode == 0:
+        assert win[2] == 0 and win[3] == 0
+    elif wide is not None:
+        assert (win[3] > 0) == wide, list(win)
     dA, dB, dd = (DeviceArray.from_numpy(v) for v in (xA, xB, d))
     out = DeviceArray(max(pl.nnz(), 1), np.float64)
     check(lib.sdl_btda_refill(pl.h, dA.ptr, dB.ptr, dd.ptr, out.ptr, None), "refill")
@@ -57,12 +69,23 @@ def refill(A, B, nA, nB, xA, xB, d, mode):
     return cp, ci, x
 
 
-@pytest.mark.parametrize("name", ["band32", "window-ladder", "patterns"])
+def wide_blocks(seed=3, nblocks=48, ncols=4096):
+    """Stacked dense blocks as CSR: output rows of 1025..4096 entries spanning
+    at most 4096 columns, with hundreds of terms per entry (no recorded
+    positions), so the rank bin's rows take the 2048- / 4096-column windows."""
+    wb = W.stacked_pd_blocks(seed, nblocks=nblocks, lo=64, hi=400, ncols=ncols)
+    p, i, _ = W.blocks_as_csr(wb)
+    return p, i, wb["n"]
+
+
+@pytest.mark.parametrize("name", ["band32", "window-ladder", "patterns", "wide-blocks",
+                                  "wide-blocks-2048"])
 def test_window_accumulator_matches_oracle_and_search(name):
     A, B, nA, nB = next((c[1], c[2], c[3], c[4]) for c in cases() if c[0] == name)
     m = len(A[0]) - 1
     xA, xB, d, Cp, Ci, Cx = oracle_btda(A, B, m, nA, nB)
-    outs = [refill(A, B, nA, nB, xA, xB, d, mode) for mode in (0, 1)]
+    outs = [refill(A, B, nA, nB, xA, xB, d, mode, wide=name.startswith("wide-blocks") or None)
+            for mode in (0, 1)]
     for cp, ci, x in outs:
         assert np.array_equal(cp, Cp.astype(np.int32)) and np.array_equal(ci, Ci), name
         assert np.array_equal(x.view(np.int64), Cx.view(np.int64)), name
