@@ -95,7 +95,8 @@ int sdl_btda_plan_stats(const sdl_btda_plan *plan, long long *out11);
    8192 columns, marked in an LDS array over that span instead of a hash
    table or a column bitmap; then, for row-accumulator plans, rows of at most
    1024 entries refilled one wavefront per row, and rows of more entries
-   (without recorded positions) refilled in 2048- / 4096-column windows. */
+   (without recorded positions) refilled in 2048- / 4096- / 8192-column
+   windows. */
 int sdl_btda_plan_window_rows(const sdl_btda_plan *plan, long long *out4);
 /* Span windows in the symbolic phase of plans created afterwards: 1
    (default) on, 0 off (hash tables and column bitmaps only); -1 only
@@ -109,9 +110,9 @@ int sdl_symbolic_window_lanes(int lanes);
 /* Row-accumulator refills (plans without a pull schedule): 1 (default) rows
    of at most 1024 entries one wavefront per row, with a dense LDS accumulator
    over the row's span when it is at most 1024 columns, else over its entries
-   with the columns searched in LDS; and longer rows spanning at most 4096
+   with the columns searched in LDS; and longer rows spanning at most 8192
    columns, when the plan records no term positions, one wavefront per row
-   over a 2048- or 4096-column window.  0: the workgroup accumulator with a
+   over a 2048-, 4096- or 8192-column window.  0: the workgroup accumulator with a
    column search and the by-rank kernel.  Read when a plan is created, -1 only
    queries.  Both give the same values bit for bit.  Returns the previous
    mode, or -1 for an unknown one. */

@@ -144,7 +144,7 @@ int sdl_btda_plan_window_rows(const sdl_btda_plan *plan, long long *o)
     o[0] = P.stats.count_window_rows;
     o[1] = P.stats.window_rows;
     o[2] = P.pull ? 0 : P.rwin_off[7] - P.rwin_off[1];
-    o[3] = P.pull ? 0 : P.rwide_off[2];
+    o[3] = P.pull ? 0 : P.rwide_off[3];
     return 0;
 }
 
@@ -372,7 +372,7 @@ int sdl_btda_plan_schedule(const sdl_btda_plan *plan, long long *out5)
     if (!plan || !out5) return 1;
     const sdl::ProductPlan &P = plan->P;
     out5[0] = P.pull ? 1 : 0;
-    out5[1] = P.pull ? 0 : P.rbin_off[2] - P.rbin_off[1] - P.rwide_off[2];
+    out5[1] = P.pull ? 0 : P.rbin_off[2] - P.rbin_off[1] - P.rwide_off[3];
     out5[2] = P.rankpos ? 1 : 0;
     out5[3] = P.rankpos ? P.nmulti : 0;
     int ns = 0;

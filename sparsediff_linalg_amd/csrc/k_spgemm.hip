@@ -2047,14 +2047,14 @@ __global__ void k_rowacc_span_class(int n, const int *__restrict__ ids,
         atomicAdd(&counts[threadIdx.x], hist[threadIdx.x]);
 }
 
-// Window class of a rank-bin row (ids[i]): 2 / 3 for a column span of at
-// most 2048 / 4096, else 1.
+// Window class of a rank-bin row (ids[i]): 2 / 3 / 4 for a column span of
+// at most 2048 / 4096 / 8192, else 1.
 __global__ void k_rowacc_wide_class(int n, const int *__restrict__ ids,
                                     const int64_t *__restrict__ Cp, const int *__restrict__ Ci,
                                     unsigned char *__restrict__ cls, int *__restrict__ counts)
 {
-    __shared__ int hist[4];
-    if (threadIdx.x < 4) hist[threadIdx.x] = 0;
+    __shared__ int hist[5];
+    if (threadIdx.x < 5) hist[threadIdx.x] = 0;
     __syncthreads();
     const int i = (int) (blockIdx.x * blockDim.x + threadIdx.x);
     if (i < n)
@@ -2062,12 +2062,12 @@ __global__ void k_rowacc_wide_class(int n, const int *__restrict__ ids,
         const int row = ids[i];
         const int64_t b = Cp[row], e = Cp[row + 1];
         const int span = Ci[e - 1] - Ci[b] + 1;
-        const int c = span <= 2048 ? 2 : span <= 4096 ? 3 : 1;
+        const int c = span <= 2048 ? 2 : span <= 4096 ? 3 : span <= 8192 ? 4 : 1;
         cls[i] = (unsigned char) c;
         atomicAdd(&hist[c], 1);
     }
     __syncthreads();
-    if (threadIdx.x < 4 && hist[threadIdx.x]) atomicAdd(&counts[threadIdx.x], hist[threadIdx.x]);
+    if (threadIdx.x < 5 && hist[threadIdx.x]) atomicAdd(&counts[threadIdx.x], hist[threadIdx.x]);
 }
 
 // Ordered row accumulator, one wavefront per row, for rows whose columns span
@@ -2096,12 +2096,16 @@ __global__ void __launch_bounds__(64 * NWV)
                      const double *__restrict__ Lval, const double *__restrict__ d,
                      const double *__restrict__ Ax, double *__restrict__ Cx)
 {
-    __shared__ double acc[NWV][WS];
+    // windows past 32 KB (8192 columns) live in dynamic LDS (64 KB, past
+    // the static limit)
+    constexpr bool DYN = (size_t) NWV * WS * sizeof(double) > 32768;
+    __shared__ double acc[DYN ? 1 : NWV][DYN ? 1 : WS];
+    extern __shared__ double dacc[];
     __shared__ int scol[SEARCH ? NWV : 1][SEARCH ? WS : 1];
     const int w = (int) threadIdx.x >> 6, lane = (int) threadIdx.x & 63;
     const int ri = (int) blockIdx.x * NWV + w;
     if (ri >= nrows) return;  // whole wavefronts; no workgroup barrier follows
-    double *a = acc[w];
+    double *a = DYN ? dacc + (size_t) w * WS : acc[DYN ? 0 : w];
     const int row = rows[ri];
     const int64_t cb = Cp[row];
     const int cnt = (int) (Cp[row + 1] - cb);
@@ -3509,26 +3513,28 @@ void product_plan(ProductPlan &P, const LView &L, const AView &A, SymMode mode,
         // rank-bin rows without recorded positions (many terms per entry)
         // whose columns span at most 4096: the window accumulator, one
         // wavefront per workgroup, first in the bin
-        P.rwide_off[0] = P.rwide_off[1] = P.rwide_off[2] = 0;
+        for (int c = 0; c < 4; c++) P.rwide_off[c] = 0;
         if (g_rowacc_window && mid > 0 && !P.rankpos && !P.rank_refill)
         {
             int *mrows = P.rbins.p + P.rbin_off[1];
             DBuf<unsigned char> cls(mid);
-            DBuf<int> counts(4), tmp(mid);
+            DBuf<int> counts(5), tmp(mid);
             dzero(counts.p, counts.bytes());
             const unsigned g = (unsigned) ((mid + 255) / 256);
             hipLaunchKernelGGL(k_rowacc_wide_class, dim3(g), dim3(256), 0, stream(), mid, mrows,
                                P.Cp.p, P.Ci.p, cls.p, counts.p);
             SDL_HIP(hipGetLastError());
-            int cnt[4];
+            int cnt[5];
             d2h(cnt, counts.p, sizeof(cnt));
-            if (cnt[2] + cnt[3] > 0)
+            if (cnt[2] + cnt[3] + cnt[4] > 0)
             {
-                // order: span <= 2048 (class 2), span <= 4096 (class 3), then
-                // the rest (class 1); cur[c] is class c's first slot
-                int cur[4] = {0, cnt[2] + cnt[3], 0, cnt[2]};
+                // order: span <= 2048 (class 2), <= 4096 (class 3), <= 8192
+                // (class 4), then the rest (class 1); cur[c] is class c's
+                // first slot
+                int cur[5] = {0, cnt[2] + cnt[3] + cnt[4], 0, cnt[2], cnt[2] + cnt[3]};
                 P.rwide_off[1] = cnt[2];
                 P.rwide_off[2] = cnt[2] + cnt[3];
+                P.rwide_off[3] = cnt[2] + cnt[3] + cnt[4];
                 h2d(counts.p, cur, sizeof(cur));
                 hipLaunchKernelGGL(k_scatter_rows, dim3(g), dim3(256), 0, stream(), (int64_t) mid,
                                    cls.p, counts.p, tmp.p, (const int *) mrows);
@@ -3586,8 +3592,19 @@ void product_numeric(const ProductPlan &P, const double *Lval, const double *d,
                                P.Ci.p, Lval, d, Ax, Cx);
         }
         // rank-bin rows in the wide windows (first in the bin)
-        const int nwide = P.rwide_off[2];
-        for (int c = 1; c <= 2 && nwide; c++)
+        const int nwide = P.rwide_off[3];
+        static bool win8k_attr = false;
+        if (!win8k_attr && P.rwide_off[3] > P.rwide_off[2])
+        {
+            for (auto f : {(const void *) k_num_rowacc_win<8192, 1, false, 1>,
+                           (const void *) k_num_rowacc_win<8192, 2, false, 1>,
+                           (const void *) k_num_rowacc_win<8192, 3, false, 1>,
+                           (const void *) k_num_rowacc_win<8192, 4, false, 1>})
+                SDL_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                            8192 * (int) sizeof(double)));
+            win8k_attr = true;
+        }
+        for (int c = 1; c <= 3 && nwide; c++)
         {
             const int nr = P.rwide_off[c] - P.rwide_off[c - 1];
             if (nr == 0) continue;
@@ -3597,9 +3614,10 @@ void product_numeric(const ProductPlan &P, const double *Lval, const double *d,
     (u == 1 ? k_num_rowacc_win<WS, 1, false, 1> : u == 2 ? k_num_rowacc_win<WS, 2, false, 1> \
                                                 : u == 3 ? k_num_rowacc_win<WS, 3, false, 1> \
                                                          : k_num_rowacc_win<WS, 4, false, 1>)
-            auto fn = c == 1 ? SDL_RW1(2048) : SDL_RW1(4096);
+            auto fn = c == 1 ? SDL_RW1(2048) : c == 2 ? SDL_RW1(4096) : SDL_RW1(8192);
 #undef SDL_RW1
-            hipLaunchKernelGGL(fn, dim3((unsigned) nr), dim3(64), 0, s, wr, nr, P.L.Lm, P.L.Lp,
+            const size_t dyn = c == 3 ? 8192 * sizeof(double) : 0;
+            hipLaunchKernelGGL(fn, dim3((unsigned) nr), dim3(64), dyn, s, wr, nr, P.L.Lm, P.L.Lp,
                                P.L.Li, P.L.Lperm, nL, P.A.Ap, P.A.Ai, P.Cp.p, P.Ci.p, Lval, d, Ax,
                                Cx);
         }

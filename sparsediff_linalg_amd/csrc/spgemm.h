@@ -149,9 +149,9 @@ struct ProductPlan
     // 1024 columns, classes 5..7 the column searches over 256 / 512 / 1024
     // entries (class c is rbins[rwin_off[c - 1] .. rwin_off[c]))
     int rwin_off[8] = {0};
-    // rank-bin rows (no recorded positions) in the 2048- / 4096-column window
-    // accumulators: the bin's first rwide_off[2] rows, the 2048 ones first
-    int rwide_off[3] = {0};
+    // rank-bin rows (no recorded positions) in the 2048- / 4096- / 8192-column
+    // window accumulators: the bin's first rwide_off[3] rows, by window size
+    int rwide_off[4] = {0};
     double mean_arow = 0;            // terms per L entry (the mean A-row length)
     bool rank = false;               // bin 1 by rank; its spills in spill[0 .. *nspill)
     int rank_grid = 0;

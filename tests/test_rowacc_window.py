@@ -43,6 +43,11 @@ def cases():
     yield "wide-blocks", (p, i), None, n, n
     p, i, n = wide_blocks(5, nblocks=40, ncols=2048)
     yield "wide-blocks-2048", (p, i), None, n, n
+    # output rows of 1500..9000 entries spanning as many columns (the 2048,
+    # 4096 and 8192 windows, and beyond them the dense accumulator: n = 2^20
+    # leaves no by-rank kernel), ~15 terms per entry
+    A, B, nA, nB = window_ladder(7, spans=(1500, 3000, 5000, 7000, 8192, 9000), blocks=3)
+    yield "wide-ladder", A, B, nA, nB
 
 
 def refill(A, B, nA, nB, xA, xB, d, mode, wide=None):
@@ -79,12 +84,12 @@ def wide_blocks(seed=3, nblocks=48, ncols=4096):
 
 
 @pytest.mark.parametrize("name", ["band32", "window-ladder", "patterns", "wide-blocks",
-                                  "wide-blocks-2048"])
+                                  "wide-blocks-2048", "wide-ladder"])
 def test_window_accumulator_matches_oracle_and_search(name):
     A, B, nA, nB = next((c[1], c[2], c[3], c[4]) for c in cases() if c[0] == name)
     m = len(A[0]) - 1
     xA, xB, d, Cp, Ci, Cx = oracle_btda(A, B, m, nA, nB)
-    outs = [refill(A, B, nA, nB, xA, xB, d, mode, wide=name.startswith("wide-blocks") or None)
+    outs = [refill(A, B, nA, nB, xA, xB, d, mode, wide=name.startswith("wide") or None)
             for mode in (0, 1)]
     for cp, ci, x in outs:
         assert np.array_equal(cp, Cp.astype(np.int32)) and np.array_equal(ci, Ci), name
