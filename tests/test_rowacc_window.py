@@ -143,3 +143,32 @@ def test_kron_blocks_through_windows():
         finally:
             lib.sdl_symbolic_window(prev[0])
             lib.sdl_rowacc_window(prev[1])
+
+
+@pytest.mark.parametrize("name", ["band32", "wide-ladder"])
+def test_batched_refill_row_accumulators(name):
+    """sdl_btda_refill_batched on a row-accumulator plan (windows on): K = 3
+    d vectors in one call, each refill bit-exact against the oracle's values
+    for its d."""
+    A, B, nA, nB = next((c[1], c[2], c[3], c[4]) for c in cases() if c[0] == name)
+    m = len(A[0]) - 1
+    lib = load()
+    pl = Plan(A, B, nA, nB)
+    assert not lib.sdl_btda_plan_is_pull(pl.h)
+    nnz = pl.nnz()
+    K = 3
+    ds = [np.linspace(0.5 + r, 3.0 + r, m) for r in range(K)]
+    ref = [oracle_btda(A, B, m, nA, nB, d) for d in ds]
+    xA, xB = ref[0][0], ref[0][1]
+    dA, dB = DeviceArray.from_numpy(xA), DeviceArray.from_numpy(xB)
+    dD = DeviceArray.from_numpy(np.concatenate(ds))
+    ldc = (nnz + 63) // 64 * 64
+    out = DeviceArray(K * ldc, np.float64)
+    check(lib.sdl_btda_refill_batched(pl.h, dA.ptr, dB.ptr, dD.ptr, m, K, out.ptr, ldc, None),
+          "batched refill")
+    lib.sdl_synchronize()
+    got = out.numpy()
+    for r in range(K):
+        Cx = ref[r][5]
+        assert np.array_equal(got[r * ldc:r * ldc + nnz].view(np.int64), Cx.view(np.int64)), r
+    pl.close()
